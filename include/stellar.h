@@ -1,0 +1,197 @@
+/*
+ * stellar.h -- C ABI of libstellar_hip, the MI355X (gfx950) Barnes-Hut step.
+ *
+ * The library replaces the launch-wrapper layer of pentilm/StellarSolver
+ * (Particle_cuda.cuh:5-13, Particle_cuda.cu:8-58): nine void functions over
+ * caller-owned raw device pointers, all on the legacy default stream, with no
+ * error reporting.  Here a context owns the device buffers and one HIP
+ * stream; every entry point returns a status code and never aborts or waits
+ * on stdin (the reference's gpuErrchk blocks on getchar(),
+ * BarnesHutParticleSystem.cpp:14-22).
+ *
+ * Threading: calls on one context must be serialised by the caller (as the
+ * reference's single host thread does).  Different contexts are independent.
+ *
+ * Stage order per step (BarnesHutParticleSystem.cpp:308-315):
+ *   stellar_bbox -> stellar_build -> stellar_com -> stellar_sort
+ *   -> stellar_forces -> stellar_update      (stellar_step runs all six)
+ * A stage called out of order returns STELLAR_ERR_STATE.
+ *
+ * Body arrays crossing the ABI are host pointers in body-id order.
+ */
+#ifndef STELLAR_H
+#define STELLAR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define STELLAR_OK 0
+#define STELLAR_ERR_ARG (-1)      /* bad argument or size                    */
+#define STELLAR_ERR_OOM (-2)      /* device or host allocation failed        */
+#define STELLAR_ERR_POOL (-3)     /* more cells than node_capacity            */
+#define STELLAR_ERR_STACK (-4)    /* walk stack overflow                      */
+#define STELLAR_ERR_DEPTH (-5)    /* too many bodies share one 32-level cell  */
+#define STELLAR_ERR_DEVICE (-6)   /* HIP runtime error                        */
+#define STELLAR_ERR_STATE (-7)    /* stage called out of order                */
+#define STELLAR_ERR_BUCKET (-8)   /* >4 bodies share a deepest-level cell     */
+
+/* Deepest cell level: bodies that share all 64 slot codes (coincident or
+ * closer than FP32 midpoints can separate) are kept, up to four, in one
+ * level-64 cell by rank instead of splitting forever. */
+#define STELLAR_MAX_LEVEL 64
+
+/* Force-path precision. */
+#define STELLAR_PRECISION_FP32 0  /* plain FP32 walk                         */
+#define STELLAR_PRECISION_MIXED 1 /* FP16-packed far field, FP32 near field  */
+
+typedef struct stellar_params {
+  /* Opening test: a cell of x-width w is accepted when every body of the
+   * walk group has w*w/theta <= dx*dx+dy*dy+eps2 (kernels.cu:376,395,406;
+   * the reference hard-codes theta=0.5, kernels.cu:9). */
+  float theta;
+  /* Softening squared (reference: eps2=0.025, kernels.cu:8). */
+  float eps2;
+  /* dt and velocity dampening d of v+=dt*a; x+=d*dt*v
+   * (SimulationParameters.h:23,25; kernels.cu:522-537). */
+  float timestep;
+  float dampening;
+  /* Kept for the interface; like the reference the step does not use it
+   * (kernels.cu:340 receives g and ignores it). */
+  float gravity;
+  /* Walk-group size: 64 (one wavefront, the native choice) or 32 (the
+   * reference's warp; two groups per wavefront). */
+  int32_t group;
+  int32_t precision; /* STELLAR_PRECISION_* */
+  int32_t flags;     /* STELLAR_FLAG_* */
+} stellar_params;
+
+/* Record per-stage HIP event timings on every step (small overhead). */
+#define STELLAR_FLAG_STAGE_TIMING 1
+
+typedef struct stellar_ctx stellar_ctx;
+
+/* Defaults matching the reference: theta 0.5, eps2 0.025, dt 0.001,
+ * dampening 1.0, gravity 1.0 (kernels.cu:8-9, main.cpp:94-97), group 64. */
+void stellar_default_params(stellar_params* p);
+
+/* Create a context for n bodies on the current HIP device.  node_capacity
+ * is the maximum number of cells; <= 0 selects 2n+12000, the reference's
+ * pool size (BarnesHutParticleSystem.cpp:33).  Replaces the 19 cudaMallocs
+ * of BarnesHutParticleSystem.cpp:54-83. */
+int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
+                   stellar_ctx** out);
+int stellar_destroy(stellar_ctx* ctx);
+int stellar_set_params(stellar_ctx* ctx, const stellar_params* p);
+int stellar_get_params(const stellar_ctx* ctx, stellar_params* p);
+int32_t stellar_num_bodies(const stellar_ctx* ctx);
+
+/* Host -> device state in body-id order (BarnesHutParticleSystem.cpp:365-371).
+ * Accelerations are zeroed. */
+int stellar_upload(stellar_ctx* ctx, const float* mass, const float* x,
+                   const float* y, const float* vx, const float* vy);
+/* Device -> host, body-id order.  Any pointer may be NULL. */
+int stellar_download(stellar_ctx* ctx, float* mass, float* x, float* y,
+                     float* vx, float* vy, float* ax, float* ay);
+/* Interleaved (x0,y0,x1,y1,...) body positions, body-id order: the first 2n
+ * floats of the reference's output buffer (kernels.cu:541-553). */
+int stellar_download_positions(stellar_ctx* ctx, float* out_xy);
+
+/* ---- the six stages (kernels.cu:62-537) -------------------------------- */
+int stellar_bbox(stellar_ctx* ctx);    /* compute_bounding_box_kernel      */
+int stellar_build(stellar_ctx* ctx);   /* build_tree_kernel                */
+int stellar_com(stellar_ctx* ctx);     /* centre_of_mass_kernel            */
+int stellar_sort(stellar_ctx* ctx);    /* sort_kernel                      */
+int stellar_forces(stellar_ctx* ctx);  /* compute_forces_kernel            */
+int stellar_update(stellar_ctx* ctx);  /* update_kernel (params dt, d)     */
+
+/* One whole step: the six stages plus the output buffer
+ * (BarnesHutParticleSystem::update, BarnesHutParticleSystem.cpp:301-347).
+ * Synchronous, like the reference (it waits on its stop event, :318). */
+int stellar_step(stellar_ctx* ctx);
+/* `steps` steps back to back with a single synchronisation at the end. */
+int stellar_run(stellar_ctx* ctx, int32_t steps);
+/* Wait for queued work and report any sticky device error. */
+int stellar_sync(stellar_ctx* ctx);
+
+/* ---- inspection --------------------------------------------------------- */
+/* Bounding box {left, right, bottom, top} of the last stellar_bbox. */
+int stellar_get_bbox(stellar_ctx* ctx, float* bbox4);
+/* Number of cells of the last stellar_build. */
+int stellar_num_cells(stellar_ctx* ctx, int64_t* ncells);
+
+/* Canonical tree of the last build: cells renumbered in DFS pre-order over
+ * slots 0..3 (the order sort_kernel, kernels.cu:292-336, lays bodies out).
+ * Child codes: -1 empty, 0..n-1 body id, n+c cell c.  root4 = the four root
+ * slots (the reference's child[0..3]).  Arrays sized by capacity (>= ncells;
+ * sorted: n).  com_* need stellar_com; any pointer may be NULL. */
+int stellar_export_tree(stellar_ctx* ctx, int32_t* root4, int32_t* child,
+                        int32_t* count, int32_t* start, int32_t* level,
+                        int32_t* sorted, float* com_x, float* com_y,
+                        float* com_m, int64_t capacity, int64_t* ncells);
+
+/* Per-body interaction counts and accelerations of the last stellar_forces,
+ * body-id order.  Either pointer may be NULL. */
+int stellar_export_forces(stellar_ctx* ctx, float* ax, float* ay,
+                          int32_t* interactions);
+
+/* Sum of interactions of the last walk (exact, 64-bit). */
+int stellar_last_interactions(stellar_ctx* ctx, int64_t* total);
+
+/* Per-stage device milliseconds of the last step when
+ * STELLAR_FLAG_STAGE_TIMING is set: [bbox, build, com, sort, forces,
+ * update, output]. */
+int stellar_stage_times(stellar_ctx* ctx, float* ms7);
+
+/* Message for the last non-OK status on this context. */
+const char* stellar_last_error(const stellar_ctx* ctx);
+const char* stellar_status_string(int status);
+
+/* ---- FP64 diagnostics on the device ------------------------------------ */
+/* Direct-sum accelerations in FP64 for the listed body ids (all if targets
+ * is NULL, then ntargets must be n), softening eps2 as given. */
+int stellar_direct_fp64(stellar_ctx* ctx, int32_t ntargets,
+                        const int32_t* targets, double eps2, double* ax,
+                        double* ay);
+/* Kinetic and Plummer-softened potential energy, linear momentum. */
+int stellar_energy_fp64(stellar_ctx* ctx, double eps2, double* kinetic,
+                        double* potential, double* px, double* py);
+
+/* ---- initial conditions (host) ------------------------------------------ */
+#define STELLAR_IC_DISK 0          /* central mass + uniform-radius disk     */
+#define STELLAR_IC_PLUMMER 1       /* 2D-projected Plummer sphere            */
+#define STELLAR_IC_COLLIDING 2     /* two disks on a collision course        */
+#define STELLAR_IC_REF_DISK 10     /* BarnesHutParticleSystem.cpp:439-483    */
+#define STELLAR_IC_REF_PLUMMER 11  /* BarnesHutParticleSystem.cpp:390-428    */
+#define STELLAR_IC_REF_COLLIDING 12 /* BarnesHutParticleSystem.cpp:494-560   */
+
+typedef struct stellar_ic_params {
+  int32_t model;       /* STELLAR_IC_* */
+  int32_t pad;
+  uint64_t seed;
+  double gravity;      /* G of the velocity set-up                           */
+  double central_mass; /* disk: mass of the central body                     */
+  double rmin, rmax;   /* disk: radii ~ U[rmin, rmax]                        */
+  double jitter;       /* disk: N(0, jitter) added to each velocity component */
+  double plummer_a;    /* plummer: scale radius a                            */
+  double plummer_rcut; /* plummer: truncation radius in units of a           */
+  double offset_x;     /* colliding: disks centred at (+-offset_x, 0)        */
+  double bulk_vx, bulk_vy; /* colliding: the +x disk moves with (bulk_vx,
+                              bulk_vy), the other with the negative          */
+} stellar_ic_params;
+
+/* Fill the parameters of a named workload: 0 disk N=4096 (M=1000,
+ * r~U[2,100]); 1 Plummer a=1; 2 colliding disks (M=1e4, r~U[2,200],
+ * +-300, -+5/+-1); 4 deep Plummer a=0.01.  Seeds 0,1,2,4. */
+int stellar_ic_defaults(int32_t workload, stellar_ic_params* p);
+/* Generate n bodies (host arrays, body-id order). */
+int stellar_generate_ic(const stellar_ic_params* p, int32_t n, float* mass,
+                        float* x, float* y, float* vx, float* vy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,810 @@
+/*
+ * oracle.c -- CPU restatement of the reference Barnes-Hut step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for libstellar_hip, loaded by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg.  Never part of the
+ * product path.
+ *
+ * Built with -ffp-contract=off so that every FP32 expression is evaluated as
+ * written (explicit fmaf where the reference's nvcc build contracts).
+ *
+ * Parity pins (see tests/test_oracle.py):
+ *   or_bbox            closed form min/max with the origin; reference debug.cpp
+ *                      BBOX check via oracle/_ref.
+ *   or_build_insert    hand-computed known answers; agreement with the
+ *                      independent path-key construction (or_build_keys);
+ *                      the reference's own debug.cpp checks via oracle/_ref.
+ *   or_com_ref/_exact  closed form sum(m x)/sum(m) over each cell's body range.
+ *   or_sort_ref        permutation + equality with key argsort.
+ *   or_walk            theta->0 equals the direct sum; 2-body closed form;
+ *                      symmetric ring; recursive restatement with exact
+ *                      interaction counts.
+ *   or_direct_fp64     2-body and ring closed forms.
+ *   or_update          closed form.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+static int nthreads(int32_t t) { return t > 0 ? t : or_num_threads(); }
+
+/* ------------------------------------------------------------------------ */
+/* Box subdivision: kernels.cu:144-157 (root), :165-178 (descent),
+ * :206-211 (old body in a split), :229-242 (new body in a split).
+ * mid = 0.5*(l+r): the FP32 sum l+r promoted to double by the double literal
+ * 0.5; the comparison x < mid happens in double; the new bound is the double
+ * mid rounded back to FP32.  Slot code: +1 if x < mid (west), +2 if y < mid
+ * (south); ties go to the east/north side. */
+static inline int subdivide(float px, float py, float* l, float* r, float* b,
+                            float* t) {
+  double mx = 0.5 * (double)(*l + *r);
+  double my = 0.5 * (double)(*t + *b);
+  int c = 0;
+  if ((double)px < mx) {
+    c += 1;
+    *r = (float)mx;
+  } else {
+    *l = (float)mx;
+  }
+  if ((double)py < my) {
+    c += 2;
+    *t = (float)my;
+  } else {
+    *b = (float)my;
+  }
+  return c;
+}
+
+static inline int slot_of(float px, float py, float l, float r, float b,
+                          float t) {
+  double mx = 0.5 * (double)(l + r);
+  double my = 0.5 * (double)(t + b);
+  return ((double)px < mx ? 1 : 0) + ((double)py < my ? 2 : 0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* compute_bounding_box_kernel (kernels.cu:62-114) after reset_arrays_kernel
+ * seeds left=right=bottom=top=0 (kernels.cu:51-58): the box always holds the
+ * origin.  A negative zero never lowers a bound below +0 (x < 0 is false). */
+int or_bbox(int32_t n, const float* x, const float* y, float* bbox) {
+  float l = 0.f, r = 0.f, b = 0.f, t = 0.f;
+  for (int32_t i = 0; i < n; ++i) {
+    if (x[i] < l) l = x[i];
+    if (x[i] > r) r = x[i];
+    if (y[i] < b) b = y[i];
+    if (y[i] > t) t = y[i];
+  }
+  bbox[0] = l;
+  bbox[1] = r;
+  bbox[2] = b;
+  bbox[3] = t;
+  return OR_OK;
+}
+
+/* Path key of one body: the slot codes along its own descent, which depend
+ * only on the body and the box (kernels.cu:144-178). */
+void or_path_keys(int32_t n, const float* x, const float* y, const float* bbox,
+                  uint64_t* k1, uint64_t* k2) {
+  for (int32_t i = 0; i < n; ++i) {
+    float l = bbox[0], r = bbox[1], b = bbox[2], t = bbox[3];
+    uint64_t a = 0, c = 0;
+    for (int lev = 1; lev <= 32; ++lev)
+      a = (a << 2) | (uint64_t)subdivide(x[i], y[i], &l, &r, &b, &t);
+    for (int lev = 33; lev <= 64; ++lev)
+      c = (c << 2) | (uint64_t)subdivide(x[i], y[i], &l, &r, &b, &t);
+    k1[i] = a;
+    k2[i] = c;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* build_tree_kernel (kernels.cu:117-271) with bodies inserted one at a time
+ * in id order, which is one legal interleaving of the concurrent kernel.
+ * Cell sums follow the kernel: every cell passed on the descent gets
+ * m*x, m*y, m, 1 added (:180-183); a split chain adds the old body then the
+ * new one to each new cell (:221-224, :247-250).  Pool: cells take ids
+ * n, n+1, ... (:199) and must stay below m.  The reference splits forever
+ * when two bodies share every slot code; here a cell at OR_MAX_LEVEL is an
+ * error instead (the path-key build stores such bodies as a bucket). */
+int or_build_insert(int32_t n, int32_t m, const float* x, const float* y,
+                    const float* mass, const float* bbox, int32_t* child,
+                    int32_t* count, int32_t* start, float* cx, float* cy,
+                    float* cm, int32_t* ncells) {
+  if (n < 0 || m < n) return OR_ERR_ARG;
+  /* reset_arrays_kernel (kernels.cu:25-59) */
+  for (int64_t i = 0; i < 4 * (int64_t)m; ++i) child[i] = -1;
+  for (int32_t i = 0; i < m; ++i) {
+    count[i] = i < n ? 1 : 0;
+    start[i] = -1;
+    if (i >= n) cx[i] = cy[i] = cm[i] = 0.f;
+  }
+  int32_t index = n;
+  for (int32_t i = 0; i < n; ++i) {
+    const float px = x[i], py = y[i], pm = mass[i];
+    float l = bbox[0], r = bbox[1], b = bbox[2], t = bbox[3];
+    int32_t temp = 0;
+    int lev = 1; /* level of the box (l,r,b,t) */
+    int32_t path = subdivide(px, py, &l, &r, &b, &t);
+    int32_t ci = child[4 * temp + path];
+    while (ci >= n) {
+      temp = ci;
+      path = subdivide(px, py, &l, &r, &b, &t);
+      ++lev;
+      cx[temp] += pm * px;
+      cy[temp] += pm * py;
+      cm[temp] += pm;
+      count[temp] += 1;
+      ci = child[4 * temp + path];
+    }
+    if (ci == -1) {
+      child[4 * temp + path] = i;
+      continue;
+    }
+    /* split: ci is a body sharing this slot */
+    const int32_t locked = 4 * temp + path;
+    int32_t patch = INT32_MAX;
+    while (ci >= 0 && ci < n) {
+      if (lev >= OR_MAX_LEVEL) return OR_ERR_DEPTH;
+      if (index >= m) return OR_ERR_POOL;
+      const int32_t cell = index++;
+      if (cell < patch) patch = cell;
+      if (patch != cell) child[4 * temp + path] = cell;
+      const int oc = slot_of(x[ci], y[ci], l, r, b, t);
+      cx[cell] += mass[ci] * x[ci];
+      cy[cell] += mass[ci] * y[ci];
+      cm[cell] += mass[ci];
+      count[cell] += count[ci];
+      child[4 * cell + oc] = ci;
+      start[cell] = -1;
+      temp = cell;
+      path = subdivide(px, py, &l, &r, &b, &t);
+      ++lev;
+      cx[cell] += pm * px;
+      cy[cell] += pm * py;
+      cm[cell] += pm;
+      count[cell] += count[i];
+      ci = child[4 * temp + path];
+    }
+    child[4 * temp + path] = i;
+    child[locked] = patch;
+  }
+  *ncells = index - n;
+  return OR_OK;
+}
+
+/* centre_of_mass_kernel (kernels.cu:275-288) */
+void or_com_ref(int32_t n, int32_t ncells, float* cx, float* cy,
+                const float* cm) {
+  for (int32_t c = n; c < n + ncells; ++c) {
+    cx[c] /= cm[c];
+    cy[c] /= cm[c];
+  }
+}
+
+/* sort_kernel (kernels.cu:292-336), run in cell-id order: a parent cell
+ * always has a smaller id than its children in a sequential insertion, so
+ * start[parent] is known when a child is reached (the kernel spin-waits for
+ * it instead). */
+void or_sort_ref(int32_t n, int32_t ncells, const int32_t* child,
+                 const int32_t* count, int32_t* start, int32_t* sorted) {
+  int32_t s = 0;
+  for (int k = 0; k < 4; ++k) {
+    const int32_t node = child[k];
+    if (node >= n) {
+      start[node] = s;
+      s += count[node];
+    } else if (node >= 0) {
+      sorted[s++] = node;
+    }
+  }
+  for (int32_t cell = n; cell < n + ncells; ++cell) {
+    s = start[cell];
+    for (int k = 0; k < 4; ++k) {
+      const int32_t node = child[4 * cell + k];
+      if (node >= n) {
+        start[node] = s;
+        s += count[node];
+      } else if (node >= 0) {
+        sorted[s++] = node;
+      }
+    }
+  }
+}
+
+/* DFS pre-order renumbering over slots 0..3, root slots first. */
+int or_canonicalize(int32_t n, int32_t ncells, const int32_t* child,
+                    const int32_t* count, const int32_t* start,
+                    const float* cx, const float* cy, const float* cm,
+                    int32_t* root, int32_t* cchild, int32_t* ccount,
+                    int32_t* cstart, int32_t* clevel, float* ccx, float* ccy,
+                    float* ccm) {
+  int32_t* newid = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ncells + 1));
+  int32_t* stk = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ncells + 8));
+  int32_t* stl = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ncells + 8));
+  if (!newid || !stk || !stl) {
+    free(newid);
+    free(stk);
+    free(stl);
+    return OR_ERR_ARG;
+  }
+  int32_t sp = 0, next = 0;
+  for (int k = 3; k >= 0; --k)
+    if (child[k] >= n) {
+      stk[sp] = child[k];
+      stl[sp++] = 1;
+    }
+  while (sp > 0) {
+    const int32_t node = stk[--sp];
+    const int32_t lev = stl[sp];
+    const int32_t c = next++;
+    newid[node - n] = c;
+    clevel[c] = lev;
+    ccount[c] = count[node];
+    cstart[c] = start[node];
+    if (ccx) {
+      ccx[c] = cx[node];
+      ccy[c] = cy[node];
+      ccm[c] = cm[node];
+    }
+    for (int k = 3; k >= 0; --k) {
+      const int32_t ch = child[4 * node + k];
+      if (ch >= n) {
+        stk[sp] = ch;
+        stl[sp++] = lev + 1;
+      }
+    }
+  }
+  int rc = next == ncells ? OR_OK : OR_ERR_ARG;
+  for (int k = 0; k < 4; ++k) {
+    const int32_t ch = child[k];
+    root[k] = ch >= n ? n + newid[ch - n] : ch;
+  }
+  for (int32_t old = 0; old < ncells && rc == OR_OK; ++old) {
+    const int32_t c = newid[old];
+    for (int k = 0; k < 4; ++k) {
+      const int32_t ch = child[4 * (n + old) + k];
+      cchild[4 * c + k] = ch >= n ? n + newid[ch - n] : ch;
+    }
+  }
+  free(newid);
+  free(stk);
+  free(stl);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Path-key construction: a cell at level L exists exactly when two or more
+ * bodies share their first L slot codes (the PR-quadtree that the insertion
+ * kernel grows).  Bodies are ordered by (k1, k2, id); every cell is the
+ * contiguous run of bodies sharing its prefix, split into four sub-runs by
+ * the next code.  Cells are numbered in DFS pre-order as they are created. */
+typedef struct keyctx {
+  int32_t n;
+  const uint64_t* k1;
+  const uint64_t* k2;
+  const int32_t* sorted;
+  int32_t* child;
+  int32_t* count;
+  int32_t* start;
+  int32_t* level;
+  int64_t capacity;
+  int64_t next;
+  int err;
+} keyctx;
+
+static inline int digit_at(const keyctx* K, int32_t s, int lev) {
+  const int32_t b = K->sorted[s];
+  if (lev <= 32) return (int)((K->k1[b] >> (2 * (32 - lev))) & 3u);
+  return (int)((K->k2[b] >> (2 * (64 - lev))) & 3u);
+}
+
+/* Node code for the run [a,e] of bodies sharing their first lev codes. */
+static int32_t key_assign(keyctx* K, int32_t a, int32_t e, int lev) {
+  if (a == e) return K->sorted[a];
+  if (K->next >= K->capacity) {
+    K->err = OR_ERR_POOL;
+    return -1;
+  }
+  const int64_t c = K->next++;
+  K->level[c] = lev;
+  K->start[c] = a;
+  K->count[c] = e - a + 1;
+  int32_t* ch = K->child + 4 * c;
+  ch[0] = ch[1] = ch[2] = ch[3] = -1;
+  if (lev == OR_MAX_LEVEL) {
+    if (e - a + 1 > 4) {
+      K->err = OR_ERR_BUCKET;
+      return -1;
+    }
+    for (int32_t s = a; s <= e; ++s) ch[s - a] = K->sorted[s];
+    return K->n + (int32_t)c;
+  }
+  int32_t u = a;
+  while (u <= e && !K->err) {
+    const int d = digit_at(K, u, lev + 1);
+    int32_t v = u;
+    while (v + 1 <= e && digit_at(K, v + 1, lev + 1) == d) ++v;
+    ch[d] = key_assign(K, u, v, lev + 1);
+    u = v + 1;
+  }
+  return K->n + (int32_t)c;
+}
+
+static const uint64_t* g_k1;
+static const uint64_t* g_k2;
+static int cmp_key(const void* pa, const void* pb) {
+  const int32_t a = *(const int32_t*)pa, b = *(const int32_t*)pb;
+  if (g_k1[a] != g_k1[b]) return g_k1[a] < g_k1[b] ? -1 : 1;
+  if (g_k2[a] != g_k2[b]) return g_k2[a] < g_k2[b] ? -1 : 1;
+  return a < b ? -1 : (a > b);
+}
+
+int or_build_keys(int32_t n, const float* x, const float* y, const float* bbox,
+                  int64_t capacity, int32_t* sorted, int32_t* root,
+                  int32_t* child, int32_t* count, int32_t* start,
+                  int32_t* level, int64_t* ncells) {
+  if (n < 0) return OR_ERR_ARG;
+  uint64_t* k1 = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(n + 1));
+  uint64_t* k2 = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(n + 1));
+  if (!k1 || !k2) {
+    free(k1);
+    free(k2);
+    return OR_ERR_ARG;
+  }
+  or_path_keys(n, x, y, bbox, k1, k2);
+  for (int32_t i = 0; i < n; ++i) sorted[i] = i;
+  g_k1 = k1;
+  g_k2 = k2;
+  qsort(sorted, (size_t)n, sizeof(int32_t), cmp_key);
+  keyctx K = {n, k1, k2, sorted, child, count, start, level, capacity, 0, 0};
+  root[0] = root[1] = root[2] = root[3] = -1;
+  int32_t u = 0;
+  while (u < n && !K.err) {
+    const int d = digit_at(&K, u, 1);
+    int32_t v = u;
+    while (v + 1 < n && digit_at(&K, v + 1, 1) == d) ++v;
+    root[d] = key_assign(&K, u, v, 1);
+    u = v + 1;
+  }
+  *ncells = K.next;
+  free(k1);
+  free(k2);
+  return K.err;
+}
+
+/* Bottom-up FP64 sums in slot order; cells are visited from the highest
+ * pre-order id down so every child cell is complete before its parent. */
+void or_com_exact(int32_t n, int64_t ncells, const int32_t* child,
+                  const float* x, const float* y, const float* mass,
+                  float* ccx, float* ccy, float* ccm) {
+  double* S = (double*)malloc(sizeof(double) * 3 * (size_t)(ncells + 1));
+  for (int64_t c = ncells - 1; c >= 0; --c) {
+    double sm = 0.0, sx = 0.0, sy = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      const int32_t ch = child[4 * c + k];
+      if (ch < 0) continue;
+      if (ch < n) {
+        const double m = (double)mass[ch];
+        sm += m;
+        sx += m * (double)x[ch];
+        sy += m * (double)y[ch];
+      } else {
+        const double* q = S + 3 * (size_t)(ch - n);
+        sm += q[0];
+        sx += q[1];
+        sy += q[2];
+      }
+    }
+    S[3 * c + 0] = sm;
+    S[3 * c + 1] = sx;
+    S[3 * c + 2] = sy;
+    ccm[c] = (float)sm;
+    ccx[c] = sm != 0.0 ? (float)(sx / sm) : 0.f;
+    ccy[c] = sm != 0.0 ? (float)(sy / sm) : 0.f;
+  }
+  free(S);
+}
+
+/* ------------------------------------------------------------------------ */
+/* compute_forces_kernel (kernels.cu:340-435).  Bodies are taken in groups of
+ * G consecutive entries of sorted[] (the reference's warps of 32, aligned by
+ * the grid stride).  Each group walks one shared stack:
+ *   radius = 0.5*(right-left)   (x extent only, :349)
+ *   root cells pushed with depth = radius^2/theta (:370-388)
+ *   pop: dp = 0.25*depth (:394-395); for each child slot 0..3 (:397):
+ *     r = dx*dx + dy*dy + eps2 (:403-405)
+ *     body, or every group member has dp <= r (:406): accumulate
+ *        f = m*rsqrt(r)^3, acc += f*(dx,dy)  (:407-411)
+ *     else push (child, dp) (:413-419)
+ * A body sitting alone in a root slot is accumulated here; the reference
+ * pushes it and then walks its (empty) child slots, so it drops out. */
+#define OR_STACK 256
+
+static inline float depth0_of(const float* bbox, float theta) {
+  const float radius = (float)(0.5 * (double)(bbox[1] - bbox[0]));
+  const float rr = radius * radius;
+  return rr / theta;
+}
+
+static inline float rsqrt_ref(float r) { return (float)(1.0 / sqrt((double)r)); }
+
+int or_walk(int32_t n, const float* x, const float* y, const float* mass,
+            const float* bbox, const int32_t* sorted, const int32_t* root,
+            const int32_t* child, int64_t ncells, const float* ccx,
+            const float* ccy, const float* ccm, const or_walk_params* p,
+            float* ax, float* ay, int32_t* inter) {
+  (void)ncells;
+  const int32_t G = p->group;
+  if (G <= 0 || G > 1024) return OR_ERR_ARG;
+  const float eps2 = p->eps2;
+  const float depth0 = depth0_of(bbox, p->theta);
+  const int32_t ngroups = (n + G - 1) / G;
+  int err = OR_OK;
+#pragma omp parallel num_threads(nthreads(p->threads))
+  {
+    float* px = (float*)malloc(sizeof(float) * (size_t)G * 5);
+    float* py = px + G;
+    float* fx = py + G;
+    float* fy = fx + G;
+    float* rr = fy + G;
+    int32_t stk[OR_STACK];
+    float dep[OR_STACK];
+#pragma omp for schedule(dynamic, 8)
+    for (int32_t g = 0; g < ngroups; ++g) {
+      const int32_t s0 = g * G;
+      const int32_t gs = (n - s0) < G ? (n - s0) : G;
+      int32_t cnt = 0;
+      for (int32_t j = 0; j < gs; ++j) {
+        const int32_t b = sorted[s0 + j];
+        px[j] = x[b];
+        py[j] = y[b];
+        fx[j] = 0.f;
+        fy[j] = 0.f;
+      }
+      int sp = 0;
+      for (int k = 0; k < 4; ++k) {
+        const int32_t ch = root[k];
+        if (ch < 0) continue;
+        if (ch < n) {
+          for (int32_t j = 0; j < gs; ++j) {
+            const float dx = x[ch] - px[j], dy = y[ch] - py[j];
+            const float r = dx * dx + dy * dy + eps2;
+            const float rs = rsqrt_ref(r);
+            const float f = mass[ch] * rs * rs * rs;
+            fx[j] = fmaf(f, dx, fx[j]);
+            fy[j] = fmaf(f, dy, fy[j]);
+          }
+          ++cnt;
+        } else {
+          stk[sp] = ch;
+          dep[sp++] = depth0;
+        }
+      }
+      while (sp > 0) {
+        --sp;
+        const int32_t node = stk[sp];
+        const float dp = 0.25f * dep[sp];
+        for (int k = 0; k < 4; ++k) {
+          const int32_t ch = child[4 * (int64_t)(node - n) + k];
+          if (ch < 0) continue;
+          float cxv, cyv, cmv;
+          if (ch < n) {
+            cxv = x[ch];
+            cyv = y[ch];
+            cmv = mass[ch];
+          } else {
+            cxv = ccx[ch - n];
+            cyv = ccy[ch - n];
+            cmv = ccm[ch - n];
+          }
+          int all = 1;
+          for (int32_t j = 0; j < gs; ++j) {
+            const float dx = cxv - px[j], dy = cyv - py[j];
+            rr[j] = dx * dx + dy * dy + eps2;
+            if (!(dp <= rr[j])) all = 0;
+          }
+          if (ch < n || all) {
+            for (int32_t j = 0; j < gs; ++j) {
+              const float dx = cxv - px[j], dy = cyv - py[j];
+              const float rs = rsqrt_ref(rr[j]);
+              const float f = cmv * rs * rs * rs;
+              fx[j] = fmaf(f, dx, fx[j]);
+              fy[j] = fmaf(f, dy, fy[j]);
+            }
+            ++cnt;
+          } else {
+            if (sp >= OR_STACK) {
+#pragma omp atomic write
+              err = OR_ERR_DEPTH;
+              sp = 0;
+              break;
+            }
+            stk[sp] = ch;
+            dep[sp++] = dp;
+          }
+        }
+      }
+      for (int32_t j = 0; j < gs; ++j) {
+        const int32_t b = sorted[s0 + j];
+        ax[b] = fx[j];
+        ay[b] = fy[j];
+        if (inter) inter[b] = cnt;
+      }
+    }
+    free(px);
+  }
+  return err;
+}
+
+/* Same acceptance rule stated recursively, accumulated in FP64. */
+typedef struct recctx {
+  int32_t n, gs;
+  const float *x, *y, *mass, *ccx, *ccy, *ccm;
+  const int32_t* child;
+  float eps2;
+  const float* px;
+  const float* py;
+  double* fx;
+  double* fy;
+  int32_t cnt;
+} recctx;
+
+static void rec_acc(recctx* R, float cxv, float cyv, float cmv) {
+  for (int32_t j = 0; j < R->gs; ++j) {
+    const float dx = cxv - R->px[j], dy = cyv - R->py[j];
+    const float r = dx * dx + dy * dy + R->eps2;
+    const double rs = 1.0 / sqrt((double)r);
+    R->fx[j] += (double)cmv * rs * rs * rs * (double)dx;
+    R->fy[j] += (double)cmv * rs * rs * rs * (double)dy;
+  }
+  R->cnt++;
+}
+
+static void rec_open(recctx* R, int32_t node, float depth) {
+  const float dp = 0.25f * depth;
+  for (int k = 0; k < 4; ++k) {
+    const int32_t ch = R->child[4 * (int64_t)(node - R->n) + k];
+    if (ch < 0) continue;
+    if (ch < R->n) {
+      rec_acc(R, R->x[ch], R->y[ch], R->mass[ch]);
+      continue;
+    }
+    const int32_t c = ch - R->n;
+    int all = 1;
+    for (int32_t j = 0; j < R->gs && all; ++j) {
+      const float dx = R->ccx[c] - R->px[j], dy = R->ccy[c] - R->py[j];
+      const float r = dx * dx + dy * dy + R->eps2;
+      if (!(dp <= r)) all = 0;
+    }
+    if (all)
+      rec_acc(R, R->ccx[c], R->ccy[c], R->ccm[c]);
+    else
+      rec_open(R, ch, dp);
+  }
+}
+
+int or_walk_recursive(int32_t n, const float* x, const float* y,
+                      const float* mass, const float* bbox,
+                      const int32_t* sorted, const int32_t* root,
+                      const int32_t* child, int64_t ncells, const float* ccx,
+                      const float* ccy, const float* ccm,
+                      const or_walk_params* p, double* ax, double* ay,
+                      int32_t* inter) {
+  (void)ncells;
+  const int32_t G = p->group;
+  if (G <= 0 || G > 1024) return OR_ERR_ARG;
+  const float depth0 = depth0_of(bbox, p->theta);
+  const int32_t ngroups = (n + G - 1) / G;
+#pragma omp parallel num_threads(nthreads(p->threads))
+  {
+    float* px = (float*)malloc(sizeof(float) * (size_t)G * 2);
+    double* f = (double*)malloc(sizeof(double) * (size_t)G * 2);
+#pragma omp for schedule(dynamic, 8)
+    for (int32_t g = 0; g < ngroups; ++g) {
+      const int32_t s0 = g * G;
+      const int32_t gs = (n - s0) < G ? (n - s0) : G;
+      for (int32_t j = 0; j < gs; ++j) {
+        px[j] = x[sorted[s0 + j]];
+        px[G + j] = y[sorted[s0 + j]];
+        f[j] = f[G + j] = 0.0;
+      }
+      recctx R = {n, gs, x, y, mass, ccx, ccy, ccm, child, p->eps2,
+                  px, px + G, f, f + G, 0};
+      for (int k = 0; k < 4; ++k) {
+        const int32_t ch = root[k];
+        if (ch < 0) continue;
+        if (ch < n)
+          rec_acc(&R, x[ch], y[ch], mass[ch]);
+        else
+          rec_open(&R, ch, depth0);
+      }
+      for (int32_t j = 0; j < gs; ++j) {
+        const int32_t b = sorted[s0 + j];
+        ax[b] = f[j];
+        ay[b] = f[G + j];
+        if (inter) inter[b] = R.cnt;
+      }
+    }
+    free(px);
+    free(f);
+  }
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+void or_direct_fp64(int32_t n, const float* x, const float* y,
+                    const float* mass, double eps2, int32_t ntargets,
+                    const int32_t* targets, double* ax, double* ay,
+                    int32_t threads) {
+  const int32_t nt = targets ? ntargets : n;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads(threads))
+  for (int32_t q = 0; q < nt; ++q) {
+    const int32_t i = targets ? targets[q] : q;
+    const double xi = x[i], yi = y[i];
+    double sx = 0.0, sy = 0.0;
+    for (int32_t j = 0; j < n; ++j) {
+      const double dx = (double)x[j] - xi, dy = (double)y[j] - yi;
+      const double r2 = dx * dx + dy * dy + eps2;
+      const double inv = 1.0 / (r2 * sqrt(r2));
+      sx += (double)mass[j] * dx * inv;
+      sy += (double)mass[j] * dy * inv;
+    }
+    ax[q] = sx;
+    ay[q] = sy;
+  }
+}
+
+/* update_kernel (kernels.cu:522-537): v += dt*a, then x += d*dt*v; nvcc
+ * contracts both to fused multiply-adds with d*dt rounded first. */
+void or_update(int32_t n, float* x, float* y, float* vx, float* vy,
+               const float* ax, const float* ay, float dt, float d) {
+  const float ddt = d * dt;
+  for (int32_t i = 0; i < n; ++i) {
+    vx[i] = fmaf(dt, ax[i], vx[i]);
+    vy[i] = fmaf(dt, ay[i], vy[i]);
+    x[i] = fmaf(ddt, vx[i], x[i]);
+    y[i] = fmaf(ddt, vy[i], y[i]);
+  }
+}
+
+/* debug.cpp:6-178 DEBUG_RUN_TESTS, as a failure bitmask. */
+int32_t or_invariants(int32_t n, int32_t m, const int32_t* count,
+                      const int32_t* sorted, const int32_t* child,
+                      const float* bbox) {
+  int32_t fail = 0;
+  for (int32_t i = n; i < m; ++i) { /* COUNT 1, :23-41 */
+    int64_t c = count[i];
+    for (int j = 0; j < 4; ++j) {
+      const int32_t ch = child[4 * (int64_t)i + j];
+      if (ch != -1 && ch >= 0 && ch < m) c -= count[ch];
+    }
+    if (c != 0) fail |= 1;
+  }
+  int64_t sum = 0;
+  for (int64_t j = 0; j < 4 * (int64_t)m; ++j) /* CHILD 1, :50-61 */
+    if (child[j] >= 0 && child[j] < n) sum += child[j];
+  if (sum != (int64_t)n * (int64_t)(n - 1) / 2) fail |= 2;
+  for (int64_t j = 0; j < 4 * (int64_t)m; ++j) { /* CHILD 2/3, :65-91 */
+    if (child[j] >= m) fail |= 4;
+    if (child[j] < -1) fail |= 8;
+  }
+  int32_t mn = 2 * n; /* CHILD 4, :95-112 */
+  for (int k = 0; k < 4; ++k)
+    if (child[k] < mn) mn = child[k];
+  for (int64_t j = 4; j < 4 * (int64_t)(mn - 1); ++j)
+    if (child[j] != -1) fail |= 16;
+  sum = 0; /* SORTED 1, :121-130 */
+  for (int32_t i = 0; i < m; ++i) sum += sorted[i];
+  if (sum != (int64_t)n * (int64_t)(n - 1) / 2) fail |= 32;
+  char* seen = (char*)calloc((size_t)n + 1, 1); /* SORTED 2, :134-153 */
+  for (int32_t i = 0; i < n; ++i)
+    if (sorted[i] >= 0 && sorted[i] < n) seen[sorted[i]] = 1;
+  for (int32_t i = 0; i < n; ++i)
+    if (!seen[i]) fail |= 64;
+  free(seen);
+  if (bbox[0] > bbox[1] || bbox[2] > bbox[3]) fail |= 128; /* BBOX 1 */
+  return fail;
+}
+
+void or_energy(int32_t n, const float* x, const float* y, const float* vx,
+               const float* vy, const float* mass, double eps2,
+               int32_t threads, double* kinetic, double* potential,
+               double* px, double* py) {
+  double k = 0.0, u = 0.0, mx = 0.0, my = 0.0;
+  for (int32_t i = 0; i < n; ++i) {
+    const double m = mass[i], v2 = (double)vx[i] * vx[i] + (double)vy[i] * vy[i];
+    k += 0.5 * m * v2;
+    mx += m * vx[i];
+    my += m * vy[i];
+  }
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : u) \
+    num_threads(nthreads(threads))
+  for (int32_t i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int32_t j = i + 1; j < n; ++j) {
+      const double dx = (double)x[j] - x[i], dy = (double)y[j] - y[i];
+      s += (double)mass[j] / sqrt(dx * dx + dy * dy + eps2);
+    }
+    u -= (double)mass[i] * s;
+  }
+  *kinetic = k;
+  *potential = u;
+  *px = mx;
+  *py = my;
+}
+
+/* ------------------------------------------------------------------------ */
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* BarnesHutParticleSystem::update (BarnesHutParticleSystem.cpp:301-316):
+ * reset, bbox, build, CoM, sort, forces, update, in that order. */
+int or_step(int32_t n, float* x, float* y, float* vx, float* vy,
+            const float* mass, float* ax, float* ay, const or_step_params* p,
+            double* stage_seconds) {
+  const int32_t m = p->m > 0 ? p->m : 2 * n + 12000;
+  int32_t* child = (int32_t*)malloc(sizeof(int32_t) * 4 * (size_t)m);
+  int32_t* count = (int32_t*)malloc(sizeof(int32_t) * (size_t)m);
+  int32_t* start = (int32_t*)malloc(sizeof(int32_t) * (size_t)m);
+  int32_t* sorted = (int32_t*)calloc((size_t)m, sizeof(int32_t));
+  float* cx = (float*)malloc(sizeof(float) * (size_t)m);
+  float* cy = (float*)malloc(sizeof(float) * (size_t)m);
+  float* cm = (float*)malloc(sizeof(float) * (size_t)m);
+  int rc = OR_OK;
+  double t[7];
+  float bbox[4];
+  int32_t ncells = 0;
+  if (!child || !count || !start || !sorted || !cx || !cy || !cm) {
+    rc = OR_ERR_ARG;
+    goto out;
+  }
+  t[0] = now_s();
+  or_bbox(n, x, y, bbox);
+  t[1] = now_s();
+  rc = or_build_insert(n, m, x, y, mass, bbox, child, count, start, cx, cy, cm,
+                       &ncells);
+  if (rc) goto out;
+  t[2] = now_s();
+  or_com_ref(n, ncells, cx, cy, cm);
+  t[3] = now_s();
+  or_sort_ref(n, ncells, child, count, start, sorted);
+  t[4] = now_s();
+  {
+    /* the walk reads the pool layout: cells at n.. with CoM in cx/cy/cm */
+    or_walk_params wp = {p->theta, p->eps2, p->group, p->threads};
+    rc = or_walk(n, x, y, mass, bbox, sorted, child, child + 4 * (size_t)n,
+                 ncells, cx + n, cy + n, cm + n, &wp, ax, ay, NULL);
+    if (rc) goto out;
+  }
+  t[5] = now_s();
+  or_update(n, x, y, vx, vy, ax, ay, p->dt, p->dampening);
+  t[6] = now_s();
+  if (stage_seconds)
+    for (int k = 0; k < 6; ++k) stage_seconds[k] = t[k + 1] - t[k];
+out:
+  free(child);
+  free(count);
+  free(start);
+  free(sorted);
+  free(cx);
+  free(cy);
+  free(cm);
+  return rc;
+}

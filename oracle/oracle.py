@@ -1,0 +1,342 @@
+"""ctypes binding of liboracle.so, the CPU restatement of the reference step.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import io
+import os
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_DEBUG_PATH = HERE / "_ref" / "libref_debug.so"
+
+OR_OK = 0
+OR_ERR_ARG = -1
+OR_ERR_POOL = -3
+OR_ERR_DEPTH = -5
+OR_ERR_BUCKET = -8
+MAX_LEVEL = 64
+
+_F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int32)
+_D = C.POINTER(C.c_double)
+_U64 = C.POINTER(C.c_uint64)
+_I64 = C.POINTER(C.c_int64)
+
+
+class WalkParams(C.Structure):
+    _fields_ = [("theta", C.c_float), ("eps2", C.c_float), ("group", C.c_int32), ("threads", C.c_int32)]
+
+
+class StepParams(C.Structure):
+    _fields_ = [("theta", C.c_float), ("eps2", C.c_float), ("dt", C.c_float), ("dampening", C.c_float),
+                ("group", C.c_int32), ("threads", C.c_int32), ("m", C.c_int32)]
+
+
+_SIG = {
+    "or_bbox": (C.c_int, [C.c_int32, _F, _F, _F]),
+    "or_path_keys": (None, [C.c_int32, _F, _F, _F, _U64, _U64]),
+    "or_build_insert": (C.c_int, [C.c_int32, C.c_int32, _F, _F, _F, _F, _I, _I, _I, _F, _F, _F, _I]),
+    "or_com_ref": (None, [C.c_int32, C.c_int32, _F, _F, _F]),
+    "or_sort_ref": (None, [C.c_int32, C.c_int32, _I, _I, _I, _I]),
+    "or_canonicalize": (C.c_int, [C.c_int32, C.c_int32, _I, _I, _I, _F, _F, _F, _I, _I, _I, _I, _I, _F, _F, _F]),
+    "or_build_keys": (C.c_int, [C.c_int32, _F, _F, _F, C.c_int64, _I, _I, _I, _I, _I, _I, _I64]),
+    "or_com_exact": (None, [C.c_int32, C.c_int64, _I, _F, _F, _F, _F, _F, _F]),
+    "or_walk": (C.c_int, [C.c_int32, _F, _F, _F, _F, _I, _I, _I, C.c_int64, _F, _F, _F,
+                          C.POINTER(WalkParams), _F, _F, _I]),
+    "or_walk_recursive": (C.c_int, [C.c_int32, _F, _F, _F, _F, _I, _I, _I, C.c_int64, _F, _F, _F,
+                                    C.POINTER(WalkParams), _D, _D, _I]),
+    "or_direct_fp64": (None, [C.c_int32, _F, _F, _F, C.c_double, C.c_int32, _I, _D, _D, C.c_int32]),
+    "or_update": (None, [C.c_int32, _F, _F, _F, _F, _F, _F, C.c_float, C.c_float]),
+    "or_invariants": (C.c_int32, [C.c_int32, C.c_int32, _I, _I, _I, _F]),
+    "or_energy": (None, [C.c_int32, _F, _F, _F, _F, _F, C.c_double, C.c_int32, _D, _D, _D, _D]),
+    "or_step": (C.c_int, [C.c_int32, _F, _F, _F, _F, _F, _F, _F, C.POINTER(StepParams), _D]),
+    "or_num_threads": (C.c_int, []),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            subprocess.run(["make", "-C", str(HERE), "liboracle.so"], check=True,
+                           capture_output=True)
+        _lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIG.items():
+            fn = getattr(_lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+def _f(a):
+    return None if a is None else a.ctypes.data_as(_F)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_I)
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class OracleError(RuntimeError):
+    def __init__(self, rc, what):
+        super().__init__(f"oracle status {rc} in {what}")
+        self.status = rc
+
+
+def num_threads() -> int:
+    return int(lib().or_num_threads())
+
+
+def bbox(x, y) -> np.ndarray:
+    x, y = f32(x), f32(y)
+    b = np.zeros(4, np.float32)
+    lib().or_bbox(x.size, _f(x), _f(y), _f(b))
+    return b
+
+
+def path_keys(x, y, bb) -> tuple[np.ndarray, np.ndarray]:
+    x, y, bb = f32(x), f32(y), f32(bb)
+    k1 = np.zeros(x.size, np.uint64)
+    k2 = np.zeros(x.size, np.uint64)
+    lib().or_path_keys(x.size, _f(x), _f(y), _f(bb), k1.ctypes.data_as(_U64), k2.ctypes.data_as(_U64))
+    return k1, k2
+
+
+def build_insert(x, y, mass, bb, m: int | None = None) -> dict:
+    """Reference-layout tree from the sequential insertion restatement."""
+    x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
+    n = x.size
+    m = 2 * n + 12000 if m is None else int(m)
+    child = np.zeros(4 * m, np.int32)
+    count = np.zeros(m, np.int32)
+    start = np.zeros(m, np.int32)
+    cx = np.zeros(m, np.float32)
+    cy = np.zeros(m, np.float32)
+    cm = np.zeros(m, np.float32)
+    nc = C.c_int32()
+    rc = lib().or_build_insert(n, m, _f(x), _f(y), _f(mass), _f(bb), _i(child), _i(count), _i(start),
+                               _f(cx), _f(cy), _f(cm), C.byref(nc))
+    if rc:
+        raise OracleError(rc, "build_insert")
+    return {"n": n, "m": m, "child": child, "count": count, "start": start, "cx": cx, "cy": cy,
+            "cm": cm, "ncells": nc.value}
+
+
+def com_ref(t: dict) -> None:
+    lib().or_com_ref(t["n"], t["ncells"], _f(t["cx"]), _f(t["cy"]), _f(t["cm"]))
+
+
+def sort_ref(t: dict) -> None:
+    t["sorted"] = np.zeros(t["m"], np.int32)
+    lib().or_sort_ref(t["n"], t["ncells"], _i(t["child"]), _i(t["count"]), _i(t["start"]),
+                      _i(t["sorted"]))
+
+
+def canonicalize(t: dict) -> dict:
+    n, c = t["n"], t["ncells"]
+    cc = max(c, 1)
+    out = {"root": np.zeros(4, np.int32), "child": np.zeros(4 * cc, np.int32),
+           "count": np.zeros(cc, np.int32), "start": np.zeros(cc, np.int32),
+           "level": np.zeros(cc, np.int32), "com_x": np.zeros(cc, np.float32),
+           "com_y": np.zeros(cc, np.float32), "com_m": np.zeros(cc, np.float32)}
+    rc = lib().or_canonicalize(n, c, _i(t["child"]), _i(t["count"]), _i(t["start"]), _f(t["cx"]),
+                               _f(t["cy"]), _f(t["cm"]), _i(out["root"]), _i(out["child"]),
+                               _i(out["count"]), _i(out["start"]), _i(out["level"]),
+                               _f(out["com_x"]), _f(out["com_y"]), _f(out["com_m"]))
+    if rc:
+        raise OracleError(rc, "canonicalize")
+    for k in ("count", "start", "level", "com_x", "com_y", "com_m"):
+        out[k] = out[k][:c]
+    out["child"] = out["child"][: 4 * c]
+    out["sorted"] = t["sorted"][:n].copy() if "sorted" in t else None
+    out["ncells"] = c
+    return out
+
+
+def build_keys(x, y, bb, capacity: int | None = None) -> dict:
+    """Canonical tree from the independent path-key construction."""
+    x, y, bb = f32(x), f32(y), f32(bb)
+    n = x.size
+    cap = 2 * n + 12000 if capacity is None else int(capacity)
+    t = {"sorted": np.zeros(max(n, 1), np.int32), "root": np.zeros(4, np.int32),
+         "child": np.zeros(4 * max(cap, 1), np.int32), "count": np.zeros(max(cap, 1), np.int32),
+         "start": np.zeros(max(cap, 1), np.int32), "level": np.zeros(max(cap, 1), np.int32)}
+    nc = C.c_int64()
+    rc = lib().or_build_keys(n, _f(x), _f(y), _f(bb), cap, _i(t["sorted"]), _i(t["root"]),
+                             _i(t["child"]), _i(t["count"]), _i(t["start"]), _i(t["level"]),
+                             C.byref(nc))
+    if rc:
+        raise OracleError(rc, "build_keys")
+    c = nc.value
+    return {"root": t["root"], "child": t["child"][: 4 * c], "count": t["count"][:c],
+            "start": t["start"][:c], "level": t["level"][:c], "sorted": t["sorted"][:n],
+            "ncells": c}
+
+
+def com_exact(tree: dict, x, y, mass) -> None:
+    x, y, mass = f32(x), f32(y), f32(mass)
+    c = tree["ncells"]
+    for k in ("com_x", "com_y", "com_m"):
+        tree[k] = np.zeros(max(c, 1), np.float32)
+    lib().or_com_exact(x.size, c, _i(tree["child"]), _f(x), _f(y), _f(mass), _f(tree["com_x"]),
+                       _f(tree["com_y"]), _f(tree["com_m"]))
+    for k in ("com_x", "com_y", "com_m"):
+        tree[k] = tree[k][:c]
+
+
+def walk(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0):
+    """Group-vote walk (kernels.cu:340-435) over a canonical tree; by body id."""
+    x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
+    n = x.size
+    p = WalkParams(theta, eps2, group, threads)
+    ax = np.zeros(n, np.float32)
+    ay = np.zeros(n, np.float32)
+    it = np.zeros(n, np.int32)
+    ch = tree["child"] if tree["child"].size else np.zeros(4, np.int32)
+    cx = tree["com_x"] if tree["com_x"].size else np.zeros(1, np.float32)
+    cy = tree["com_y"] if tree["com_y"].size else np.zeros(1, np.float32)
+    cm = tree["com_m"] if tree["com_m"].size else np.zeros(1, np.float32)
+    rc = lib().or_walk(n, _f(x), _f(y), _f(mass), _f(bb), _i(tree["sorted"]), _i(tree["root"]),
+                       _i(ch), tree["ncells"], _f(cx), _f(cy), _f(cm), C.byref(p), _f(ax), _f(ay),
+                       _i(it))
+    if rc:
+        raise OracleError(rc, "walk")
+    return ax, ay, it
+
+
+def walk_recursive(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0):
+    x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
+    n = x.size
+    p = WalkParams(theta, eps2, group, threads)
+    ax = np.zeros(n, np.float64)
+    ay = np.zeros(n, np.float64)
+    it = np.zeros(n, np.int32)
+    ch = tree["child"] if tree["child"].size else np.zeros(4, np.int32)
+    cx = tree["com_x"] if tree["com_x"].size else np.zeros(1, np.float32)
+    cy = tree["com_y"] if tree["com_y"].size else np.zeros(1, np.float32)
+    cm = tree["com_m"] if tree["com_m"].size else np.zeros(1, np.float32)
+    rc = lib().or_walk_recursive(n, _f(x), _f(y), _f(mass), _f(bb), _i(tree["sorted"]),
+                                 _i(tree["root"]), _i(ch), tree["ncells"], _f(cx), _f(cy), _f(cm),
+                                 C.byref(p), _d(ax), _d(ay), _i(it))
+    if rc:
+        raise OracleError(rc, "walk_recursive")
+    return ax, ay, it
+
+
+def direct_fp64(x, y, mass, eps2: float, targets=None, threads: int = 0):
+    x, y, mass = f32(x), f32(y), f32(mass)
+    if targets is None:
+        nt, tp = x.size, None
+    else:
+        targets = np.ascontiguousarray(targets, dtype=np.int32)
+        nt, tp = targets.size, _i(targets)
+    ax = np.zeros(nt, np.float64)
+    ay = np.zeros(nt, np.float64)
+    lib().or_direct_fp64(x.size, _f(x), _f(y), _f(mass), float(eps2), nt, tp, _d(ax), _d(ay), threads)
+    return ax, ay
+
+
+def update(x, y, vx, vy, ax, ay, dt: float, d: float):
+    x, y, vx, vy = (f32(a).copy() for a in (x, y, vx, vy))
+    ax, ay = f32(ax), f32(ay)
+    lib().or_update(x.size, _f(x), _f(y), _f(vx), _f(vy), _f(ax), _f(ay), dt, d)
+    return x, y, vx, vy
+
+
+def invariants(n: int, m: int, count, sorted_, child, bb) -> int:
+    count, sorted_, child = (np.ascontiguousarray(a, dtype=np.int32) for a in (count, sorted_, child))
+    return int(lib().or_invariants(n, m, _i(count), _i(sorted_), _i(child), _f(f32(bb))))
+
+
+def energy(x, y, vx, vy, mass, eps2: float, threads: int = 0) -> dict:
+    v = [C.c_double() for _ in range(4)]
+    a = [f32(q) for q in (x, y, vx, vy, mass)]
+    lib().or_energy(a[0].size, *(_f(q) for q in a), float(eps2), threads, *(C.byref(q) for q in v))
+    return {"kinetic": v[0].value, "potential": v[1].value, "px": v[2].value, "py": v[3].value}
+
+
+def step(state: dict, theta: float, eps2: float, dt: float, d: float, group: int = 32,
+         threads: int = 0, m: int = 0):
+    """One reference step on the CPU, in place on state (x, y, vx, vy, mass)."""
+    for k in ("x", "y", "vx", "vy", "mass"):
+        state[k] = f32(state[k])
+    n = state["x"].size
+    ax = np.zeros(n, np.float32)
+    ay = np.zeros(n, np.float32)
+    secs = np.zeros(6, np.float64)
+    p = StepParams(theta, eps2, dt, d, group, threads, m)
+    rc = lib().or_step(n, _f(state["x"]), _f(state["y"]), _f(state["vx"]), _f(state["vy"]),
+                       _f(state["mass"]), _f(ax), _f(ay), C.byref(p), _d(secs))
+    if rc:
+        raise OracleError(rc, "step")
+    state["ax"], state["ay"] = ax, ay
+    return secs
+
+
+def to_reference_layout(tree: dict, n: int, m: int | None = None) -> dict:
+    """Canonical tree -> reference pool arrays (cells at n + c, root in child[0..3])."""
+    c = tree["ncells"]
+    m = 2 * n + 12000 if m is None else m
+    child = np.full(4 * m, -1, np.int32)
+    count = np.zeros(m, np.int32)
+    count[:n] = 1
+    if c:
+        child[4 * n: 4 * (n + c)] = tree["child"]
+        count[n: n + c] = tree["count"]
+    child[0:4] = tree["root"]
+    sorted_ = np.zeros(m, np.int32)
+    sorted_[:n] = tree["sorted"]
+    return {"child": child, "count": count, "sorted": sorted_, "m": m}
+
+
+# --------------------------------------------------------------------------
+# The reference's own debug.cpp (DEBUG_RUN_TESTS), compiled from its source
+# into _ref/ by `make -C oracle ref`; it prints PASS/FAIL lines.
+_REF_SYM = "_Z15DEBUG_RUN_TESTSPfS_S_PiS0_S0_S0_S_S_S_S_ii"
+
+
+def ref_debug_available() -> bool:
+    return REF_DEBUG_PATH.exists()
+
+
+def ref_debug_run(x, y, mass, count, start, sorted_, child, bb, n: int, m: int) -> list[str]:
+    """Run DEBUG_RUN_TESTS; returns its result lines (captured from fd 1)."""
+    lib_ref = C.CDLL(str(REF_DEBUG_PATH))
+    fn = getattr(lib_ref, _REF_SYM)
+    fn.restype = None
+    arrs_f = [np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, mass)]
+    arrs_i = [np.ascontiguousarray(a, dtype=np.int32) for a in (count, start, sorted_, child)]
+    b = [np.array([v], np.float32) for v in bb]
+    libc = C.CDLL(None)
+    with tempfile.TemporaryFile(mode="w+b") as tmp:
+        libc.fflush(None)
+        saved = os.dup(1)
+        os.dup2(tmp.fileno(), 1)
+        try:
+            fn(*(_f(a) for a in arrs_f), *(_i(a) for a in arrs_i), *(_f(q) for q in b),
+               C.c_int(n), C.c_int(m))
+            libc.fflush(None)
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+        tmp.seek(0)
+        text = tmp.read().decode()
+    return [ln.strip() for ln in io.StringIO(text) if "RESULT" in ln]

@@ -1,0 +1,270 @@
+// Per-body stages for gfx950: tree-order sort of the body state, the
+// leapfrog/dampening update, the output buffer, and FP64 diagnostics.
+#include "engine.h"
+
+namespace stellar {
+
+namespace {
+
+inline unsigned grid_for(int64_t items, int threads) {
+  return (unsigned)((items + threads - 1) / threads);
+}
+
+// sort_kernel (kernels.cu:292-336) lays bodies out in DFS order through
+// sorted[]; the build here already produced that order (perm), so this stage
+// moves the body state itself into it, making every later access coalesced.
+__global__ __launch_bounds__(256) void permute_kernel(
+    int32_t n, const uint32_t* __restrict__ perm, const float* __restrict__ x,
+    const float* __restrict__ y, const float* __restrict__ vx,
+    const float* __restrict__ vy, const float* __restrict__ m,
+    const int32_t* __restrict__ id, float* __restrict__ x2,
+    float* __restrict__ y2, float* __restrict__ vx2, float* __restrict__ vy2,
+    float* __restrict__ m2, int32_t* __restrict__ id2) {
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t p = perm[s];
+  x2[s] = x[p];
+  y2[s] = y[p];
+  vx2[s] = vx[p];
+  vy2[s] = vy[p];
+  m2[s] = m[p];
+  id2[s] = id[p];
+}
+
+// update_kernel (kernels.cu:522-537): v += dt*a; x += d*dt*v, both as fused
+// multiply-adds with d*dt rounded first, as nvcc contracts the reference.
+__global__ __launch_bounds__(256) void update_kernel(
+    int32_t n, float* __restrict__ x, float* __restrict__ y,
+    float* __restrict__ vx, float* __restrict__ vy,
+    const float* __restrict__ ax, const float* __restrict__ ay, float dt,
+    float d) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t n4 = n >> 2;
+  const float ddt = d * dt;
+  if (i < n4) {
+    float4* x4 = reinterpret_cast<float4*>(x);
+    float4* y4 = reinterpret_cast<float4*>(y);
+    float4* vx4 = reinterpret_cast<float4*>(vx);
+    float4* vy4 = reinterpret_cast<float4*>(vy);
+    const float4 a = reinterpret_cast<const float4*>(ax)[i];
+    const float4 b = reinterpret_cast<const float4*>(ay)[i];
+    float4 u = vx4[i], v = vy4[i], p = x4[i], q = y4[i];
+    u.x = __builtin_fmaf(dt, a.x, u.x);
+    u.y = __builtin_fmaf(dt, a.y, u.y);
+    u.z = __builtin_fmaf(dt, a.z, u.z);
+    u.w = __builtin_fmaf(dt, a.w, u.w);
+    v.x = __builtin_fmaf(dt, b.x, v.x);
+    v.y = __builtin_fmaf(dt, b.y, v.y);
+    v.z = __builtin_fmaf(dt, b.z, v.z);
+    v.w = __builtin_fmaf(dt, b.w, v.w);
+    p.x = __builtin_fmaf(ddt, u.x, p.x);
+    p.y = __builtin_fmaf(ddt, u.y, p.y);
+    p.z = __builtin_fmaf(ddt, u.z, p.z);
+    p.w = __builtin_fmaf(ddt, u.w, p.w);
+    q.x = __builtin_fmaf(ddt, v.x, q.x);
+    q.y = __builtin_fmaf(ddt, v.y, q.y);
+    q.z = __builtin_fmaf(ddt, v.z, q.z);
+    q.w = __builtin_fmaf(ddt, v.w, q.w);
+    vx4[i] = u;
+    vy4[i] = v;
+    x4[i] = p;
+    y4[i] = q;
+  }
+  const int32_t t = 4 * n4 + i;
+  if (i < 4 && t < n) {
+    vx[t] = __builtin_fmaf(dt, ax[t], vx[t]);
+    vy[t] = __builtin_fmaf(dt, ay[t], vy[t]);
+    x[t] = __builtin_fmaf(ddt, vx[t], x[t]);
+    y[t] = __builtin_fmaf(ddt, vy[t], y[t]);
+  }
+}
+
+// copy_kernel (kernels.cu:541-553): interleaved positions in body-id order,
+// followed by the cell centres of mass.
+__global__ __launch_bounds__(256) void output_kernel(
+    int32_t n, int64_t cap, const float* __restrict__ x,
+    const float* __restrict__ y, const int32_t* __restrict__ id,
+    const float4* __restrict__ ccom, const DevScalars* __restrict__ dev,
+    float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int32_t b = id[i];
+    reinterpret_cast<float2*>(out)[b] = make_float2(x[i], y[i]);
+    return;
+  }
+  const int64_t c = i - n;
+  if (c < dev->ncells && c < cap) {
+    const float4 q = ccom[c + 1];
+    reinterpret_cast<float2*>(out)[n + c] = make_float2(q.x, q.y);
+  }
+}
+
+__global__ void zero_counter_kernel(DevScalars* dev) { dev->interactions = 0; }
+
+// Positions scattered to body-id order (for diagnostics by id).
+__global__ __launch_bounds__(256) void by_id_kernel(
+    int32_t n, const float* __restrict__ x, const float* __restrict__ y,
+    const int32_t* __restrict__ id, float* __restrict__ xi,
+    float* __restrict__ yi) {
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  xi[id[s]] = x[s];
+  yi[id[s]] = y[s];
+}
+
+constexpr int kTile = 256;
+
+// FP64 direct sum for a list of targets over all sources (sources staged
+// through LDS in tiles).
+__global__ __launch_bounds__(kTile) void direct_fp64_kernel(
+    int32_t n, int32_t nt, const int32_t* __restrict__ targets,
+    const float* __restrict__ xid, const float* __restrict__ yid,
+    const float* __restrict__ x, const float* __restrict__ y,
+    const float* __restrict__ m, double eps2, double* __restrict__ ax,
+    double* __restrict__ ay) {
+  __shared__ float sx[kTile], sy[kTile], sm[kTile];
+  const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = q < nt;
+  const int32_t b = active ? (targets ? targets[q] : q) : 0;
+  const double xi = active ? (double)xid[b] : 0.0;
+  const double yi = active ? (double)yid[b] : 0.0;
+  double fx = 0.0, fy = 0.0;
+  for (int32_t base = 0; base < n; base += kTile) {
+    const int32_t j = base + threadIdx.x;
+    __syncthreads();
+    if (j < n) {
+      sx[threadIdx.x] = x[j];
+      sy[threadIdx.x] = y[j];
+      sm[threadIdx.x] = m[j];
+    } else {
+      sx[threadIdx.x] = 0.f;
+      sy[threadIdx.x] = 0.f;
+      sm[threadIdx.x] = 0.f;
+    }
+    __syncthreads();
+    const int32_t cnt = (n - base) < kTile ? (n - base) : kTile;
+    for (int32_t k = 0; k < cnt; ++k) {
+      const double dx = (double)sx[k] - xi, dy = (double)sy[k] - yi;
+      const double r2 = dx * dx + dy * dy + eps2;
+      const double inv = 1.0 / (r2 * sqrt(r2));
+      fx += (double)sm[k] * dx * inv;
+      fy += (double)sm[k] * dy * inv;
+    }
+  }
+  if (active) {
+    ax[q] = fx;
+    ay[q] = fy;
+  }
+}
+
+// Plummer-softened potential: per block, sum over its targets i of
+// m_i * sum_{j != i} m_j / sqrt(r^2 + eps2); kinetic energy and momentum of
+// the block's targets.  part[4*block + {0..3}] = pot, kin, px, py.
+__global__ __launch_bounds__(kTile) void energy_fp64_kernel(
+    int32_t n, const float* __restrict__ x, const float* __restrict__ y,
+    const float* __restrict__ vx, const float* __restrict__ vy,
+    const float* __restrict__ m, double eps2, double* __restrict__ part) {
+  __shared__ float sx[kTile], sy[kTile], sm[kTile];
+  __shared__ double red[4][kTile / 64];
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < n;
+  const double xi = active ? (double)x[i] : 0.0;
+  const double yi = active ? (double)y[i] : 0.0;
+  double pot = 0.0;
+  for (int32_t base = 0; base < n; base += kTile) {
+    const int32_t j = base + threadIdx.x;
+    __syncthreads();
+    sx[threadIdx.x] = j < n ? x[j] : 0.f;
+    sy[threadIdx.x] = j < n ? y[j] : 0.f;
+    sm[threadIdx.x] = j < n ? m[j] : 0.f;
+    __syncthreads();
+    const int32_t cnt = (n - base) < kTile ? (n - base) : kTile;
+    for (int32_t k = 0; k < cnt; ++k) {
+      if (base + k == i) continue;
+      const double dx = (double)sx[k] - xi, dy = (double)sy[k] - yi;
+      pot += (double)sm[k] / sqrt(dx * dx + dy * dy + eps2);
+    }
+  }
+  double vals[4] = {0.0, 0.0, 0.0, 0.0};
+  if (active) {
+    const double mi = m[i], u = vx[i], v = vy[i];
+    vals[0] = -0.5 * mi * pot;
+    vals[1] = 0.5 * mi * (u * u + v * v);
+    vals[2] = mi * u;
+    vals[3] = mi * v;
+  }
+  for (int k = 0; k < 4; ++k)
+    for (int off = 32; off > 0; off >>= 1) vals[k] += __shfl_xor(vals[k], off);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 4; ++k) red[k][w] = vals[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 4; ++k) {
+      double s = 0.0;
+      for (int q = 0; q < kTile / 64; ++q) s += red[k][q];
+      part[4 * (int64_t)blockIdx.x + k] = s;
+    }
+  }
+}
+
+}  // namespace
+
+void launch_sort(stellar_ctx* c) {
+  const int32_t n = c->n;
+  if (n == 0) return;
+  Buffers& B = c->b;
+  permute_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(
+      n, B.perm, B.x, B.y, B.vx, B.vy, B.m, B.id, B.x2, B.y2, B.vx2, B.vy2,
+      B.m2, B.id2);
+  std::swap(B.x, B.x2);
+  std::swap(B.y, B.y2);
+  std::swap(B.vx, B.vx2);
+  std::swap(B.vy, B.vy2);
+  std::swap(B.m, B.m2);
+  std::swap(B.id, B.id2);
+}
+
+void launch_update(stellar_ctx* c) {
+  const int32_t n = c->n;
+  if (n == 0) return;
+  Buffers& B = c->b;
+  const int32_t threads = (n >> 2) > 4 ? (n >> 2) : 4;
+  update_kernel<<<grid_for(threads, 256), 256, 0, c->stream>>>(
+      n, B.x, B.y, B.vx, B.vy, B.ax, B.ay, c->p.timestep, c->p.dampening);
+}
+
+void launch_output(stellar_ctx* c) {
+  const int32_t n = c->n;
+  if (n == 0) return;
+  Buffers& B = c->b;
+  output_kernel<<<grid_for(n + c->cap, 256), 256, 0, c->stream>>>(
+      n, c->cap, B.x, B.y, B.id, B.ccom, B.dev, B.out);
+}
+
+void launch_zero_counter(stellar_ctx* c) {
+  zero_counter_kernel<<<1, 1, 0, c->stream>>>(c->b.dev);
+}
+
+void launch_direct_fp64(stellar_ctx* c, int32_t nt, const int32_t* dtargets,
+                        double eps2, double* dax, double* day) {
+  Buffers& B = c->b;
+  const int32_t n = c->n;
+  // positions by id, staged in the sort's spare buffers
+  by_id_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, B.x, B.y, B.id,
+                                                         B.x2, B.y2);
+  direct_fp64_kernel<<<grid_for(nt, kTile), kTile, 0, c->stream>>>(
+      n, nt, dtargets, B.x2, B.y2, B.x, B.y, B.m, eps2, dax, day);
+}
+
+int energy_parts(stellar_ctx* c) { return (int)grid_for(c->n, kTile); }
+
+void launch_energy_fp64(stellar_ctx* c, double eps2, double* dpart,
+                        int32_t nparts) {
+  (void)nparts;
+  Buffers& B = c->b;
+  energy_fp64_kernel<<<grid_for(c->n, kTile), kTile, 0, c->stream>>>(
+      c->n, B.x, B.y, B.vx, B.vy, B.m, eps2, dpart);
+}
+
+}  // namespace stellar

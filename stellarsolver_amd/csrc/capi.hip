@@ -1,0 +1,653 @@
+// libstellar_hip C ABI: context, buffers, stage sequencing, status codes.
+//
+// Replaces the launch wrappers of Particle_cuda.cu:8-58 and the buffer
+// management of BarnesHutParticleSystem.cpp:28-84, 249-292.
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "engine.h"
+
+using namespace stellar;
+
+namespace {
+
+const char* status_text(int s) {
+  switch (s) {
+    case STELLAR_OK: return "ok";
+    case STELLAR_ERR_ARG: return "invalid argument";
+    case STELLAR_ERR_OOM: return "out of memory";
+    case STELLAR_ERR_POOL: return "node pool overflow (more cells than node_capacity)";
+    case STELLAR_ERR_STACK: return "walk stack overflow";
+    case STELLAR_ERR_DEPTH: return "too many bodies share one level-32 cell";
+    case STELLAR_ERR_DEVICE: return "HIP runtime error";
+    case STELLAR_ERR_STATE: return "stage called out of order";
+    case STELLAR_ERR_BUCKET: return "more than 4 bodies share a level-64 cell";
+    default: return "unknown status";
+  }
+}
+
+int fail(stellar_ctx* c, int status, const char* what) {
+  if (c) {
+    c->err = std::string(status_text(status)) + ": " + what;
+  }
+  return status;
+}
+
+int hip_fail(stellar_ctx* c, hipError_t e, const char* what) {
+  if (c) {
+    c->err = std::string("HIP runtime error in ") + what + ": " +
+             hipGetErrorString(e);
+  }
+  return STELLAR_ERR_DEVICE;
+}
+
+#define HIP_TRY(ctx, expr)                                   \
+  do {                                                       \
+    hipError_t e_ = (expr);                                  \
+    if (e_ != hipSuccess) return hip_fail((ctx), e_, #expr); \
+  } while (0)
+
+bool params_ok(const stellar_params* p) {
+  if (!p) return false;
+  if (!(p->theta > 0.f) || !(p->eps2 > 0.f)) return false;
+  if (p->group != 64 && p->group != 32 && p->group != 16) return false;
+  if (p->precision != STELLAR_PRECISION_FP32 &&
+      p->precision != STELLAR_PRECISION_MIXED)
+    return false;
+  return true;
+}
+
+template <typename T>
+hipError_t dmalloc(T** p, size_t count) {
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T) + 256);
+}
+
+void free_all(stellar_ctx* c) {
+  Buffers& B = c->b;
+  void* ptrs[] = {B.x,     B.y,      B.vx,      B.vy,     B.m,      B.id,
+                  B.x2,    B.y2,     B.vx2,     B.vy2,    B.m2,     B.id2,
+                  B.ax,    B.ay,     B.inter,   B.key,    B.key_alt, B.perm,
+                  B.perm_alt, B.key2, B.ncell,  B.coff,   B.tmp,    B.blk,
+                  B.cstart, B.ccount, B.cparent, B.clevel, B.cslot, B.ckc,
+                  B.arrive, B.csum,  B.ccom,    B.out,    B.dev};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (B.host_dev) (void)hipHostFree(B.host_dev);
+  B = Buffers{};
+}
+
+// Read the sticky error word (synchronises the stream).
+int check_device(stellar_ctx* c) {
+  HIP_TRY(c, hipMemcpyAsync(c->b.host_dev, c->b.dev, sizeof(DevScalars),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const uint32_t e = c->b.host_dev->err;
+  if (e == 0) return STELLAR_OK;
+  // clear so that a later, independent call is not reported again
+  HIP_TRY(c, hipMemsetAsync(&c->b.dev->err, 0, sizeof(uint32_t), c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->stage = kNone;
+  if (e & kErrPool) return fail(c, STELLAR_ERR_POOL, "raise node_capacity");
+  if (e & kErrTieRun) return fail(c, STELLAR_ERR_DEPTH, "tie run too long");
+  if (e & kErrBucket) return fail(c, STELLAR_ERR_BUCKET, "coincident bodies");
+  if (e & kErrStack) return fail(c, STELLAR_ERR_STACK, "walk stack");
+  return fail(c, STELLAR_ERR_DEVICE, "unknown device error");
+}
+
+bool timing(const stellar_ctx* c) {
+  return (c->p.flags & STELLAR_FLAG_STAGE_TIMING) != 0;
+}
+
+// Events of step `step` of the current run (step < 0: the single-step set).
+hipEvent_t* step_events(stellar_ctx* c, int32_t step) {
+  return step < 0 ? c->ev : c->evrun.data() + 8 * (size_t)step;
+}
+
+void mark(stellar_ctx* c, hipEvent_t* ev, int k) {
+  if (timing(c)) (void)hipEventRecord(ev[k], c->stream);
+}
+
+int run_step(stellar_ctx* c, int32_t step) {
+  hipEvent_t* ev = timing(c) ? step_events(c, step) : nullptr;
+  mark(c, ev, 0);
+  launch_bbox(c);
+  mark(c, ev, 1);
+  launch_build(c);
+  mark(c, ev, 2);
+  launch_com(c);
+  mark(c, ev, 3);
+  launch_sort(c);
+  mark(c, ev, 4);
+  launch_forces(c);
+  mark(c, ev, 5);
+  launch_update(c);
+  mark(c, ev, 6);
+  launch_output(c);
+  mark(c, ev, 7);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(c, e, "kernel launch");
+  c->stage = kUpdateDone;
+  c->sorted_storage = true;
+  return STELLAR_OK;
+}
+
+// Make room for `steps` sets of stage events.
+int reserve_run_events(stellar_ctx* c, int32_t steps) {
+  const size_t need = 8 * (size_t)steps;
+  while (c->evrun.size() < need) {
+    hipEvent_t e;
+    HIP_TRY(c, hipEventCreate(&e));
+    c->evrun.push_back(e);
+  }
+  return STELLAR_OK;
+}
+
+// Mean per-stage milliseconds over steps [0, steps) (steps < 0: single set).
+int collect_times(stellar_ctx* c, int32_t steps) {
+  if (!timing(c)) return STELLAR_OK;
+  const int32_t nsets = steps < 0 ? 1 : steps;
+  if (nsets == 0) return STELLAR_OK;
+  double acc[7] = {};
+  for (int32_t s = 0; s < nsets; ++s) {
+    hipEvent_t* ev = step_events(c, steps < 0 ? -1 : s);
+    for (int k = 0; k < 7; ++k) {
+      float ms = 0.f;
+      HIP_TRY(c, hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      acc[k] += ms;
+    }
+  }
+  for (int k = 0; k < 7; ++k) c->stage_ms[k] = (float)(acc[k] / nsets);
+  return STELLAR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void stellar_default_params(stellar_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->theta = 0.5f;
+  p->eps2 = 0.025f;
+  p->timestep = 0.001f;
+  p->dampening = 1.0f;
+  p->gravity = 1.0f;
+  p->group = 64;
+  p->precision = STELLAR_PRECISION_FP32;
+  p->flags = 0;
+}
+
+const char* stellar_status_string(int status) { return status_text(status); }
+
+const char* stellar_last_error(const stellar_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
+                   stellar_ctx** out) {
+  if (!out || n < 0 || !params_ok(p)) return STELLAR_ERR_ARG;
+  *out = nullptr;
+  stellar_ctx* c = new (std::nothrow) stellar_ctx();
+  if (!c) return STELLAR_ERR_OOM;
+  c->p = *p;
+  c->n = n;
+  c->cap = node_capacity > 0 ? node_capacity : 2 * (int64_t)n + 12000;
+  if (c->cap >= (int64_t)INT32_MAX - 1) {
+    delete c;
+    return STELLAR_ERR_ARG;
+  }
+  (void)hipGetDevice(&c->device);
+  Buffers& B = c->b;
+  const size_t N = (size_t)(n > 0 ? n : 1);
+  const size_t C = (size_t)c->cap;
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t r) {
+    if (e == hipSuccess && r != hipSuccess) e = r;
+  };
+  chk(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  chk(dmalloc(&B.x, N));
+  chk(dmalloc(&B.y, N));
+  chk(dmalloc(&B.vx, N));
+  chk(dmalloc(&B.vy, N));
+  chk(dmalloc(&B.m, N));
+  chk(dmalloc(&B.id, N));
+  chk(dmalloc(&B.x2, N));
+  chk(dmalloc(&B.y2, N));
+  chk(dmalloc(&B.vx2, N));
+  chk(dmalloc(&B.vy2, N));
+  chk(dmalloc(&B.m2, N));
+  chk(dmalloc(&B.id2, N));
+  chk(dmalloc(&B.ax, N));
+  chk(dmalloc(&B.ay, N));
+  chk(dmalloc(&B.inter, N));
+  chk(dmalloc(&B.key, N));
+  chk(dmalloc(&B.key_alt, N));
+  chk(dmalloc(&B.perm, N));
+  chk(dmalloc(&B.perm_alt, N));
+  chk(dmalloc(&B.key2, N));
+  chk(dmalloc(&B.ncell, N));
+  chk(dmalloc(&B.coff, N));
+  B.tmp_bytes = build_scratch_bytes((int32_t)N);
+  chk(hipMalloc(&B.tmp, B.tmp_bytes));
+  chk(dmalloc(&B.blk, 4 * (C + 1)));
+  chk(dmalloc(&B.cstart, C + 1));
+  chk(dmalloc(&B.ccount, C + 1));
+  chk(dmalloc(&B.cparent, C + 1));
+  chk(dmalloc(&B.clevel, C + 1));
+  chk(dmalloc(&B.cslot, C + 1));
+  chk(dmalloc(&B.ckc, C + 1));
+  chk(dmalloc(&B.arrive, C + 1));
+  chk(dmalloc(&B.csum, 3 * (C + 1)));
+  chk(dmalloc(&B.ccom, C + 1));
+  chk(dmalloc(&B.out, 2 * (N + C)));
+  chk(dmalloc(&B.dev, 1));
+  chk(hipHostMalloc(reinterpret_cast<void**>(&B.host_dev), sizeof(DevScalars),
+                    hipHostMallocDefault));
+  for (int k = 0; k < 8; ++k) chk(hipEventCreate(&c->ev[k]));
+  if (e != hipSuccess) {
+    free_all(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    for (int k = 0; k < 8; ++k)
+      if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    delete c;
+    (void)hipGetLastError();
+    return e == hipErrorOutOfMemory ? STELLAR_ERR_OOM : STELLAR_ERR_DEVICE;
+  }
+  // clear the output buffer and the cell records once
+  chk(hipMemsetAsync(B.out, 0, 2 * (N + C) * sizeof(float), c->stream));
+  chk(hipMemsetAsync(B.blk, 0, 4 * (C + 1) * sizeof(ChildRec), c->stream));
+  chk(hipMemsetAsync(B.ax, 0, N * sizeof(float), c->stream));
+  chk(hipMemsetAsync(B.ay, 0, N * sizeof(float), c->stream));
+  launch_init_scalars(c);
+  chk(hipGetLastError());
+  chk(hipStreamSynchronize(c->stream));
+  if (e != hipSuccess) {
+    int rc = hip_fail(c, e, "stellar_create");
+    stellar_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return STELLAR_OK;
+}
+
+int stellar_destroy(stellar_ctx* c) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  for (int k = 0; k < 8; ++k)
+    if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+  for (hipEvent_t e : c->evrun) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return STELLAR_OK;
+}
+
+int stellar_set_params(stellar_ctx* c, const stellar_params* p) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (!params_ok(p)) return fail(c, STELLAR_ERR_ARG, "parameters");
+  c->p = *p;
+  return STELLAR_OK;
+}
+
+int stellar_get_params(const stellar_ctx* c, stellar_params* p) {
+  if (!c || !p) return STELLAR_ERR_ARG;
+  *p = c->p;
+  return STELLAR_OK;
+}
+
+int32_t stellar_num_bodies(const stellar_ctx* c) { return c ? c->n : -1; }
+
+int stellar_upload(stellar_ctx* c, const float* mass, const float* x,
+                   const float* y, const float* vx, const float* vy) {
+  if (!c) return STELLAR_ERR_ARG;
+  const int32_t n = c->n;
+  if (n > 0 && (!mass || !x || !y || !vx || !vy))
+    return fail(c, STELLAR_ERR_ARG, "null body array");
+  Buffers& B = c->b;
+  const size_t bytes = (size_t)n * sizeof(float);
+  std::vector<int32_t> ids((size_t)n);
+  for (int32_t i = 0; i < n; ++i) ids[(size_t)i] = i;
+  if (n > 0) {
+    HIP_TRY(c, hipMemcpyAsync(B.m, mass, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(B.x, x, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(B.y, y, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(B.vx, vx, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(B.vy, vy, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(B.id, ids.data(), bytes, hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, hipMemsetAsync(B.ax, 0, bytes, c->stream));
+    HIP_TRY(c, hipMemsetAsync(B.ay, 0, bytes, c->stream));
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->uploaded = true;
+  c->stage = kNone;
+  c->sorted_storage = false;
+  return STELLAR_OK;
+}
+
+int stellar_download(stellar_ctx* c, float* mass, float* x, float* y,
+                     float* vx, float* vy, float* ax, float* ay) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
+  const int32_t n = c->n;
+  if (n == 0) return STELLAR_OK;
+  Buffers& B = c->b;
+  const size_t bytes = (size_t)n * sizeof(float);
+  std::vector<int32_t> id((size_t)n);
+  std::vector<float> tmp((size_t)n);
+  HIP_TRY(c, hipMemcpyAsync(id.data(), B.id, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const float* src[7] = {B.m, B.x, B.y, B.vx, B.vy, B.ax, B.ay};
+  float* dst[7] = {mass, x, y, vx, vy, ax, ay};
+  for (int k = 0; k < 7; ++k) {
+    if (!dst[k]) continue;
+    HIP_TRY(c, hipMemcpyAsync(tmp.data(), src[k], bytes, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int32_t s = 0; s < n; ++s) dst[k][id[(size_t)s]] = tmp[(size_t)s];
+  }
+  return STELLAR_OK;
+}
+
+int stellar_download_positions(stellar_ctx* c, float* out_xy) {
+  if (!c || !out_xy) return STELLAR_ERR_ARG;
+  if (c->stage != kUpdateDone)
+    return fail(c, STELLAR_ERR_STATE, "output buffer is written by a step");
+  if (c->n == 0) return STELLAR_OK;
+  HIP_TRY(c, hipMemcpyAsync(out_xy, c->b.out, 2 * (size_t)c->n * sizeof(float),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return STELLAR_OK;
+}
+
+int stellar_bbox(stellar_ctx* c) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
+  launch_bbox(c);
+  HIP_TRY(c, hipGetLastError());
+  c->stage = kBboxDone;
+  return check_device(c);
+}
+
+int stellar_build(stellar_ctx* c) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (c->stage != kBboxDone) return fail(c, STELLAR_ERR_STATE, "build needs bbox");
+  launch_build(c);
+  HIP_TRY(c, hipGetLastError());
+  c->stage = kBuildDone;
+  c->sorted_storage = false;
+  return check_device(c);
+}
+
+int stellar_com(stellar_ctx* c) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (c->stage != kBuildDone) return fail(c, STELLAR_ERR_STATE, "com needs build");
+  launch_com(c);
+  HIP_TRY(c, hipGetLastError());
+  c->stage = kComDone;
+  return check_device(c);
+}
+
+int stellar_sort(stellar_ctx* c) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (c->stage != kComDone) return fail(c, STELLAR_ERR_STATE, "sort needs com");
+  launch_sort(c);
+  HIP_TRY(c, hipGetLastError());
+  c->stage = kSortDone;
+  c->sorted_storage = true;
+  return check_device(c);
+}
+
+int stellar_forces(stellar_ctx* c) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (c->stage != kSortDone && c->stage != kForcesDone)
+    return fail(c, STELLAR_ERR_STATE, "forces need sort");
+  launch_forces(c);
+  HIP_TRY(c, hipGetLastError());
+  c->stage = kForcesDone;
+  return check_device(c);
+}
+
+int stellar_update(stellar_ctx* c) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (c->stage != kForcesDone) return fail(c, STELLAR_ERR_STATE, "update needs forces");
+  launch_update(c);
+  launch_output(c);
+  HIP_TRY(c, hipGetLastError());
+  c->stage = kUpdateDone;
+  return check_device(c);
+}
+
+int stellar_step(stellar_ctx* c) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
+  int rc = run_step(c, -1);
+  if (rc) return rc;
+  rc = check_device(c);
+  if (rc) return rc;
+  return collect_times(c, -1);
+}
+
+int stellar_run(stellar_ctx* c, int32_t steps) {
+  if (!c || steps < 0) return STELLAR_ERR_ARG;
+  if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
+  if (timing(c)) {
+    int rc = reserve_run_events(c, steps);
+    if (rc) return rc;
+  }
+  for (int32_t k = 0; k < steps; ++k) {
+    int rc = run_step(c, k);
+    if (rc) return rc;
+  }
+  int rc = check_device(c);
+  if (rc) return rc;
+  return collect_times(c, steps);
+}
+
+int stellar_sync(stellar_ctx* c) {
+  if (!c) return STELLAR_ERR_ARG;
+  return check_device(c);
+}
+
+int stellar_get_bbox(stellar_ctx* c, float* bbox4) {
+  if (!c || !bbox4) return STELLAR_ERR_ARG;
+  int rc = check_device(c);
+  if (rc) return rc;
+  std::memcpy(bbox4, c->b.host_dev->bbox, 4 * sizeof(float));
+  return STELLAR_OK;
+}
+
+int stellar_num_cells(stellar_ctx* c, int64_t* ncells) {
+  if (!c || !ncells) return STELLAR_ERR_ARG;
+  int rc = check_device(c);
+  if (rc) return rc;
+  *ncells = c->b.host_dev->ncells;
+  return STELLAR_OK;
+}
+
+int stellar_last_interactions(stellar_ctx* c, int64_t* total) {
+  if (!c || !total) return STELLAR_ERR_ARG;
+  int rc = check_device(c);
+  if (rc) return rc;
+  *total = (int64_t)c->b.host_dev->interactions;
+  return STELLAR_OK;
+}
+
+int stellar_stage_times(stellar_ctx* c, float* ms7) {
+  if (!c || !ms7) return STELLAR_ERR_ARG;
+  std::memcpy(ms7, c->stage_ms, sizeof(c->stage_ms));
+  return STELLAR_OK;
+}
+
+int stellar_export_tree(stellar_ctx* c, int32_t* root4, int32_t* child,
+                        int32_t* count, int32_t* start, int32_t* level,
+                        int32_t* sorted, float* com_x, float* com_y,
+                        float* com_m, int64_t capacity, int64_t* ncells) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (c->stage < kBuildDone) return fail(c, STELLAR_ERR_STATE, "no tree built");
+  int rc = check_device(c);
+  if (rc) return rc;
+  const int32_t n = c->n;
+  const int64_t C = c->b.host_dev->ncells;
+  if (ncells) *ncells = C;
+  if (capacity < C && (child || count || start || level || com_x || com_y || com_m))
+    return fail(c, STELLAR_ERR_ARG, "export capacity below cell count");
+  Buffers& B = c->b;
+  std::vector<int32_t> id((size_t)(n > 0 ? n : 1)), perm((size_t)(n > 0 ? n : 1));
+  std::vector<int32_t> srt((size_t)(n > 0 ? n : 1));
+  if (n > 0) {
+    HIP_TRY(c, hipMemcpy(id.data(), B.id, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (c->sorted_storage) {
+      for (int32_t s = 0; s < n; ++s) srt[(size_t)s] = id[(size_t)s];
+    } else {
+      HIP_TRY(c, hipMemcpy(perm.data(), B.perm, (size_t)n * 4, hipMemcpyDeviceToHost));
+      for (int32_t s = 0; s < n; ++s) srt[(size_t)s] = id[(size_t)perm[(size_t)s]];
+    }
+  }
+  if (sorted && n > 0) std::memcpy(sorted, srt.data(), (size_t)n * 4);
+  std::vector<ChildRec> blk((size_t)(4 * (C + 1)));
+  HIP_TRY(c, hipMemcpy(blk.data(), B.blk, blk.size() * sizeof(ChildRec),
+                       hipMemcpyDeviceToHost));
+  auto conv = [&](int32_t code) -> int32_t {
+    if (code == 0) return -1;
+    if (code < 0) return srt[(size_t)(-1 - code)];
+    return n + (code - 1);
+  };
+  if (root4)
+    for (int k = 0; k < 4; ++k) root4[k] = conv(blk[(size_t)k].code);
+  if (child)
+    for (int64_t q = 0; q < C; ++q)
+      for (int k = 0; k < 4; ++k)
+        child[4 * q + k] = conv(blk[(size_t)(4 * (q + 1) + k)].code);
+  if (C > 0 && (count || start || level)) {
+    std::vector<int32_t> tmp((size_t)(C + 1));
+    std::vector<uint8_t> lv((size_t)(C + 1));
+    if (count) {
+      HIP_TRY(c, hipMemcpy(tmp.data(), B.ccount, (size_t)(C + 1) * 4, hipMemcpyDeviceToHost));
+      std::memcpy(count, tmp.data() + 1, (size_t)C * 4);
+    }
+    if (start) {
+      HIP_TRY(c, hipMemcpy(tmp.data(), B.cstart, (size_t)(C + 1) * 4, hipMemcpyDeviceToHost));
+      std::memcpy(start, tmp.data() + 1, (size_t)C * 4);
+    }
+    if (level) {
+      HIP_TRY(c, hipMemcpy(lv.data(), B.clevel, (size_t)(C + 1), hipMemcpyDeviceToHost));
+      for (int64_t q = 0; q < C; ++q) level[q] = lv[(size_t)(q + 1)];
+    }
+  }
+  if (C > 0 && (com_x || com_y || com_m)) {
+    if (c->stage < kComDone) return fail(c, STELLAR_ERR_STATE, "no centre of mass yet");
+    std::vector<float4> com((size_t)(C + 1));
+    HIP_TRY(c, hipMemcpy(com.data(), B.ccom, (size_t)(C + 1) * sizeof(float4),
+                         hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < C; ++q) {
+      if (com_x) com_x[q] = com[(size_t)(q + 1)].x;
+      if (com_y) com_y[q] = com[(size_t)(q + 1)].y;
+      if (com_m) com_m[q] = com[(size_t)(q + 1)].z;
+    }
+  }
+  return STELLAR_OK;
+}
+
+int stellar_export_forces(stellar_ctx* c, float* ax, float* ay,
+                          int32_t* interactions) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (c->stage < kForcesDone) return fail(c, STELLAR_ERR_STATE, "no forces yet");
+  int rc = check_device(c);
+  if (rc) return rc;
+  const int32_t n = c->n;
+  if (n == 0) return STELLAR_OK;
+  Buffers& B = c->b;
+  std::vector<int32_t> id((size_t)n), it((size_t)n);
+  std::vector<float> tmp((size_t)n);
+  HIP_TRY(c, hipMemcpy(id.data(), B.id, (size_t)n * 4, hipMemcpyDeviceToHost));
+  if (ax) {
+    HIP_TRY(c, hipMemcpy(tmp.data(), B.ax, (size_t)n * 4, hipMemcpyDeviceToHost));
+    for (int32_t s = 0; s < n; ++s) ax[id[(size_t)s]] = tmp[(size_t)s];
+  }
+  if (ay) {
+    HIP_TRY(c, hipMemcpy(tmp.data(), B.ay, (size_t)n * 4, hipMemcpyDeviceToHost));
+    for (int32_t s = 0; s < n; ++s) ay[id[(size_t)s]] = tmp[(size_t)s];
+  }
+  if (interactions) {
+    HIP_TRY(c, hipMemcpy(it.data(), B.inter, (size_t)n * 4, hipMemcpyDeviceToHost));
+    for (int32_t s = 0; s < n; ++s) interactions[id[(size_t)s]] = it[(size_t)s];
+  }
+  return STELLAR_OK;
+}
+
+int stellar_direct_fp64(stellar_ctx* c, int32_t ntargets, const int32_t* targets,
+                        double eps2, double* ax, double* ay) {
+  if (!c || !ax || !ay || ntargets < 0) return STELLAR_ERR_ARG;
+  if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
+  if (!targets && ntargets != c->n) return fail(c, STELLAR_ERR_ARG, "ntargets");
+  const int32_t n = c->n;
+  if (ntargets == 0 || n == 0) return STELLAR_OK;
+  if (targets)
+    for (int32_t q = 0; q < ntargets; ++q)
+      if (targets[q] < 0 || targets[q] >= n)
+        return fail(c, STELLAR_ERR_ARG, "target id out of range");
+  int32_t* dt = nullptr;
+  double *dax = nullptr, *day = nullptr;
+  HIP_TRY(c, hipMalloc(&dax, (size_t)ntargets * sizeof(double)));
+  if (hipMalloc(&day, (size_t)ntargets * sizeof(double)) != hipSuccess) {
+    (void)hipFree(dax);
+    return fail(c, STELLAR_ERR_OOM, "direct sum buffers");
+  }
+  if (targets) {
+    if (hipMalloc(&dt, (size_t)ntargets * sizeof(int32_t)) != hipSuccess) {
+      (void)hipFree(dax);
+      (void)hipFree(day);
+      return fail(c, STELLAR_ERR_OOM, "direct sum buffers");
+    }
+    (void)hipMemcpyAsync(dt, targets, (size_t)ntargets * 4, hipMemcpyHostToDevice,
+                         c->stream);
+  }
+  launch_direct_fp64(c, ntargets, dt, eps2, dax, day);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ax, dax, (size_t)ntargets * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ay, day, (size_t)ntargets * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dax);
+  (void)hipFree(day);
+  if (dt) (void)hipFree(dt);
+  if (e != hipSuccess) return hip_fail(c, e, "stellar_direct_fp64");
+  return STELLAR_OK;
+}
+
+int stellar_energy_fp64(stellar_ctx* c, double eps2, double* kinetic,
+                        double* potential, double* px, double* py) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
+  double k = 0, u = 0, mx = 0, my = 0;
+  if (c->n > 0) {
+    const int parts = energy_parts(c);
+    double* dpart = nullptr;
+    HIP_TRY(c, hipMalloc(&dpart, (size_t)parts * 4 * sizeof(double)));
+    launch_energy_fp64(c, eps2, dpart, parts);
+    std::vector<double> h((size_t)parts * 4);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h.data(), dpart, h.size() * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dpart);
+    if (e != hipSuccess) return hip_fail(c, e, "stellar_energy_fp64");
+    for (int q = 0; q < parts; ++q) {
+      u += h[(size_t)(4 * q + 0)];
+      k += h[(size_t)(4 * q + 1)];
+      mx += h[(size_t)(4 * q + 2)];
+      my += h[(size_t)(4 * q + 3)];
+    }
+  }
+  if (kinetic) *kinetic = k;
+  if (potential) *potential = u;
+  if (px) *px = mx;
+  if (py) *py = my;
+  return STELLAR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// Internal definitions shared by the libstellar_hip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "stellar.h"
+
+namespace stellar {
+
+// Sticky device error bits (ctx->dev_err), mapped to status codes on sync.
+enum : uint32_t {
+  kErrPool = 1u,    // cell count exceeded the node capacity
+  kErrStack = 2u,   // walk stack overflow
+  kErrTieRun = 4u,  // more than kMaxTieRun bodies share a 32-level key
+  kErrBucket = 8u,  // more than 4 bodies share a level-64 cell
+};
+
+constexpr int kMaxLevel = STELLAR_MAX_LEVEL;
+constexpr int kMaxTieRun = 256;
+constexpr int kWalkStack = 256;  // >= 4 + 3*(kMaxLevel-1)
+constexpr int kWalkBlock = 256;  // threads per walk workgroup (4 wavefronts)
+
+// Child record of a tree block: the walk's only view of the tree.  A block is
+// the four child slots of one node (block 0 = root, block 1+c = cell c),
+// 64 bytes, loaded whole with one scalar load per visited node.
+//   x, y : centre of mass (cell) or position (body)
+//   m    : mass
+//   code : 0 empty, -1-s body at tree position s, 1+c cell c
+struct alignas(16) ChildRec {
+  float x, y, m;
+  int32_t code;
+};
+
+// Device scalars, one 256-byte line.
+struct DevScalars {
+  uint32_t bbox_acc[4];  // ordered-int min x, max x, min y, max y
+  uint32_t bbox_ticket;  // blocks finished in the current bbox launch
+  uint32_t err;          // sticky error bits
+  int32_t ncells;        // cells of the current tree
+  int32_t pad0;
+  float bbox[4];         // left, right, bottom, top
+  unsigned long long interactions;
+  uint32_t pad[46];
+};
+
+struct Buffers {
+  // body state in tree order of the last sort (double-buffered for the sort)
+  float *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr,
+        *m = nullptr;
+  int32_t* id = nullptr;
+  float *x2 = nullptr, *y2 = nullptr, *vx2 = nullptr, *vy2 = nullptr,
+        *m2 = nullptr;
+  int32_t* id2 = nullptr;
+  float *ax = nullptr, *ay = nullptr;
+  int32_t* inter = nullptr;  // per-body interaction count (tree order)
+  // build
+  uint64_t *key = nullptr, *key_alt = nullptr;  // level 1..32 codes
+  uint32_t *perm = nullptr, *perm_alt = nullptr;
+  uint64_t* key2 = nullptr;   // level 33..64 codes (valid inside tie runs)
+  int32_t* ncell = nullptr;   // cells opened by each adjacent pair
+  int32_t* coff = nullptr;    // exclusive scan of ncell
+  void* tmp = nullptr;        // rocprim scratch
+  size_t tmp_bytes = 0;
+  // cells
+  ChildRec* blk = nullptr;    // (1+cap) blocks of 4 records
+  int32_t *cstart = nullptr, *ccount = nullptr, *cparent = nullptr;
+  uint8_t *clevel = nullptr, *cslot = nullptr, *ckc = nullptr;
+  int32_t* arrive = nullptr;
+  double* csum = nullptr;     // 3 per cell: sum m, sum m x, sum m y
+  float4* ccom = nullptr;     // per cell: com x, com y, mass, 0
+  float* out = nullptr;       // 2*(n+cap) interleaved output buffer
+  DevScalars* dev = nullptr;
+  DevScalars* host_dev = nullptr;  // pinned mirror
+};
+
+enum Stage : int {
+  kNone = 0,
+  kBboxDone = 1,
+  kBuildDone = 2,
+  kComDone = 3,
+  kSortDone = 4,
+  kForcesDone = 5,
+  kUpdateDone = 6,
+};
+
+}  // namespace stellar
+
+struct stellar_ctx {
+  stellar_params p;
+  int32_t n = 0;
+  int64_t cap = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  stellar::Buffers b;
+  int stage = stellar::kNone;
+  bool sorted_storage = false;  // storage order == tree order of the build
+  bool uploaded = false;
+  hipEvent_t ev[8] = {};
+  // per-step stage events of the current stellar_run (8 per step)
+  std::vector<hipEvent_t> evrun;
+  int32_t evrun_steps = 0;
+  float stage_ms[7] = {};
+  std::string err;
+};
+
+namespace stellar {
+
+// Kernel launchers (stream-ordered, no host synchronisation).
+void launch_bbox(stellar_ctx* c);
+void launch_build(stellar_ctx* c);
+void launch_com(stellar_ctx* c);
+void launch_sort(stellar_ctx* c);
+void launch_forces(stellar_ctx* c);
+void launch_update(stellar_ctx* c);
+void launch_output(stellar_ctx* c);
+void launch_zero_counter(stellar_ctx* c);
+void launch_init_scalars(stellar_ctx* c);
+size_t build_scratch_bytes(int32_t n);
+void launch_direct_fp64(stellar_ctx* c, int32_t nt, const int32_t* dtargets,
+                        double eps2, double* dax, double* day);
+void launch_energy_fp64(stellar_ctx* c, double eps2, double* dpart,
+                        int32_t nparts);
+int energy_parts(stellar_ctx* c);
+
+}  // namespace stellar

@@ -1,0 +1,177 @@
+// Tree-walk force kernel for gfx950 (compute_forces_kernel, kernels.cu:340-435).
+//
+// A walk group is G consecutive bodies in tree order (the reference uses warps
+// of 32).  The group shares one traversal: for each child of a popped node it
+// evaluates r = dx*dx + dy*dy + eps2 per body and accepts the child when it is
+// a body or when every body of the group passes dp <= r (the reference's
+// __all vote); otherwise the child is pushed.  Accepted children add
+// f = m*rsqrt(r)^3 * (dx, dy) to every body of the group.
+//
+// gfx950 mapping (G = 64, one wavefront per group):
+//   * the node index is wavefront-uniform, so a node's four child records
+//     (one 64-byte block) arrive through scalar loads;
+//   * the vote is one 64-lane ballot;
+//   * the traversal stack lives in LDS, one region per wavefront, written by
+//     lane 0 and read back as a uniform value;
+//   * bodies are stored in tree order, so the group's positions and the
+//     outputs are contiguous 256-byte rows.
+// G = 32 and 16 run 2 or 4 independent groups per wavefront (ballot masks
+// per group), which reproduces the reference's warp-of-32 vote exactly.
+//
+// Traversal order matches the reference (children in slot order, accepted
+// ones accumulated at once, opened ones pushed and popped LIFO), so the set
+// of interactions and the summation order are those of the CPU restatement.
+// One deliberate difference: a body alone in a root slot is accumulated; the
+// reference pushes it like a cell and then finds no children (kernels.cu:
+// 373-387, 397-398), so that body exerts no force.
+#include "engine.h"
+
+namespace stellar {
+
+namespace {
+
+template <int G>
+__global__ __launch_bounds__(kWalkBlock) void walk_kernel(
+    const float* __restrict__ x, const float* __restrict__ y,
+    const ChildRec* __restrict__ blk, int32_t n, float eps2, float theta,
+    DevScalars* __restrict__ dev, float* __restrict__ ax,
+    float* __restrict__ ay, int32_t* __restrict__ inter) {
+  constexpr int kGroupsPerWave = 64 / G;
+  constexpr int kWaves = kWalkBlock / 64;
+  __shared__ int32_t stk_node[kWaves * kGroupsPerWave][kWalkStack];
+  __shared__ float stk_dep[kWaves * kGroupsPerWave][kWalkStack];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int sub = lane / G;             // group within the wavefront
+  const int glane = lane & (G - 1);     // lane within the group
+  const int slot = wave * kGroupsPerWave + sub;
+  const int64_t s0 = ((int64_t)blockIdx.x * kWaves + wave) * 64 + sub * G;
+  if (((int64_t)blockIdx.x * kWaves + wave) * 64 >= n) return;
+  const int64_t s = s0 + glane;
+  const bool valid = s < n;
+  const float px = valid ? x[s] : 0.f;
+  const float py = valid ? y[s] : 0.f;
+
+  // depth = radius^2/theta with radius = half the x extent (kernels.cu:349,376)
+  const float bl = dev->bbox[0], br = dev->bbox[1];
+  const float radius = (float)(0.5 * (double)(br - bl));
+  const float depth0 = (float)((double)(radius * radius) / (double)theta);
+
+  const uint64_t gmask =
+      G == 64 ? ~0ull : (((1ull << G) - 1ull) << (sub * G));
+
+  float fx = 0.f, fy = 0.f;
+  int32_t cnt = 0;
+  int sp = 0;
+  bool overflow = false;
+
+  auto accumulate = [&](float cx, float cy, float cm, float r, float dx,
+                        float dy) {
+    const float rs = __builtin_amdgcn_rsqf(r);
+    const float f = cm * rs * rs * rs;
+    fx = __builtin_fmaf(f, dx, fx);
+    fy = __builtin_fmaf(f, dy, fy);
+    ++cnt;
+  };
+
+  // root block: level-1 cells are always opened (kernels.cu:370-388)
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const ChildRec c = blk[k];
+    if (c.code == 0) continue;
+    if (c.code < 0) {
+      const float dx = c.x - px, dy = c.y - py;
+      const float r = dx * dx + dy * dy + eps2;
+      accumulate(c.x, c.y, c.m, r, dx, dy);
+    } else {
+      if (glane == 0) {
+        stk_node[slot][sp] = c.code;
+        stk_dep[slot][sp] = depth0;
+      }
+      ++sp;
+    }
+  }
+
+  while (sp > 0) {
+    --sp;
+    int32_t node = stk_node[slot][sp];
+    float depth = stk_dep[slot][sp];
+    if (G == 64) {
+      node = __builtin_amdgcn_readfirstlane(node);
+      depth = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(depth)));
+    }
+    const float dp = 0.25f * depth;
+    // the whole 64-byte block first (one scalar load for G = 64), then the
+    // four children from registers
+    const int4* B = reinterpret_cast<const int4*>(blk + 4 * (int64_t)node);
+    int4 q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = B[k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const ChildRec c{__int_as_float(q[k].x), __int_as_float(q[k].y),
+                       __int_as_float(q[k].z), q[k].w};
+      if (c.code == 0) continue;
+      const float dx = c.x - px, dy = c.y - py;
+      const float r = dx * dx + dy * dy + eps2;
+      bool take = c.code < 0;
+      if (!take) {
+        const uint64_t bad = __ballot(valid && !(dp <= r)) & gmask;
+        take = bad == 0;
+      }
+      if (take) {
+        accumulate(c.x, c.y, c.m, r, dx, dy);
+      } else {
+        if (sp >= kWalkStack) {
+          overflow = true;
+          break;
+        }
+        if (glane == 0) {
+          stk_node[slot][sp] = c.code;
+          stk_dep[slot][sp] = dp;
+        }
+        ++sp;
+      }
+    }
+    if (overflow) break;
+  }
+  if (overflow && glane == 0) atomicOr(&dev->err, kErrStack);
+  if (valid) {
+    ax[s] = fx;
+    ay[s] = fy;
+    inter[s] = cnt;
+  }
+  if (glane == 0) {
+    const int64_t members = (n - s0) < G ? (n - s0) : G;
+    atomicAdd(&dev->interactions,
+              (unsigned long long)cnt * (unsigned long long)members);
+  }
+}
+
+}  // namespace
+
+void launch_forces(stellar_ctx* c) {
+  const int32_t n = c->n;
+  if (n == 0) return;
+  Buffers& B = c->b;
+  const int64_t waves = (n + 63) / 64;
+  const unsigned grid = (unsigned)((waves + (kWalkBlock / 64) - 1) / (kWalkBlock / 64));
+  launch_zero_counter(c);
+  switch (c->p.group) {
+    case 32:
+      walk_kernel<32><<<grid, kWalkBlock, 0, c->stream>>>(
+          B.x, B.y, B.blk, n, c->p.eps2, c->p.theta, B.dev, B.ax, B.ay, B.inter);
+      break;
+    case 16:
+      walk_kernel<16><<<grid, kWalkBlock, 0, c->stream>>>(
+          B.x, B.y, B.blk, n, c->p.eps2, c->p.theta, B.dev, B.ax, B.ay, B.inter);
+      break;
+    default:
+      walk_kernel<64><<<grid, kWalkBlock, 0, c->stream>>>(
+          B.x, B.y, B.blk, n, c->p.eps2, c->p.theta, B.dev, B.ax, B.ay, B.inter);
+      break;
+  }
+}
+
+}  // namespace stellar

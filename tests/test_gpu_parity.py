@@ -1,0 +1,239 @@
+"""GPU parity: libstellar_hip (through its C ABI) against the CPU oracle.
+
+Bar: bit-exact for the bounding box, the tree (canonical child/count/start/
+level/sorted), the centres of mass, the per-body interaction counts and the
+update; forces within the rsqrt tolerance stated in each test (the GPU uses
+the hardware v_rsq_f32, the oracle a correctly rounded 1/sqrt).
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import first_tree_difference, ic, rms_rel, tree_equal
+
+pytestmark = pytest.mark.gpu
+
+EPS = {0: 0.01, 1: 0.005, 2: 0.01, 4: 1e-4}
+
+# Forces: the walk makes identical accept/open decisions (bit-exact counts);
+# the accumulated values differ only through rsqrt rounding (<= 2 ulp per
+# term) and so agree to a few parts in 1e7.
+FORCE_RMS_TOL = 2e-6
+FORCE_MAX_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def S():
+    from tests.conftest import gpu_available
+
+    if not gpu_available():
+        pytest.skip("no GPU")
+    import stellarsolver_amd as S
+
+    S.load_library()
+    return S
+
+
+def make_engine(S, st, eps2, theta=0.5, group=64, cap=0, **kw):
+    n = st["x"].size
+    p = S.default_params(theta=theta, eps2=eps2, group=group, **kw)
+    e = S.Engine(n, p, node_capacity=cap)
+    e.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
+    return e
+
+
+def oracle_tree(oracle, st):
+    bb = oracle.bbox(st["x"], st["y"])
+    kt = oracle.build_keys(st["x"], st["y"], bb)
+    oracle.com_exact(kt, st["x"], st["y"], st["mass"])
+    return bb, kt
+
+
+def gpu_tree(e):
+    e.bbox()
+    bb = e.get_bbox()
+    e.build()
+    e.com()
+    t = e.export_tree()
+    return bb, t
+
+
+@pytest.mark.parametrize("workload,n", [(0, 4096), (1, 65536), (2, 65536), (1, 262144)])
+def test_tree_bit_exact(S, oracle, workload, n):
+    print(f"tree parity workload={workload} n={n}", flush=True)
+    st = ic(workload, n)
+    eps2 = np.float32(EPS[workload] ** 2)
+    with make_engine(S, st, eps2) as e:
+        gbb, gt = gpu_tree(e)
+    obb, ot = oracle_tree(oracle, st)
+    assert np.array_equal(gbb, obb)
+    assert tree_equal(gt, ot), first_tree_difference(gt, ot)
+    for k in ("com_x", "com_y", "com_m"):
+        assert np.array_equal(gt[k], ot[k]), k
+
+
+@pytest.mark.parametrize("group", [64, 32, 16])
+@pytest.mark.parametrize("workload,n", [(0, 4096), (2, 65536), (1, 65536)])
+def test_forces_parity(S, oracle, workload, n, group):
+    print(f"force parity workload={workload} n={n} group={group}", flush=True)
+    st = ic(workload, n)
+    eps2 = float(np.float32(EPS[workload] ** 2))
+    with make_engine(S, st, eps2, group=group) as e:
+        gbb, gt = gpu_tree(e)
+        e.sort()
+        e.forces()
+        f = e.export_forces()
+        total = e.last_interactions()
+    obb, ot = oracle_tree(oracle, st)
+    ax, ay, it = oracle.walk(ot, st["x"], st["y"], st["mass"], obb, 0.5, eps2, group)
+    assert np.array_equal(f["inter"], it)
+    assert total == int(it.astype(np.int64).sum())
+    rel = np.sqrt(((f["ax"] - ax.astype(np.float64)) ** 2 + (f["ay"] - ay.astype(np.float64)) ** 2)
+                  / (ax.astype(np.float64) ** 2 + ay.astype(np.float64) ** 2))
+    assert np.sqrt(np.mean(rel ** 2)) < FORCE_RMS_TOL
+    assert rel.max() < FORCE_MAX_TOL
+
+
+def test_steps_match_oracle(S, oracle):
+    """Each GPU step = oracle walk on the GPU's own pre-step state, then the
+    oracle update applied to the GPU forces (bit-exact)."""
+    n, workload = 20000, 2
+    st = ic(workload, n)
+    eps2 = float(np.float32(1e-4))
+    dt, d = np.float32(1e-3), np.float32(1.0)
+    with make_engine(S, st, eps2, timestep=float(dt), dampening=float(d)) as e:
+        for k in range(3):
+            print(f"step {k}", flush=True)
+            pre = e.download()
+            e.update()
+            post = e.download()
+            obb, ot = oracle_tree(oracle, pre)
+            ax, ay, it = oracle.walk(ot, pre["x"], pre["y"], pre["mass"], obb, 0.5, eps2, 64)
+            assert rms_rel(post["ax"], post["ay"], ax, ay) < FORCE_RMS_TOL
+            x, y, vx, vy = oracle.update(pre["x"], pre["y"], pre["vx"], pre["vy"], post["ax"],
+                                         post["ay"], dt, d)
+            for a, b in ((x, post["x"]), (y, post["y"]), (vx, post["vx"]), (vy, post["vy"])):
+                assert np.array_equal(a, b)
+            out = e.get_output_buffer()
+            assert np.array_equal(out[0::2], post["x"]) and np.array_equal(out[1::2], post["y"])
+
+
+def test_run_is_deterministic(S):
+    st = ic(1, 65536)
+    eps2 = float(np.float32(0.005 ** 2))
+    res = []
+    for _ in range(2):
+        with make_engine(S, st, eps2, timestep=5e-4) as e:
+            e.run(5)
+            res.append(e.download())
+    for k in res[0]:
+        assert np.array_equal(res[0][k], res[1][k]), k
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 63, 65, 100, 1000])
+def test_small_sizes(S, oracle, n):
+    rng = np.random.default_rng(n)
+    st = {"x": rng.normal(0, 3, n).astype(np.float32), "y": rng.normal(1, 2, n).astype(np.float32),
+          "vx": np.zeros(n, np.float32), "vy": np.zeros(n, np.float32),
+          "mass": rng.uniform(0.5, 2, n).astype(np.float32)}
+    with make_engine(S, st, 0.01) as e:
+        gbb, gt = gpu_tree(e)
+        e.sort()
+        e.forces()
+        f = e.export_forces()
+    obb, ot = oracle_tree(oracle, st)
+    assert tree_equal(gt, ot), first_tree_difference(gt, ot)
+    ax, ay, it = oracle.walk(ot, st["x"], st["y"], st["mass"], obb, 0.5, 0.01, 64)
+    assert np.array_equal(f["inter"], it)
+    if n > 1:
+        assert rms_rel(f["ax"], f["ay"], ax, ay) < FORCE_RMS_TOL
+
+
+def test_three_body_known_answer(S):
+    st = {"x": np.array([1, 2, 3], np.float32), "y": np.array([1, 2, 3], np.float32),
+          "vx": np.zeros(3, np.float32), "vy": np.zeros(3, np.float32),
+          "mass": np.ones(3, np.float32)}
+    with make_engine(S, st, 0.01) as e:
+        bb, t = gpu_tree(e)
+    assert list(bb) == [0, 3, 0, 3]
+    assert list(t["root"]) == [3, -1, -1, 0] and list(t["child"]) == [2, -1, -1, 1]
+    assert list(t["sorted"]) == [2, 1, 0] and list(t["count"]) == [2]
+    assert t["com_x"][0] == 2.5 and t["com_m"][0] == 2.0
+
+
+def test_coincident_bodies_bucket(S, oracle):
+    rng = np.random.default_rng(1)
+    n = 200
+    x = rng.normal(0, 1, n).astype(np.float32)
+    y = rng.normal(0, 1, n).astype(np.float32)
+    x[10:13] = x[5]
+    y[10:13] = y[5]  # four bodies at one point -> one level-64 cell
+    st = {"x": x, "y": y, "vx": np.zeros(n, np.float32), "vy": np.zeros(n, np.float32),
+          "mass": np.ones(n, np.float32)}
+    with make_engine(S, st, 0.01) as e:
+        gbb, gt = gpu_tree(e)
+        e.sort()
+        e.forces()
+        f = e.export_forces()
+    obb, ot = oracle_tree(oracle, st)
+    assert tree_equal(gt, ot), first_tree_difference(gt, ot)
+    assert (gt["level"] == 64).sum() == 1
+    ax, ay, it = oracle.walk(ot, x, y, st["mass"], obb, 0.5, 0.01, 64)
+    assert np.array_equal(f["inter"], it)
+
+
+def test_error_codes(S):
+    n = 100
+    x = np.zeros(n, np.float32)
+    st = {"x": x, "y": x, "vx": x, "vy": x, "mass": np.ones(n, np.float32)}
+    with make_engine(S, st, 0.01) as e:
+        e.bbox()
+        with pytest.raises(S.StellarError) as err:
+            e.build()
+        assert err.value.status == S.STELLAR_ERR_BUCKET
+    st = ic(1, 5000)
+    with make_engine(S, st, 0.01, cap=100) as e:
+        e.bbox()
+        with pytest.raises(S.StellarError) as err:
+            e.build()
+        assert err.value.status == S.STELLAR_ERR_POOL
+    with make_engine(S, st, 0.01) as e:
+        with pytest.raises(S.StellarError) as err:
+            e.forces()
+        assert err.value.status == S.STELLAR_ERR_STATE
+        e.update()  # a full step still works after a refused stage
+
+
+def test_direct_and_energy_fp64(S, oracle):
+    st = ic(0, 4096)
+    with make_engine(S, st, 1e-4) as e:
+        tg = np.arange(0, 4096, 7, dtype=np.int32)
+        gx, gy = e.direct_fp64(tg, eps2=1e-4)
+        en = e.energy_fp64(eps2=1e-4)
+    ox, oy = oracle.direct_fp64(st["x"], st["y"], st["mass"], 1e-4, targets=tg)
+    assert np.allclose(gx, ox, rtol=1e-11, atol=1e-13) and np.allclose(gy, oy, rtol=1e-11, atol=1e-13)
+    oe = oracle.energy(st["x"], st["y"], st["vx"], st["vy"], st["mass"], 1e-4)
+    for k in oe:
+        assert np.isclose(en[k], oe[k], rtol=1e-11, atol=1e-9), k
+
+
+def test_headline_size_tree_and_counts(S, oracle):
+    """N = 2^20 colliding disks: bit-exact tree and interaction counts; force
+    error against the FP64 direct sum on a seeded 8192-body subsample."""
+    n = 1 << 20
+    print(f"headline parity n={n}", flush=True)
+    st = ic(2, n)
+    eps2 = float(np.float32(1e-4))
+    with make_engine(S, st, eps2) as e:
+        gbb, gt = gpu_tree(e)
+        e.sort()
+        e.forces()
+        f = e.export_forces()
+        tg = np.sort(np.random.default_rng(2).choice(n, 8192, replace=False)).astype(np.int32)
+        dx, dy = e.direct_fp64(tg)
+    obb, ot = oracle_tree(oracle, st)
+    assert tree_equal(gt, ot), first_tree_difference(gt, ot)
+    ax, ay, it = oracle.walk(ot, st["x"], st["y"], st["mass"], obb, 0.5, eps2, 64)
+    assert np.array_equal(f["inter"], it)
+    err = rms_rel(f["ax"][tg], f["ay"][tg], dx, dy)
+    print(f"rms relative error vs FP64 direct sum: {err:.3e}")
+    assert err < 3e-2  # the reference opening test at theta = 0.5 (monopole)

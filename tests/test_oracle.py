@@ -1,0 +1,336 @@
+"""Pins for the CPU oracle (oracle/oracle.c) -- CPU only.
+
+Every oracle function is checked against something that does not share its
+code: hand-computed known answers, closed forms, an independent construction
+of the same object, brute force, or the reference's own debug.cpp checker
+(oracle/_ref, built from the reference source where it is available).
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from tests.helpers import ic, tree_equal
+
+
+# ---------------------------------------------------------------- bbox ----
+def test_bbox_closed_form_includes_origin(oracle):
+    rng = np.random.default_rng(3)
+    for lo, hi in ((1.0, 5.0), (-5.0, -1.0), (-3.0, 4.0)):
+        x = rng.uniform(lo, hi, 1000).astype(np.float32)
+        y = rng.uniform(lo, hi, 1000).astype(np.float32)
+        b = oracle.bbox(x, y)
+        exp = [min(0.0, x.min()), max(0.0, x.max()), min(0.0, y.min()), max(0.0, y.max())]
+        assert np.array_equal(b, np.array(exp, np.float32))
+
+
+def test_bbox_negative_zero_does_not_replace_origin(oracle):
+    b = oracle.bbox(np.array([-0.0, 1.0], np.float32), np.array([2.0, -0.0], np.float32))
+    assert not np.signbit(b[0]) and not np.signbit(b[2])
+
+
+# ----------------------------------------------------- build: known answers
+def _full_build(oracle, x, y, m=None):
+    x = np.asarray(x, np.float32)
+    y = np.asarray(y, np.float32)
+    m = np.ones_like(x) if m is None else np.asarray(m, np.float32)
+    bb = oracle.bbox(x, y)
+    t = oracle.build_insert(x, y, m, bb)
+    oracle.com_ref(t)
+    oracle.sort_ref(t)
+    return bb, t, oracle.canonicalize(t), oracle.build_keys(x, y, bb)
+
+
+def test_two_bodies_known_answer(oracle):
+    # bbox [-1,1]^2, midpoints 0: (1,1) is east/north -> slot 0, (-1,-1) slot 3
+    bb, t, can, kt = _full_build(oracle, [1, -1], [1, -1])
+    assert list(bb) == [-1, 1, -1, 1]
+    assert list(can["root"]) == [0, -1, -1, 1] == list(kt["root"])
+    assert can["ncells"] == 0 == kt["ncells"]
+    assert list(kt["sorted"]) == [0, 1]
+
+
+def test_three_bodies_known_answer(oracle):
+    # bbox [0,3]^2, mid 1.5: (1,1) -> slot 3; (2,2),(3,3) share slot 0 and are
+    # split by a level-1 cell over [1.5,3]^2 with mid 2.25: (3,3) slot 0,
+    # (2,2) slot 3.  DFS order: cell{body2, body1}, body0.
+    n = 3
+    bb, t, can, kt = _full_build(oracle, [1, 2, 3], [1, 2, 3])
+    assert list(bb) == [0, 3, 0, 3]
+    assert list(kt["root"]) == [n + 0, -1, -1, 0]
+    assert list(kt["child"]) == [2, -1, -1, 1]
+    assert list(kt["count"]) == [2] and list(kt["start"]) == [0] and list(kt["level"]) == [1]
+    assert list(kt["sorted"]) == [2, 1, 0]
+    assert tree_equal(can, kt)
+    # reference pool layout: the cell took id n (kernels.cu:199)
+    assert t["ncells"] == 1 and list(t["child"][4 * n: 4 * n + 4]) == [2, -1, -1, 1]
+    # centre of mass of the cell: ((2+3)/2, (2+3)/2)
+    assert t["cx"][n] == 2.5 and t["cy"][n] == 2.5 and t["cm"][n] == 2.0
+
+
+def test_tie_goes_east_north(oracle):
+    # bbox [0,2]^2, mid 1: (1,1) sits on both midpoints -> slot 0 (x<mid false)
+    bb, t, can, kt = _full_build(oracle, [0, 2, 1], [0, 2, 1])
+    assert list(kt["root"])[3] == 0
+    cell = kt["root"][0] - 3
+    # level-1 cell [1,2]^2 (mid 1.5): (2,2) slot 0, (1,1) slot 3
+    assert list(kt["child"][4 * cell: 4 * cell + 4]) == [1, -1, -1, 2]
+
+
+def test_midpoint_is_fp32_sum_halved(oracle):
+    """kernels.cu:144: mid = 0.5*(l+r) with l+r rounded to FP32 first.
+    When the FP32 sum rounds down, a body sitting exactly on fl(l+r)/2 is
+    below the exact midpoint but not below the FP32 one: the reference sends
+    it east (slot bit 0 clear); an exact-midpoint rule would send it west."""
+    rng = np.random.default_rng(11)
+    found = 0
+    for _ in range(200000):
+        r = np.float32(rng.uniform(1.0, 2.0))
+        l = np.float32(r * np.float32(rng.uniform(0.3, 0.7)))
+        s = np.float32(l + r)
+        fmid = Fraction(float(s)) / 2
+        emid = (Fraction(float(l)) + Fraction(float(r))) / 2
+        if not fmid < emid:
+            continue
+        cand = np.float32(float(fmid))
+        assert Fraction(float(cand)) == fmid
+        bb = np.array([l, r, 0.0, 1.0], np.float32)
+        k1, _ = oracle.path_keys(np.array([cand], np.float32), np.array([0.9], np.float32), bb)
+        west = int(k1[0] >> np.uint64(62)) & 1
+        assert west == 0
+        found += 1
+        if found >= 20:
+            break
+    assert found >= 5
+
+
+# ----------------------------------------- build: two constructions agree --
+@pytest.mark.parametrize("workload,n", [(0, 4096), (1, 20000), (2, 30000)])
+def test_insertion_and_key_builds_agree(oracle, workload, n):
+    st = ic(workload, n)
+    bb, t, can, kt = _full_build(oracle, st["x"], st["y"], st["mass"])
+    assert tree_equal(can, kt)
+    # sorted = stable argsort of the path keys (the DFS-by-slot order)
+    k1, k2 = oracle.path_keys(st["x"], st["y"], bb)
+    order = np.lexsort((np.arange(n), k2, k1))
+    assert np.array_equal(order, kt["sorted"])
+
+
+def test_brute_force_cells_tiny(oracle):
+    """A cell at level L exists iff >= 2 bodies share L slot codes."""
+    rng = np.random.default_rng(5)
+    for trial in range(30):
+        n = int(rng.integers(2, 40))
+        x = rng.normal(0, 1, n).astype(np.float32)
+        y = rng.normal(0, 1, n).astype(np.float32)
+        bb = oracle.bbox(x, y)
+        k1, _ = oracle.path_keys(x, y, bb)
+        kt = oracle.build_keys(x, y, bb)
+        expect = 0
+        for L in range(1, 33):
+            pref = k1 >> np.uint64(64 - 2 * L)
+            _, cnt = np.unique(pref, return_counts=True)
+            expect += int((cnt >= 2).sum())
+        assert kt["ncells"] == expect
+        # every cell's count equals the number of bodies sharing its prefix
+        for c in range(kt["ncells"]):
+            L = kt["level"][c]
+            b0 = kt["sorted"][kt["start"][c]]
+            pref = k1 >> np.uint64(64 - 2 * L)
+            assert kt["count"][c] == int((pref == pref[b0]).sum())
+
+
+def test_reference_debug_checks_pass(oracle):
+    if not oracle.ref_debug_available():
+        pytest.skip("reference debug.cpp not built (oracle/_ref)")
+    st = ic(2, 8192)
+    bb, t, can, kt = _full_build(oracle, st["x"], st["y"], st["mass"])
+    ref = oracle.to_reference_layout(kt, 8192)
+    m = ref["m"]
+    lines = oracle.ref_debug_run(np.zeros(m), np.zeros(m), np.zeros(m), ref["count"],
+                                 np.zeros(m, np.int32), ref["sorted"], ref["child"], bb, 8192, m)
+    assert len(lines) == 8 and all("PASS" in ln for ln in lines), lines
+    assert oracle.invariants(8192, m, ref["count"], ref["sorted"], ref["child"], bb) == 0
+
+
+def test_invariants_catch_planted_mistakes(oracle):
+    """Plant mistakes in an oracle tree; the restated checks and the
+    reference's own debug.cpp must both flag them."""
+    st = ic(0, 4096)
+    bb, t, can, kt = _full_build(oracle, st["x"], st["y"], st["mass"])
+    base = oracle.to_reference_layout(kt, 4096)
+    m = base["m"]
+    cases = []
+    bad = {k: v.copy() for k, v in base.items() if k != "m"}
+    bad["count"][4096] += 1
+    cases.append((bad, 1))
+    bad = {k: v.copy() for k, v in base.items() if k != "m"}
+    bad["sorted"][5] = bad["sorted"][6]
+    cases.append((bad, 32 | 64))
+    bad = {k: v.copy() for k, v in base.items() if k != "m"}
+    i = int(np.flatnonzero(bad["child"][4 * 4096:] >= 0)[0]) + 4 * 4096
+    bad["child"][i] = -2
+    cases.append((bad, 8))
+    for arrays, bits in cases:
+        got = oracle.invariants(4096, m, arrays["count"], arrays["sorted"], arrays["child"], bb)
+        assert got & bits, (got, bits)
+        if oracle.ref_debug_available():
+            lines = oracle.ref_debug_run(np.zeros(m), np.zeros(m), np.zeros(m), arrays["count"],
+                                         np.zeros(m, np.int32), arrays["sorted"], arrays["child"],
+                                         bb, 4096, m)
+            assert any("FAIL" in ln for ln in lines)
+
+
+def test_coincident_bodies(oracle):
+    x = np.array([0.5, 0.5, 0.5, -1.0], np.float32)
+    y = np.array([0.25, 0.25, 0.25, 2.0], np.float32)
+    bb = oracle.bbox(x, y)
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.build_insert(x, y, np.ones(4, np.float32), bb)
+    assert e.value.status == oracle.OR_ERR_DEPTH
+    kt = oracle.build_keys(x, y, bb)
+    deep = [c for c in range(kt["ncells"]) if kt["level"][c] == oracle.MAX_LEVEL]
+    assert len(deep) == 1
+    c = deep[0]
+    assert list(kt["child"][4 * c: 4 * c + 4]) == [0, 1, 2, -1]  # by rank (id)
+    x5 = np.full(5, 0.5, np.float32)
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.build_keys(x5, x5, oracle.bbox(x5, x5))
+    assert e.value.status == oracle.OR_ERR_BUCKET
+
+
+def test_pool_overflow(oracle):
+    st = ic(0, 2000)
+    bb = oracle.bbox(st["x"], st["y"])
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.build_insert(st["x"], st["y"], st["mass"], bb, m=2000 + 10)
+    assert e.value.status == oracle.OR_ERR_POOL
+
+
+# ------------------------------------------------------ centre of mass ----
+def test_com_matches_range_sums(oracle):
+    st = ic(1, 20000)
+    x, y, m = st["x"], st["y"], st["mass"]
+    bb = oracle.bbox(x, y)
+    kt = oracle.build_keys(x, y, bb)
+    oracle.com_exact(kt, x, y, m)
+    s = kt["sorted"]
+    cm = np.cumsum(np.concatenate([[0.0], m[s].astype(np.float64)]))
+    cx = np.cumsum(np.concatenate([[0.0], (m[s].astype(np.float64) * x[s])]))
+    cy = np.cumsum(np.concatenate([[0.0], (m[s].astype(np.float64) * y[s])]))
+    a, c = kt["start"], kt["count"]
+    M = cm[a + c] - cm[a]
+    X = (cx[a + c] - cx[a]) / M
+    Y = (cy[a + c] - cy[a]) / M
+    ulp = lambda v: np.spacing(np.abs(v).astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(kt["com_x"] - X) <= ulp(X) + 1e-12 * np.abs(bb).max())
+    assert np.all(np.abs(kt["com_y"] - Y) <= ulp(Y) + 1e-12 * np.abs(bb).max())
+    assert np.all(np.abs(kt["com_m"] - M) <= ulp(M))
+
+
+def test_reference_fp32_com_close_to_exact(oracle):
+    st = ic(2, 20000)
+    bb, t, can, kt = _full_build(oracle, st["x"], st["y"], st["mass"])
+    oracle.com_exact(kt, st["x"], st["y"], st["mass"])
+    rel = np.abs(can["com_x"] - kt["com_x"]) / (np.abs(kt["com_x"]) + 1.0)
+    assert rel.max() < 1e-4
+    assert np.array_equal(can["com_m"], kt["com_m"])  # integer masses sum exactly
+
+
+# ----------------------------------------------------------------- walk ---
+def _tree(oracle, st):
+    bb = oracle.bbox(st["x"], st["y"])
+    kt = oracle.build_keys(st["x"], st["y"], bb)
+    oracle.com_exact(kt, st["x"], st["y"], st["mass"])
+    return bb, kt
+
+
+def test_walk_theta_to_zero_is_direct_sum(oracle):
+    st = ic(1, 3000)
+    bb, kt = _tree(oracle, st)
+    eps2 = 1e-3
+    ax, ay, it = oracle.walk(kt, st["x"], st["y"], st["mass"], bb, 1e-30, eps2, 32)
+    dx, dy = oracle.direct_fp64(st["x"], st["y"], st["mass"], eps2)
+    assert np.all(it == 3000)
+    rel = np.sqrt(((ax - dx) ** 2 + (ay - dy) ** 2) / (dx ** 2 + dy ** 2))
+    assert rel.max() < 2e-4
+
+
+def test_two_body_closed_form(oracle):
+    x = np.array([0.0, 3.0], np.float32)
+    y = np.array([0.0, 4.0], np.float32)
+    m = np.array([2.0, 5.0], np.float32)
+    eps2 = 0.25
+    st = {"x": x, "y": y, "mass": m}
+    bb, kt = _tree(oracle, st)
+    ax, ay, it = oracle.walk(kt, x, y, m, bb, 0.5, eps2, 32)
+    r3 = (25.0 + eps2) ** 1.5
+    assert np.allclose([ax[0], ay[0]], [5.0 * 3 / r3, 5.0 * 4 / r3], rtol=1e-6)
+    assert np.allclose([ax[1], ay[1]], [-2.0 * 3 / r3, -2.0 * 4 / r3], rtol=1e-6)
+    dx, dy = oracle.direct_fp64(x, y, m, eps2)
+    assert np.allclose(dx, [5.0 * 3 / r3, -2.0 * 3 / r3], rtol=1e-12)
+
+
+def test_symmetric_ring_cancels(oracle):
+    k = 16
+    ang = 2 * np.pi * np.arange(k) / k
+    x = np.concatenate([[0.0], 10 * np.cos(ang)]).astype(np.float32)
+    y = np.concatenate([[0.0], 10 * np.sin(ang)]).astype(np.float32)
+    m = np.ones(k + 1, np.float32)
+    dx, dy = oracle.direct_fp64(x, y, m, 0.01, targets=np.array([0], np.int32))
+    assert abs(dx[0]) < 1e-7 and abs(dy[0]) < 1e-7
+    st = {"x": x, "y": y, "mass": m}
+    bb, kt = _tree(oracle, st)
+    ax, ay, _ = oracle.walk(kt, x, y, m, bb, 1e-30, 0.01, 32)
+    assert abs(ax[0]) < 1e-6 and abs(ay[0]) < 1e-6
+
+
+@pytest.mark.parametrize("group", [1, 16, 32, 64])
+def test_walk_matches_recursive_statement(oracle, group):
+    st = ic(2, 20000)
+    bb, kt = _tree(oracle, st)
+    ax, ay, it = oracle.walk(kt, st["x"], st["y"], st["mass"], bb, 0.5, 1e-4, group)
+    rx, ry, rit = oracle.walk_recursive(kt, st["x"], st["y"], st["mass"], bb, 0.5, 1e-4, group)
+    assert np.array_equal(it, rit)
+    rel = np.sqrt(((ax - rx) ** 2 + (ay - ry) ** 2) / (rx ** 2 + ry ** 2))
+    assert np.sqrt(np.mean(rel ** 2)) < 5e-6
+
+
+def test_walk_error_falls_with_theta_and_group_is_conservative(oracle):
+    st = ic(2, 20000)
+    bb, kt = _tree(oracle, st)
+    x, y, m = st["x"], st["y"], st["mass"]
+    dx, dy = oracle.direct_fp64(x, y, m, 1e-4)
+    errs, counts = [], []
+    for th in (1.0, 0.5, 0.25):
+        ax, ay, it = oracle.walk(kt, x, y, m, bb, th, 1e-4, 32)
+        errs.append(np.sqrt(np.mean(((ax - dx) ** 2 + (ay - dy) ** 2) / (dx ** 2 + dy ** 2))))
+        counts.append(it.mean())
+    assert errs[0] > errs[1] > errs[2] and counts[0] < counts[1] < counts[2]
+    _, _, i1 = oracle.walk(kt, x, y, m, bb, 0.5, 1e-4, 1)
+    _, _, i64 = oracle.walk(kt, x, y, m, bb, 0.5, 1e-4, 64)
+    assert i1.mean() < counts[1] < i64.mean()
+
+
+# --------------------------------------------------------------- update ---
+def test_update_closed_form(oracle):
+    rng = np.random.default_rng(9)
+    a = [rng.normal(0, 1, 1000).astype(np.float32) for _ in range(6)]
+    dt, d = np.float32(1e-3), np.float32(0.9)
+    x, y, vx, vy = oracle.update(*a, dt, d)
+    ex_vx = a[2].astype(np.float64) + float(dt) * a[4]
+    ex_x = a[0].astype(np.float64) + float(np.float32(d * dt)) * vx.astype(np.float64)
+    assert np.all(np.abs(vx - ex_vx) <= np.spacing(np.abs(vx)))
+    assert np.all(np.abs(x - ex_x) <= np.spacing(np.abs(x)))
+
+
+def test_energy_two_body(oracle):
+    x = np.array([0.0, 3.0], np.float32)
+    y = np.array([0.0, 4.0], np.float32)
+    m = np.array([2.0, 5.0], np.float32)
+    vx = np.array([1.0, 0.0], np.float32)
+    vy = np.array([0.0, 2.0], np.float32)
+    e = oracle.energy(x, y, vx, vy, m, 0.0)
+    assert np.isclose(e["kinetic"], 0.5 * 2 * 1 + 0.5 * 5 * 4)
+    assert np.isclose(e["potential"], -10.0 / 5.0)
+    assert np.isclose(e["px"], 2.0) and np.isclose(e["py"], 10.0)
