@@ -71,6 +71,13 @@ typedef struct stellar_params {
 
 /* Record per-stage HIP event timings on every step (small overhead). */
 #define STELLAR_FLAG_STAGE_TIMING 1
+/* Accepted cells add their quadrupole term (not in the reference):
+ * a += -Q.d/r^5 + (5/2)(d.Q.d) d/r^7 with r^2 = |d|^2 + eps2, moments
+ * accumulated bottom-up in FP64. */
+#define STELLAR_FLAG_QUADRUPOLE 2
+/* Opening test w^2/theta^2 <= d^2+eps2, i.e. the textbook w/d <= theta of
+ * the reference's older, commented-out kernel (kernels.cu:488-489). */
+#define STELLAR_FLAG_TEXTBOOK_OPENING 4
 
 typedef struct stellar_ctx stellar_ctx;
 

@@ -431,16 +431,44 @@ void or_com_exact(int32_t n, int64_t ncells, const int32_t* child,
  *        f = m*rsqrt(r)^3, acc += f*(dx,dy)  (:407-411)
  *     else push (child, dp) (:413-419)
  * A body sitting alone in a root slot is accumulated here; the reference
- * pushes it and then walks its (empty) child slots, so it drops out. */
+ * pushes it and then walks its (empty) child slots, so it drops out.
+ *
+ * Extensions (not in the reference, off by default):
+ *   opening = 1: depth = radius^2/theta^2, i.e. the textbook w/d <= theta of
+ *     the reference's older, commented-out kernel (kernels.cu:488-489).
+ *   cq != NULL: accepted cells add their quadrupole term
+ *     a += -Q.d/r^5 + (5/2)(d.Q.d) d/r^7, r^2 = |d|^2 + eps2. */
 #define OR_STACK 256
 
-static inline float depth0_of(const float* bbox, float theta) {
+static inline float depth0_of(const float* bbox, float theta, int opening) {
   const float radius = (float)(0.5 * (double)(bbox[1] - bbox[0]));
   const float rr = radius * radius;
-  return rr / theta;
+  const float t = opening ? theta * theta : theta;
+  return rr / t;
 }
 
 static inline float rsqrt_ref(float r) { return (float)(1.0 / sqrt((double)r)); }
+
+/* FP32 accumulation of one accepted node for one body. */
+static inline void acc_node(float* fx, float* fy, float dx, float dy, float r,
+                            float cmv, const float* q) {
+  const float rs = rsqrt_ref(r);
+  const float f = cmv * rs * rs * rs;
+  if (!q) {
+    *fx = fmaf(f, dx, *fx);
+    *fy = fmaf(f, dy, *fy);
+    return;
+  }
+  const float rs2 = rs * rs;
+  const float rs5 = rs2 * rs2 * rs;
+  const float rs7 = rs5 * rs2;
+  const float qdx = q[0] * dx + q[1] * dy;
+  const float qdy = q[1] * dx + q[2] * dy;
+  const float dqd = dx * qdx + dy * qdy;
+  const float g = f + 2.5f * dqd * rs7;
+  *fx = fmaf(-rs5, qdx, fmaf(g, dx, *fx));
+  *fy = fmaf(-rs5, qdy, fmaf(g, dy, *fy));
+}
 
 int or_walk(int32_t n, const float* x, const float* y, const float* mass,
             const float* bbox, const int32_t* sorted, const int32_t* root,
@@ -451,7 +479,8 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
   const int32_t G = p->group;
   if (G <= 0 || G > 1024) return OR_ERR_ARG;
   const float eps2 = p->eps2;
-  const float depth0 = depth0_of(bbox, p->theta);
+  const float depth0 = depth0_of(bbox, p->theta, p->opening);
+  const float* cq = p->cq;
   const int32_t ngroups = (n + G - 1) / G;
   int err = OR_OK;
 #pragma omp parallel num_threads(nthreads(p->threads))
@@ -483,10 +512,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
           for (int32_t j = 0; j < gs; ++j) {
             const float dx = x[ch] - px[j], dy = y[ch] - py[j];
             const float r = dx * dx + dy * dy + eps2;
-            const float rs = rsqrt_ref(r);
-            const float f = mass[ch] * rs * rs * rs;
-            fx[j] = fmaf(f, dx, fx[j]);
-            fy[j] = fmaf(f, dy, fy[j]);
+            acc_node(&fx[j], &fy[j], dx, dy, r, mass[ch], NULL);
           }
           ++cnt;
         } else {
@@ -502,6 +528,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
           const int32_t ch = child[4 * (int64_t)(node - n) + k];
           if (ch < 0) continue;
           float cxv, cyv, cmv;
+          const float* q = NULL;
           if (ch < n) {
             cxv = x[ch];
             cyv = y[ch];
@@ -510,6 +537,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
             cxv = ccx[ch - n];
             cyv = ccy[ch - n];
             cmv = ccm[ch - n];
+            if (cq) q = cq + 3 * (int64_t)(ch - n);
           }
           int all = 1;
           for (int32_t j = 0; j < gs; ++j) {
@@ -520,10 +548,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
           if (ch < n || all) {
             for (int32_t j = 0; j < gs; ++j) {
               const float dx = cxv - px[j], dy = cyv - py[j];
-              const float rs = rsqrt_ref(rr[j]);
-              const float f = cmv * rs * rs * rs;
-              fx[j] = fmaf(f, dx, fx[j]);
-              fy[j] = fmaf(f, dy, fy[j]);
+              acc_node(&fx[j], &fy[j], dx, dy, rr[j], cmv, q);
             }
             ++cnt;
           } else {
@@ -553,7 +578,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
 /* Same acceptance rule stated recursively, accumulated in FP64. */
 typedef struct recctx {
   int32_t n, gs;
-  const float *x, *y, *mass, *ccx, *ccy, *ccm;
+  const float *x, *y, *mass, *ccx, *ccy, *ccm, *cq;
   const int32_t* child;
   float eps2;
   const float* px;
@@ -563,13 +588,23 @@ typedef struct recctx {
   int32_t cnt;
 } recctx;
 
-static void rec_acc(recctx* R, float cxv, float cyv, float cmv) {
+static void rec_acc(recctx* R, float cxv, float cyv, float cmv, const float* q) {
   for (int32_t j = 0; j < R->gs; ++j) {
     const float dx = cxv - R->px[j], dy = cyv - R->py[j];
     const float r = dx * dx + dy * dy + R->eps2;
     const double rs = 1.0 / sqrt((double)r);
-    R->fx[j] += (double)cmv * rs * rs * rs * (double)dx;
-    R->fy[j] += (double)cmv * rs * rs * rs * (double)dy;
+    const double X = dx, Y = dy;
+    double gx = (double)cmv * rs * rs * rs * X, gy = (double)cmv * rs * rs * rs * Y;
+    if (q) {
+      const double rs2 = rs * rs, rs5 = rs2 * rs2 * rs, rs7 = rs5 * rs2;
+      const double qdx = (double)q[0] * X + (double)q[1] * Y;
+      const double qdy = (double)q[1] * X + (double)q[2] * Y;
+      const double dqd = X * qdx + Y * qdy;
+      gx += -rs5 * qdx + 2.5 * dqd * rs7 * X;
+      gy += -rs5 * qdy + 2.5 * dqd * rs7 * Y;
+    }
+    R->fx[j] += gx;
+    R->fy[j] += gy;
   }
   R->cnt++;
 }
@@ -580,7 +615,7 @@ static void rec_open(recctx* R, int32_t node, float depth) {
     const int32_t ch = R->child[4 * (int64_t)(node - R->n) + k];
     if (ch < 0) continue;
     if (ch < R->n) {
-      rec_acc(R, R->x[ch], R->y[ch], R->mass[ch]);
+      rec_acc(R, R->x[ch], R->y[ch], R->mass[ch], NULL);
       continue;
     }
     const int32_t c = ch - R->n;
@@ -591,7 +626,7 @@ static void rec_open(recctx* R, int32_t node, float depth) {
       if (!(dp <= r)) all = 0;
     }
     if (all)
-      rec_acc(R, R->ccx[c], R->ccy[c], R->ccm[c]);
+      rec_acc(R, R->ccx[c], R->ccy[c], R->ccm[c], R->cq ? R->cq + 3 * (int64_t)c : NULL);
     else
       rec_open(R, ch, dp);
   }
@@ -607,7 +642,7 @@ int or_walk_recursive(int32_t n, const float* x, const float* y,
   (void)ncells;
   const int32_t G = p->group;
   if (G <= 0 || G > 1024) return OR_ERR_ARG;
-  const float depth0 = depth0_of(bbox, p->theta);
+  const float depth0 = depth0_of(bbox, p->theta, p->opening);
   const int32_t ngroups = (n + G - 1) / G;
 #pragma omp parallel num_threads(nthreads(p->threads))
   {
@@ -622,13 +657,13 @@ int or_walk_recursive(int32_t n, const float* x, const float* y,
         px[G + j] = y[sorted[s0 + j]];
         f[j] = f[G + j] = 0.0;
       }
-      recctx R = {n, gs, x, y, mass, ccx, ccy, ccm, child, p->eps2,
+      recctx R = {n, gs, x, y, mass, ccx, ccy, ccm, p->cq, child, p->eps2,
                   px, px + G, f, f + G, 0};
       for (int k = 0; k < 4; ++k) {
         const int32_t ch = root[k];
         if (ch < 0) continue;
         if (ch < n)
-          rec_acc(&R, x[ch], y[ch], mass[ch]);
+          rec_acc(&R, x[ch], y[ch], mass[ch], NULL);
         else
           rec_open(&R, ch, depth0);
       }
@@ -643,6 +678,71 @@ int or_walk_recursive(int32_t n, const float* x, const float* y,
     free(f);
   }
   return OR_OK;
+}
+
+/* Quadrupole moments, bottom-up in FP64 (see oracle.h). */
+void or_quad_exact(int32_t n, int64_t ncells, const int32_t* child,
+                   const float* x, const float* y, const float* mass,
+                   float* cq) {
+  double* S = (double*)malloc(sizeof(double) * 6 * (size_t)(ncells + 1));
+  for (int64_t c = ncells - 1; c >= 0; --c) {
+    double cm[4], cx[4], cy[4];
+    const double* qq[4];
+    double sm = 0.0, sx = 0.0, sy = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      const int32_t ch = child[4 * c + k];
+      qq[k] = NULL;
+      cm[k] = 0.0;
+      cx[k] = cy[k] = 0.0;
+      if (ch < 0) continue;
+      if (ch < n) {
+        cm[k] = (double)mass[ch];
+        cx[k] = (double)x[ch];
+        cy[k] = (double)y[ch];
+        sm += cm[k];
+        sx += cm[k] * cx[k];
+        sy += cm[k] * cy[k];
+      } else {
+        const double* q = S + 6 * (size_t)(ch - n);
+        cm[k] = q[0];
+        cx[k] = q[0] != 0.0 ? q[1] / q[0] : 0.0;
+        cy[k] = q[0] != 0.0 ? q[2] / q[0] : 0.0;
+        qq[k] = q + 3;
+        sm += q[0];
+        sx += q[1];
+        sy += q[2];
+      }
+    }
+    const double ox = sm != 0.0 ? sx / sm : 0.0, oy = sm != 0.0 ? sy / sm : 0.0;
+    double qxx = 0.0, qxy = 0.0, qyy = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      if (child[4 * c + k] < 0) continue;
+      const double dx = cx[k] - ox, dy = cy[k] - oy;
+      const double t1 = dx * dx, t2 = dy * dy;
+      double axx = cm[k] * (2.0 * t1 - t2);
+      double axy = cm[k] * (3.0 * (dx * dy));
+      double ayy = cm[k] * (2.0 * t2 - t1);
+      if (qq[k]) {
+        axx = qq[k][0] + axx;
+        axy = qq[k][1] + axy;
+        ayy = qq[k][2] + ayy;
+      }
+      qxx += axx;
+      qxy += axy;
+      qyy += ayy;
+    }
+    double* o = S + 6 * (size_t)c;
+    o[0] = sm;
+    o[1] = sx;
+    o[2] = sy;
+    o[3] = qxx;
+    o[4] = qxy;
+    o[5] = qyy;
+    cq[3 * c + 0] = (float)qxx;
+    cq[3 * c + 1] = (float)qxy;
+    cq[3 * c + 2] = (float)qyy;
+  }
+  free(S);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -788,7 +888,7 @@ int or_step(int32_t n, float* x, float* y, float* vx, float* vy,
   t[4] = now_s();
   {
     /* the walk reads the pool layout: cells at n.. with CoM in cx/cy/cm */
-    or_walk_params wp = {p->theta, p->eps2, p->group, p->threads};
+    or_walk_params wp = {p->theta, p->eps2, p->group, p->threads, 0, 0, NULL};
     rc = or_walk(n, x, y, mass, bbox, sorted, child, child + 4 * (size_t)n,
                  ncells, cx + n, cy + n, cm + n, &wp, ax, ay, NULL);
     if (rc) goto out;

@@ -83,6 +83,14 @@ void or_com_exact(int32_t n, int64_t ncells, const int32_t* child,
                   const float* x, const float* y, const float* mass,
                   float* ccx, float* ccy, float* ccm);
 
+/* Traceless quadrupole moments of every cell about its centre of mass
+ * (Qxx, Qxy, Qyy of sum m (3 d d^T - |d|^2 I) for points in the plane),
+ * accumulated bottom-up in FP64 in slot order with the parallel-axis shift
+ * from each child's FP64 centre of mass; written as FP32, 3 per cell. */
+void or_quad_exact(int32_t n, int64_t ncells, const int32_t* child,
+                   const float* x, const float* y, const float* mass,
+                   float* cq);
+
 /* compute_forces_kernel restated: group-of-G vote walk over a canonical tree.
  * Positions/mass are by body id; ax/ay/inter are written by body id. */
 typedef struct or_walk_params {
@@ -90,6 +98,9 @@ typedef struct or_walk_params {
   float eps2;
   int32_t group;     /* walk-group size (32 = reference warp) */
   int32_t threads;   /* OpenMP threads, <=0 = all */
+  int32_t opening;   /* 0: w^2/theta <= r (reference); 1: w^2/theta^2 <= r */
+  int32_t pad;
+  const float* cq;   /* NULL: monopole; else 3 quadrupole terms per cell */
 } or_walk_params;
 
 int or_walk(int32_t n, const float* x, const float* y, const float* mass,

@@ -33,7 +33,8 @@ _I64 = C.POINTER(C.c_int64)
 
 
 class WalkParams(C.Structure):
-    _fields_ = [("theta", C.c_float), ("eps2", C.c_float), ("group", C.c_int32), ("threads", C.c_int32)]
+    _fields_ = [("theta", C.c_float), ("eps2", C.c_float), ("group", C.c_int32), ("threads", C.c_int32),
+                ("opening", C.c_int32), ("pad", C.c_int32), ("cq", _F)]
 
 
 class StepParams(C.Structure):
@@ -50,6 +51,7 @@ _SIG = {
     "or_canonicalize": (C.c_int, [C.c_int32, C.c_int32, _I, _I, _I, _F, _F, _F, _I, _I, _I, _I, _I, _F, _F, _F]),
     "or_build_keys": (C.c_int, [C.c_int32, _F, _F, _F, C.c_int64, _I, _I, _I, _I, _I, _I, _I64]),
     "or_com_exact": (None, [C.c_int32, C.c_int64, _I, _F, _F, _F, _F, _F, _F]),
+    "or_quad_exact": (None, [C.c_int32, C.c_int64, _I, _F, _F, _F, _F]),
     "or_walk": (C.c_int, [C.c_int32, _F, _F, _F, _F, _I, _I, _I, C.c_int64, _F, _F, _F,
                           C.POINTER(WalkParams), _F, _F, _I]),
     "or_walk_recursive": (C.c_int, [C.c_int32, _F, _F, _F, _F, _I, _I, _I, C.c_int64, _F, _F, _F,
@@ -202,11 +204,29 @@ def com_exact(tree: dict, x, y, mass) -> None:
         tree[k] = tree[k][:c]
 
 
-def walk(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0):
+def quad_exact(tree: dict, x, y, mass) -> None:
+    """Quadrupole moments (3 per cell) into tree["quad"]."""
+    x, y, mass = f32(x), f32(y), f32(mass)
+    c = tree["ncells"]
+    q = np.zeros(3 * max(c, 1), np.float32)
+    lib().or_quad_exact(x.size, c, _i(tree["child"]), _f(x), _f(y), _f(mass), _f(q))
+    tree["quad"] = q[: 3 * c]
+
+
+def _wp(tree, theta, eps2, group, threads, opening, quad):
+    cq = None
+    if quad:
+        cq = tree["quad"] if tree["quad"].size else np.zeros(3, np.float32)
+    p = WalkParams(theta, eps2, group, threads, opening, 0, _f(cq))
+    return p, cq
+
+
+def walk(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0,
+         opening: int = 0, quad: bool = False):
     """Group-vote walk (kernels.cu:340-435) over a canonical tree; by body id."""
     x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
     n = x.size
-    p = WalkParams(theta, eps2, group, threads)
+    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad)
     ax = np.zeros(n, np.float32)
     ay = np.zeros(n, np.float32)
     it = np.zeros(n, np.int32)
@@ -222,10 +242,11 @@ def walk(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, thre
     return ax, ay, it
 
 
-def walk_recursive(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0):
+def walk_recursive(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0,
+                   opening: int = 0, quad: bool = False):
     x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
     n = x.size
-    p = WalkParams(theta, eps2, group, threads)
+    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad)
     ax = np.zeros(n, np.float64)
     ay = np.zeros(n, np.float64)
     it = np.zeros(n, np.int32)

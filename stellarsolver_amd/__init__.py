@@ -34,6 +34,8 @@ STELLAR_MAX_LEVEL = 64
 PRECISION_FP32 = 0
 PRECISION_MIXED = 1
 FLAG_STAGE_TIMING = 1
+FLAG_QUADRUPOLE = 2
+FLAG_TEXTBOOK_OPENING = 4
 
 IC_DISK = 0
 IC_PLUMMER = 1

@@ -118,6 +118,7 @@ __global__ void init_scalars_kernel(DevScalars* dev) {
   dev->bbox_ticket = 0;
   dev->err = 0;
   dev->ncells = 0;
+  dev->nspine = 0;
   dev->bbox[0] = dev->bbox[1] = dev->bbox[2] = dev->bbox[3] = 0.f;
   dev->interactions = 0;
 }
@@ -268,6 +269,7 @@ struct LinkArgs {
   uint8_t* clevel;
   uint8_t* cslot;
   uint8_t* ckc;
+  int32_t* cend;
   int32_t* arrive;
   DevScalars* dev;
 };
@@ -341,7 +343,9 @@ __device__ void link_node(const LinkArgs& A, int32_t blkid, int32_t a,
         rec[d] = ChildRec{A.x[p], A.y[p], A.m[p], -1 - u};
       } else {
         const int64_t c2 = (int64_t)A.coff[u] + (lev + 1 - lam_before(A, u) - 1);
-        if (c2 < A.cap) {
+        // a child cell always follows its parent in pre-order; anything else
+        // means the keys were left inconsistent by an earlier error
+        if (c2 < A.cap && c2 + 1 > blkid) {
           const int32_t b2 = (int32_t)(c2 + 1);
           rec[d] = ChildRec{0.f, 0.f, 0.f, b2};
           A.cparent[b2] = blkid;
@@ -356,6 +360,7 @@ __device__ void link_node(const LinkArgs& A, int32_t blkid, int32_t a,
 #pragma unroll
   for (int k = 0; k < 4; ++k) dst[k] = rec[k];
   A.ckc[blkid] = (uint8_t)kc;
+  A.cend[blkid] = A.coff[e];  // pairs s..e-1 open the subtree's cells
   A.arrive[blkid] = 0;
   A.cstart[blkid] = a;
   A.ccount[blkid] = e - a + 1;
@@ -372,6 +377,7 @@ __global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
     const int64_t total = (int64_t)A.coff[s] + A.ncell[s];
     if (total > A.cap) atomicOr(&A.dev->err, kErrPool);
     A.dev->ncells = (int32_t)(total < A.cap ? total : A.cap);
+    A.dev->nspine = 0;
     link_node(A, 0, 0, A.n - 1, 0);
     return;
   }
@@ -443,7 +449,7 @@ void launch_build(stellar_ctx* c) {
   LinkArgs A{n,        c->cap,    B.key,     B.key2,    B.coff,
              B.ncell,  B.perm,    B.x,       B.y,       B.m,
              B.blk,    B.cstart,  B.ccount,  B.cparent, B.clevel,
-             B.cslot,  B.ckc,     B.arrive,  B.dev};
+             B.cslot,  B.ckc,     B.cend,    B.arrive,  B.dev};
   link_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(A);
 }
 

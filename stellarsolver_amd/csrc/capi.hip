@@ -71,7 +71,8 @@ void free_all(stellar_ctx* c) {
                   B.ax,    B.ay,     B.inter,   B.key,    B.key_alt, B.perm,
                   B.perm_alt, B.key2, B.ncell,  B.coff,   B.tmp,    B.blk,
                   B.cstart, B.ccount, B.cparent, B.clevel, B.cslot, B.ckc,
-                  B.arrive, B.csum,  B.ccom,    B.out,    B.dev};
+                  B.arrive, B.csum,  B.ccom,    B.out,    B.dev,
+                  B.cend,  B.qblk,  B.spine,   B.ready};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (B.host_dev) (void)hipHostFree(B.host_dev);
@@ -239,7 +240,11 @@ int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
   chk(dmalloc(&B.cslot, C + 1));
   chk(dmalloc(&B.ckc, C + 1));
   chk(dmalloc(&B.arrive, C + 1));
-  chk(dmalloc(&B.csum, 3 * (C + 1)));
+  chk(dmalloc(&B.csum, kSumStride * (C + 1)));
+  chk(dmalloc(&B.cend, C + 1));
+  chk(dmalloc(&B.qblk, 4 * (C + 1)));
+  chk(dmalloc(&B.spine, C + 1));
+  chk(dmalloc(&B.ready, C + 1));
   chk(dmalloc(&B.ccom, C + 1));
   chk(dmalloc(&B.out, 2 * (N + C)));
   chk(dmalloc(&B.dev, 1));
@@ -258,6 +263,7 @@ int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
   // clear the output buffer and the cell records once
   chk(hipMemsetAsync(B.out, 0, 2 * (N + C) * sizeof(float), c->stream));
   chk(hipMemsetAsync(B.blk, 0, 4 * (C + 1) * sizeof(ChildRec), c->stream));
+  chk(hipMemsetAsync(B.qblk, 0, 4 * (C + 1) * sizeof(float4), c->stream));
   chk(hipMemsetAsync(B.ax, 0, N * sizeof(float), c->stream));
   chk(hipMemsetAsync(B.ay, 0, N * sizeof(float), c->stream));
   launch_init_scalars(c);

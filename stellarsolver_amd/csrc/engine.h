@@ -18,11 +18,15 @@ enum : uint32_t {
   kErrTieRun = 4u,  // more than kMaxTieRun bodies share a 32-level key
   kErrBucket = 8u,  // more than 4 bodies share a level-64 cell
 };
+// Errors after which the current tree must not be walked or summed.
+constexpr uint32_t kBuildErrors = kErrPool | kErrTieRun | kErrBucket;
 
 constexpr int kMaxLevel = STELLAR_MAX_LEVEL;
 constexpr int kMaxTieRun = 256;
 constexpr int kWalkStack = 256;  // >= 4 + 3*(kMaxLevel-1)
 constexpr int kWalkBlock = 256;  // threads per walk workgroup (4 wavefronts)
+// FP64 moments per cell: sum m, sum m x, sum m y, Qxx, Qxy, Qyy
+constexpr int kSumStride = 6;
 
 // Child record of a tree block: the walk's only view of the tree.  A block is
 // the four child slots of one node (block 0 = root, block 1+c = cell c),
@@ -41,7 +45,7 @@ struct DevScalars {
   uint32_t bbox_ticket;  // blocks finished in the current bbox launch
   uint32_t err;          // sticky error bits
   int32_t ncells;        // cells of the current tree
-  int32_t pad0;
+  int32_t nspine;        // cells left to the second centre-of-mass pass
   float bbox[4];         // left, right, bottom, top
   unsigned long long interactions;
   uint32_t pad[46];
@@ -69,9 +73,13 @@ struct Buffers {
   ChildRec* blk = nullptr;    // (1+cap) blocks of 4 records
   int32_t *cstart = nullptr, *ccount = nullptr, *cparent = nullptr;
   uint8_t *clevel = nullptr, *cslot = nullptr, *ckc = nullptr;
+  int32_t* cend = nullptr;    // exclusive end (cell id) of each subtree
   int32_t* arrive = nullptr;
-  double* csum = nullptr;     // 3 per cell: sum m, sum m x, sum m y
+  double* csum = nullptr;     // kSumStride FP64 moments per cell
   float4* ccom = nullptr;     // per cell: com x, com y, mass, 0
+  float4* qblk = nullptr;     // per child record: Qxx, Qxy, Qyy, 0
+  int32_t* spine = nullptr;   // cells finished by the second CoM pass
+  uint8_t* ready = nullptr;
   float* out = nullptr;       // 2*(n+cap) interleaved output buffer
   DevScalars* dev = nullptr;
   DevScalars* host_dev = nullptr;  // pinned mirror

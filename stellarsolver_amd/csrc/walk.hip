@@ -24,23 +24,40 @@
 // One deliberate difference: a body alone in a root slot is accumulated; the
 // reference pushes it like a cell and then finds no children (kernels.cu:
 // 373-387, 397-398), so that body exerts no force.
+//
+// QUAD adds each accepted cell's quadrupole term (a second 64-byte block of
+// Qxx, Qxy, Qyy per node); bodies stay monopoles.
 #include "engine.h"
 
 namespace stellar {
 
 namespace {
 
-template <int G>
-__global__ __launch_bounds__(kWalkBlock) void walk_kernel(
-    const float* __restrict__ x, const float* __restrict__ y,
-    const ChildRec* __restrict__ blk, int32_t n, float eps2, float theta,
-    DevScalars* __restrict__ dev, float* __restrict__ ax,
-    float* __restrict__ ay, int32_t* __restrict__ inter) {
+struct WalkArgs {
+  const float* x;
+  const float* y;
+  const ChildRec* blk;
+  const float4* qblk;
+  int32_t n;
+  float eps2;
+  float theta_div;  // theta (reference opening) or theta^2 (textbook)
+  DevScalars* dev;
+  float* ax;
+  float* ay;
+  int32_t* inter;
+};
+
+template <int G, bool QUAD>
+__global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs A) {
   constexpr int kGroupsPerWave = 64 / G;
   constexpr int kWaves = kWalkBlock / 64;
   __shared__ int32_t stk_node[kWaves * kGroupsPerWave][kWalkStack];
   __shared__ float stk_dep[kWaves * kGroupsPerWave][kWalkStack];
 
+  const ChildRec* __restrict__ blk = A.blk;
+  const float4* __restrict__ qblk = A.qblk;
+  const int32_t n = A.n;
+  const float eps2 = A.eps2;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int sub = lane / G;             // group within the wavefront
@@ -48,15 +65,16 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
   const int slot = wave * kGroupsPerWave + sub;
   const int64_t s0 = ((int64_t)blockIdx.x * kWaves + wave) * 64 + sub * G;
   if (((int64_t)blockIdx.x * kWaves + wave) * 64 >= n) return;
+  if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
   const int64_t s = s0 + glane;
   const bool valid = s < n;
-  const float px = valid ? x[s] : 0.f;
-  const float py = valid ? y[s] : 0.f;
+  const float px = valid ? A.x[s] : 0.f;
+  const float py = valid ? A.y[s] : 0.f;
 
   // depth = radius^2/theta with radius = half the x extent (kernels.cu:349,376)
-  const float bl = dev->bbox[0], br = dev->bbox[1];
+  const float bl = A.dev->bbox[0], br = A.dev->bbox[1];
   const float radius = (float)(0.5 * (double)(br - bl));
-  const float depth0 = (float)((double)(radius * radius) / (double)theta);
+  const float depth0 = (float)((double)(radius * radius) / (double)A.theta_div);
 
   const uint64_t gmask =
       G == 64 ? ~0ull : (((1ull << G) - 1ull) << (sub * G));
@@ -66,12 +84,25 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
   int sp = 0;
   bool overflow = false;
 
-  auto accumulate = [&](float cx, float cy, float cm, float r, float dx,
-                        float dy) {
+  auto accumulate = [&](float cm, float r, float dx, float dy) {
     const float rs = __builtin_amdgcn_rsqf(r);
     const float f = cm * rs * rs * rs;
     fx = __builtin_fmaf(f, dx, fx);
     fy = __builtin_fmaf(f, dy, fy);
+    ++cnt;
+  };
+  auto accumulate_q = [&](float cm, float r, float dx, float dy, float4 q) {
+    const float rs = __builtin_amdgcn_rsqf(r);
+    const float f = cm * rs * rs * rs;
+    const float rs2 = rs * rs;
+    const float rs5 = rs2 * rs2 * rs;
+    const float rs7 = rs5 * rs2;
+    const float qdx = q.x * dx + q.y * dy;
+    const float qdy = q.y * dx + q.z * dy;
+    const float dqd = dx * qdx + dy * qdy;
+    const float g = f + 2.5f * dqd * rs7;
+    fx = __builtin_fmaf(-rs5, qdx, __builtin_fmaf(g, dx, fx));
+    fy = __builtin_fmaf(-rs5, qdy, __builtin_fmaf(g, dy, fy));
     ++cnt;
   };
 
@@ -83,7 +114,7 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
     if (c.code < 0) {
       const float dx = c.x - px, dy = c.y - py;
       const float r = dx * dx + dy * dy + eps2;
-      accumulate(c.x, c.y, c.m, r, dx, dy);
+      accumulate(c.m, r, dx, dy);
     } else {
       if (glane == 0) {
         stk_node[slot][sp] = c.code;
@@ -102,12 +133,17 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
       depth = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(depth)));
     }
     const float dp = 0.25f * depth;
-    // the whole 64-byte block first (one scalar load for G = 64), then the
-    // four children from registers
+    // the whole 64-byte block first (scalar loads for G = 64), then the four
+    // children from registers
     const int4* B = reinterpret_cast<const int4*>(blk + 4 * (int64_t)node);
     int4 q[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) q[k] = B[k];
+    float4 qm[4];
+    if (QUAD) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) qm[k] = qblk[4 * (int64_t)node + k];
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const ChildRec c{__int_as_float(q[k].x), __int_as_float(q[k].y),
@@ -115,13 +151,16 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
       if (c.code == 0) continue;
       const float dx = c.x - px, dy = c.y - py;
       const float r = dx * dx + dy * dy + eps2;
-      bool take = c.code < 0;
-      if (!take) {
-        const uint64_t bad = __ballot(valid && !(dp <= r)) & gmask;
-        take = bad == 0;
+      if (c.code < 0) {
+        accumulate(c.m, r, dx, dy);
+        continue;
       }
-      if (take) {
-        accumulate(c.x, c.y, c.m, r, dx, dy);
+      const uint64_t bad = __ballot(valid && !(dp <= r)) & gmask;
+      if (bad == 0) {
+        if (QUAD)
+          accumulate_q(c.m, r, dx, dy, qm[k]);
+        else
+          accumulate(c.m, r, dx, dy);
       } else {
         if (sp >= kWalkStack) {
           overflow = true;
@@ -136,17 +175,25 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
     }
     if (overflow) break;
   }
-  if (overflow && glane == 0) atomicOr(&dev->err, kErrStack);
+  if (overflow && glane == 0) atomicOr(&A.dev->err, kErrStack);
   if (valid) {
-    ax[s] = fx;
-    ay[s] = fy;
-    inter[s] = cnt;
+    A.ax[s] = fx;
+    A.ay[s] = fy;
+    A.inter[s] = cnt;
   }
   if (glane == 0) {
     const int64_t members = (n - s0) < G ? (n - s0) : G;
-    atomicAdd(&dev->interactions,
+    atomicAdd(&A.dev->interactions,
               (unsigned long long)cnt * (unsigned long long)members);
   }
+}
+
+template <int G>
+void launch_g(stellar_ctx* c, unsigned grid, const WalkArgs& A) {
+  if (c->p.flags & STELLAR_FLAG_QUADRUPOLE)
+    walk_kernel<G, true><<<grid, kWalkBlock, 0, c->stream>>>(A);
+  else
+    walk_kernel<G, false><<<grid, kWalkBlock, 0, c->stream>>>(A);
 }
 
 }  // namespace
@@ -157,19 +204,19 @@ void launch_forces(stellar_ctx* c) {
   Buffers& B = c->b;
   const int64_t waves = (n + 63) / 64;
   const unsigned grid = (unsigned)((waves + (kWalkBlock / 64) - 1) / (kWalkBlock / 64));
+  const float theta = c->p.theta;
+  const float tdiv = (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? theta * theta : theta;
+  WalkArgs A{B.x, B.y, B.blk, B.qblk, n, c->p.eps2, tdiv, B.dev, B.ax, B.ay, B.inter};
   launch_zero_counter(c);
   switch (c->p.group) {
     case 32:
-      walk_kernel<32><<<grid, kWalkBlock, 0, c->stream>>>(
-          B.x, B.y, B.blk, n, c->p.eps2, c->p.theta, B.dev, B.ax, B.ay, B.inter);
+      launch_g<32>(c, grid, A);
       break;
     case 16:
-      walk_kernel<16><<<grid, kWalkBlock, 0, c->stream>>>(
-          B.x, B.y, B.blk, n, c->p.eps2, c->p.theta, B.dev, B.ax, B.ay, B.inter);
+      launch_g<16>(c, grid, A);
       break;
     default:
-      walk_kernel<64><<<grid, kWalkBlock, 0, c->stream>>>(
-          B.x, B.y, B.blk, n, c->p.eps2, c->p.theta, B.dev, B.ax, B.ay, B.inter);
+      launch_g<64>(c, grid, A);
       break;
   }
 }

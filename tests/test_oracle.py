@@ -334,3 +334,69 @@ def test_energy_two_body(oracle):
     assert np.isclose(e["kinetic"], 0.5 * 2 * 1 + 0.5 * 5 * 4)
     assert np.isclose(e["potential"], -10.0 / 5.0)
     assert np.isclose(e["px"], 2.0) and np.isclose(e["py"], 10.0)
+
+
+# ------------------------------------------------------------ quadrupole ---
+def test_quadrupole_matches_body_range_moments(oracle):
+    """Q of each cell = sum over its bodies of m (3 d d^T - |d|^2 I), d from
+    the cell's centre of mass, computed directly from the body range."""
+    st = ic(1, 5000)
+    x, y, m = st["x"], st["y"], st["mass"]
+    bb = oracle.bbox(x, y)
+    kt = oracle.build_keys(x, y, bb)
+    oracle.com_exact(kt, x, y, m)
+    oracle.quad_exact(kt, x, y, m)
+    s = kt["sorted"]
+    X, Y, M = x[s].astype(np.float64), y[s].astype(np.float64), m[s].astype(np.float64)
+    for c in range(0, kt["ncells"], 7):
+        a, k = kt["start"][c], kt["count"][c]
+        mm, xx, yy = M[a:a + k], X[a:a + k], Y[a:a + k]
+        ox, oy = (mm * xx).sum() / mm.sum(), (mm * yy).sum() / mm.sum()
+        dx, dy = xx - ox, yy - oy
+        q = np.array([(mm * (2 * dx * dx - dy * dy)).sum(), (mm * 3 * dx * dy).sum(),
+                      (mm * (2 * dy * dy - dx * dx)).sum()])
+        scale = (mm * (dx * dx + dy * dy)).sum() + 1e-300
+        assert np.all(np.abs(kt["quad"][3 * c: 3 * c + 3] - q) <= 1e-6 * scale), c
+
+
+def test_quadrupole_improves_far_field(oracle):
+    """A tight cluster seen from far away: the quadrupole-corrected cell term
+    is much closer to the direct sum than the monopole alone."""
+    rng = np.random.default_rng(4)
+    k = 200
+    x = np.concatenate([rng.normal(0, 1.0, k) * [1.0], [40.0]]).astype(np.float32)
+    y = np.concatenate([rng.normal(0, 0.3, k), [25.0]]).astype(np.float32)
+    m = np.concatenate([rng.uniform(0.5, 1.5, k), [1.0]]).astype(np.float32)
+    st = {"x": x, "y": y, "mass": m}
+    bb, kt = _tree(oracle, st)
+    oracle.quad_exact(kt, x, y, m)
+    tg = np.array([k], np.int32)
+    dx, dy = oracle.direct_fp64(x, y, m, 1e-4, targets=tg)
+    errs = []
+    for quad in (False, True):
+        ax, ay, it = oracle.walk(kt, x, y, m, bb, 1.0, 1e-4, 1, quad=quad)
+        errs.append(np.hypot(ax[k] - dx[0], ay[k] - dy[0]) / np.hypot(dx[0], dy[0]))
+        assert it[k] < 20  # the cluster is taken as a few far cells
+    assert errs[1] < 0.2 * errs[0]
+
+
+@pytest.mark.parametrize("opening", [0, 1])
+def test_quad_walk_matches_recursive_statement(oracle, opening):
+    st = ic(2, 20000)
+    bb, kt = _tree(oracle, st)
+    oracle.quad_exact(kt, st["x"], st["y"], st["mass"])
+    args = (kt, st["x"], st["y"], st["mass"], bb, 0.5, 1e-4, 64)
+    ax, ay, it = oracle.walk(*args, opening=opening, quad=True)
+    rx, ry, rit = oracle.walk_recursive(*args, opening=opening, quad=True)
+    assert np.array_equal(it, rit)
+    assert np.sqrt(np.mean(((ax - rx) ** 2 + (ay - ry) ** 2) / (rx ** 2 + ry ** 2))) < 5e-6
+
+
+def test_textbook_opening_is_theta_squared(oracle):
+    """w/d <= theta (textbook) equals the reference test at theta^2."""
+    st = ic(2, 20000)
+    bb, kt = _tree(oracle, st)
+    x, y, m = st["x"], st["y"], st["mass"]
+    _, _, a = oracle.walk(kt, x, y, m, bb, 0.5, 1e-4, 64, opening=1)
+    _, _, b = oracle.walk(kt, x, y, m, bb, 0.25, 1e-4, 64, opening=0)
+    assert np.array_equal(a, b)
