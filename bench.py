@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--group", type=int, default=64)
     ap.add_argument("--precision", choices=("fp32", "mixed"), default="fp32")
+    ap.add_argument("--quad", action="store_true", help="add cell quadrupole terms")
+    ap.add_argument("--textbook", action="store_true",
+                    help="opening test w/d <= theta instead of the reference's w^2/theta <= d^2")
     ap.add_argument("--error-sample", type=int, default=8192)
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3, help="timed oracle steps after 1 warm-up (0: skip)")
@@ -104,7 +107,9 @@ def main():
     st = S.generate_ic(S.ic_params(args.workload), n)
     prec = S.PRECISION_MIXED if args.precision == "mixed" else S.PRECISION_FP32
     p = S.default_params(theta=args.theta, eps2=eps2, timestep=args.dt, dampening=1.0,
-                         group=args.group, precision=prec, flags=S.FLAG_STAGE_TIMING)
+                         group=args.group, precision=prec,
+                         flags=S.FLAG_STAGE_TIMING | (S.FLAG_QUADRUPOLE if args.quad else 0)
+                         | (S.FLAG_TEXTBOOK_OPENING if args.textbook else 0))
     eng = S.Engine(n, p)
     eng.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
     e0 = None if args.no_energy else eng.energy_fp64()
@@ -164,7 +169,9 @@ def main():
         "config": {"workload": "colliding disks" if args.workload == 2 else f"workload {args.workload}",
                    "n_bodies": n, "theta": args.theta, "eps": args.eps, "dt": args.dt,
                    "group": args.group, "precision": args.precision, "seed": 2,
-                   "opening": "reference (w^2/theta <= d^2+eps^2, group vote)"},
+                   "opening": ("textbook (w/d <= theta" if args.textbook else
+                               "reference (w^2/theta <= d^2+eps^2") + ", group vote)",
+                   "multipoles": "monopole+quadrupole" if args.quad else "monopole"},
         "rms_rel_force_err": err_rms,
         "force_err_p99": err_p99,
         "force_err_max": err_max,

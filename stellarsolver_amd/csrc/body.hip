@@ -19,8 +19,10 @@ __global__ __launch_bounds__(256) void permute_kernel(
     const float* __restrict__ vy, const float* __restrict__ m,
     const int32_t* __restrict__ id, float* __restrict__ x2,
     float* __restrict__ y2, float* __restrict__ vx2, float* __restrict__ vy2,
-    float* __restrict__ m2, int32_t* __restrict__ id2) {
+    float* __restrict__ m2, int32_t* __restrict__ id2,
+    DevScalars* __restrict__ dev) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) dev->interactions = 0;  // for the walk that follows
   if (s >= n) return;
   const uint32_t p = perm[s];
   x2[s] = x[p];
@@ -216,7 +218,7 @@ void launch_sort(stellar_ctx* c) {
   Buffers& B = c->b;
   permute_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(
       n, B.perm, B.x, B.y, B.vx, B.vy, B.m, B.id, B.x2, B.y2, B.vx2, B.vy2,
-      B.m2, B.id2);
+      B.m2, B.id2, B.dev);
   std::swap(B.x, B.x2);
   std::swap(B.y, B.y2);
   std::swap(B.vx, B.vx2);

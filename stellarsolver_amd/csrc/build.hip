@@ -29,26 +29,19 @@ namespace stellar {
 namespace {
 
 constexpr int kBboxThreads = 256;
-constexpr int kBboxMaxBlocks = 1024;
-
-// Order-preserving map of floats to unsigned ints (for atomic min/max).
-__device__ __forceinline__ uint32_t ord_enc(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float ord_dec(uint32_t e) {
-  const uint32_t u = (e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e;
-  return __uint_as_float(u);
-}
 
 // min/max that never replace +0 by -0 and ignore NaN (the reference's
 // fminf/fmaxf over a box seeded with zeros, kernels.cu:51-58,62-114).
 __device__ __forceinline__ float mn(float a, float v) { return v < a ? v : a; }
 __device__ __forceinline__ float mx(float a, float v) { return v > a ? v : a; }
 
+// Pass 1: per-block min/max into part[blockIdx]; pass 2 (one block) folds
+// the partials.  Two launches instead of a last-block ticket: the ticket
+// pattern needs a device-scope release per block, which on gfx950 writes
+// back the issuing XCD's L2.
 __global__ __launch_bounds__(kBboxThreads) void bbox_kernel(
     const float* __restrict__ x, const float* __restrict__ y, int32_t n,
-    DevScalars* __restrict__ dev) {
+    float4* __restrict__ part) {
   float l = 0.f, r = 0.f, b = 0.f, t = 0.f;
   const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int32_t stride = gridDim.x * blockDim.x;
@@ -91,31 +84,51 @@ __global__ __launch_bounds__(kBboxThreads) void bbox_kernel(
       b = mn(b, red[2][k]);
       t = mx(t, red[3][k]);
     }
-    atomicMin(&dev->bbox_acc[0], ord_enc(l));
-    atomicMax(&dev->bbox_acc[1], ord_enc(r));
-    atomicMin(&dev->bbox_acc[2], ord_enc(b));
-    atomicMax(&dev->bbox_acc[3], ord_enc(t));
-    __threadfence();
-    const uint32_t ticket = atomicAdd(&dev->bbox_ticket, 1u);
-    if (ticket == gridDim.x - 1) {
-      // last block: publish the box and re-arm the accumulators at the
-      // origin for the next launch
-      __threadfence();
-      const uint32_t zero = ord_enc(0.f);
-      dev->bbox[0] = ord_dec(atomicExch(&dev->bbox_acc[0], zero));
-      dev->bbox[1] = ord_dec(atomicExch(&dev->bbox_acc[1], zero));
-      dev->bbox[2] = ord_dec(atomicExch(&dev->bbox_acc[2], zero));
-      dev->bbox[3] = ord_dec(atomicExch(&dev->bbox_acc[3], zero));
-      atomicExch(&dev->bbox_ticket, 0u);
+    part[blockIdx.x] = make_float4(l, r, b, t);
+  }
+}
+
+__global__ __launch_bounds__(kBboxThreads) void bbox_final_kernel(
+    const float4* __restrict__ part, int32_t nparts,
+    DevScalars* __restrict__ dev) {
+  float l = 0.f, r = 0.f, b = 0.f, t = 0.f;
+  for (int32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+    const float4 q = part[i];
+    l = mn(l, q.x);
+    r = mx(r, q.y);
+    b = mn(b, q.z);
+    t = mx(t, q.w);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    l = mn(l, __shfl_xor(l, off));
+    r = mx(r, __shfl_xor(r, off));
+    b = mn(b, __shfl_xor(b, off));
+    t = mx(t, __shfl_xor(t, off));
+  }
+  __shared__ float red[4][kBboxThreads / 64];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = l;
+    red[1][w] = r;
+    red[2][w] = b;
+    red[3][w] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBboxThreads / 64; ++k) {
+      l = mn(l, red[0][k]);
+      r = mx(r, red[1][k]);
+      b = mn(b, red[2][k]);
+      t = mx(t, red[3][k]);
     }
+    dev->bbox[0] = l;
+    dev->bbox[1] = r;
+    dev->bbox[2] = b;
+    dev->bbox[3] = t;
   }
 }
 
 __global__ void init_scalars_kernel(DevScalars* dev) {
-  const uint32_t zero = ord_enc(0.f);
-  dev->bbox_acc[0] = dev->bbox_acc[1] = dev->bbox_acc[2] = dev->bbox_acc[3] =
-      zero;
-  dev->bbox_ticket = 0;
   dev->err = 0;
   dev->ncells = 0;
   dev->nspine = 0;
@@ -418,11 +431,13 @@ void launch_init_scalars(stellar_ctx* c) {
 
 void launch_bbox(stellar_ctx* c) {
   const int32_t n = c->n;
-  int blocks = (int)grid_for((n + 3) / 4, kBboxThreads);
+  int blocks = (int)grid_for((n + 3) / 4, kBboxThreads * 4);
   if (blocks < 1) blocks = 1;
   if (blocks > kBboxMaxBlocks) blocks = kBboxMaxBlocks;
   bbox_kernel<<<blocks, kBboxThreads, 0, c->stream>>>(c->b.x, c->b.y, n,
-                                                       c->b.dev);
+                                                       c->b.bbox_part);
+  bbox_final_kernel<<<1, kBboxThreads, 0, c->stream>>>(c->b.bbox_part, blocks,
+                                                        c->b.dev);
 }
 
 void launch_build(stellar_ctx* c) {

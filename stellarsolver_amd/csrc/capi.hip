@@ -72,6 +72,7 @@ void free_all(stellar_ctx* c) {
                   B.perm_alt, B.key2, B.ncell,  B.coff,   B.tmp,    B.blk,
                   B.cstart, B.ccount, B.cparent, B.clevel, B.cslot, B.ckc,
                   B.arrive, B.csum,  B.ccom,    B.out,    B.dev,
+                  B.bbox_part, B.scount,
                   B.cend,  B.qblk,  B.spine,   B.ready};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -121,11 +122,11 @@ int run_step(stellar_ctx* c, int32_t step) {
   mark(c, ev, 3);
   launch_sort(c);
   mark(c, ev, 4);
-  launch_forces(c);
+  launch_forces(c, false);  // the sort stage zeroed the interaction counter
   mark(c, ev, 5);
   launch_update(c);
   mark(c, ev, 6);
-  launch_output(c);
+  // the output buffer is filled when it is read (stellar_download_positions)
   mark(c, ev, 7);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(c, e, "kernel launch");
@@ -243,11 +244,13 @@ int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
   chk(dmalloc(&B.csum, kSumStride * (C + 1)));
   chk(dmalloc(&B.cend, C + 1));
   chk(dmalloc(&B.qblk, 4 * (C + 1)));
-  chk(dmalloc(&B.spine, C + 1));
-  chk(dmalloc(&B.ready, C + 1));
+  chk(dmalloc(&B.spine, C / 64 + 2 * (size_t)kMaxLevel + 1024));
+  chk(dmalloc(&B.scount, C / 64 + 64));
+  chk(dmalloc(&B.ready, C / 64 + 2 * (size_t)kMaxLevel + 1024));
   chk(dmalloc(&B.ccom, C + 1));
   chk(dmalloc(&B.out, 2 * (N + C)));
   chk(dmalloc(&B.dev, 1));
+  chk(dmalloc(&B.bbox_part, kBboxMaxBlocks));
   chk(hipHostMalloc(reinterpret_cast<void**>(&B.host_dev), sizeof(DevScalars),
                     hipHostMallocDefault));
   for (int k = 0; k < 8; ++k) chk(hipEventCreate(&c->ev[k]));
@@ -359,9 +362,10 @@ int stellar_download(stellar_ctx* c, float* mass, float* x, float* y,
 
 int stellar_download_positions(stellar_ctx* c, float* out_xy) {
   if (!c || !out_xy) return STELLAR_ERR_ARG;
-  if (c->stage != kUpdateDone)
-    return fail(c, STELLAR_ERR_STATE, "output buffer is written by a step");
+  if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
   if (c->n == 0) return STELLAR_OK;
+  launch_output(c);
+  HIP_TRY(c, hipGetLastError());
   HIP_TRY(c, hipMemcpyAsync(out_xy, c->b.out, 2 * (size_t)c->n * sizeof(float),
                             hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -410,7 +414,7 @@ int stellar_forces(stellar_ctx* c) {
   if (!c) return STELLAR_ERR_ARG;
   if (c->stage != kSortDone && c->stage != kForcesDone)
     return fail(c, STELLAR_ERR_STATE, "forces need sort");
-  launch_forces(c);
+  launch_forces(c, true);
   HIP_TRY(c, hipGetLastError());
   c->stage = kForcesDone;
   return check_device(c);
@@ -420,7 +424,6 @@ int stellar_update(stellar_ctx* c) {
   if (!c) return STELLAR_ERR_ARG;
   if (c->stage != kForcesDone) return fail(c, STELLAR_ERR_STATE, "update needs forces");
   launch_update(c);
-  launch_output(c);
   HIP_TRY(c, hipGetLastError());
   c->stage = kUpdateDone;
   return check_device(c);

@@ -26,9 +26,12 @@ namespace stellar {
 
 namespace {
 
-constexpr int kRange = 1024;     // cells per pass-1 workgroup
-constexpr int kComThreads = 256;
+constexpr int kRange = 4096;      // cells per pass-1 workgroup
+constexpr int kComThreads = 1024;
 constexpr int kSpineThreads = 1024;
+// A range's leftover cells are ancestors of the first cell after the range,
+// so there are at most kMaxLevel of them: each range owns kMaxLevel slots.
+constexpr int kSpineSlots = kMaxLevel;
 
 struct ComArgs {
   ChildRec* blk;
@@ -40,7 +43,8 @@ struct ComArgs {
   int32_t* arrive;
   double* csum;  // kSumStride per block
   float4* ccom;
-  int32_t* spine;
+  int32_t* spine;   // kSpineSlots per range
+  int32_t* scount;  // leftover cells per range
   uint8_t* ready;
   DevScalars* dev;
   int quad;
@@ -125,16 +129,22 @@ __device__ void cell_moments(const ComArgs& A, int32_t b) {
 
 __global__ __launch_bounds__(kComThreads) void com_local_kernel(ComArgs A) {
   __shared__ int32_t larr[kRange];
+  __shared__ int32_t lspine;
   const int32_t ncells = A.dev->ncells;
   const int32_t a = blockIdx.x * kRange;  // cell ids [a, hi)
   if (a >= ncells) return;
   if (A.dev->err & kBuildErrors) return;  // no complete tree to sum
   const int32_t hi = (a + kRange < ncells) ? a + kRange : ncells;
   for (int i = threadIdx.x; i < kRange; i += blockDim.x) larr[i] = 0;
+  if (threadIdx.x == 0) lspine = 0;
   __syncthreads();
+  int32_t* myspine = A.spine + (int64_t)blockIdx.x * kSpineSlots;
   for (int32_t c = a + threadIdx.x; c < hi; c += blockDim.x) {
     const int32_t b = c + 1;
-    if (A.cend[b] > hi) A.spine[atomicAdd(&A.dev->nspine, 1)] = b;
+    if (A.cend[b] > hi) {
+      const int32_t k = atomicAdd(&lspine, 1);
+      if (k < kSpineSlots) myspine[k] = b;
+    }
     if (A.ckc[b] != 0) continue;
     int32_t cur = b;
     while (true) {
@@ -154,19 +164,27 @@ __global__ __launch_bounds__(kComThreads) void com_local_kernel(ComArgs A) {
       cur = p;
     }
   }
+  __syncthreads();
+  if (threadIdx.x == 0) A.scount[blockIdx.x] = lspine < kSpineSlots ? lspine : kSpineSlots;
 }
 
 __global__ __launch_bounds__(kSpineThreads) void com_spine_kernel(ComArgs A) {
   if (A.dev->err & kBuildErrors) return;
-  const int32_t ns = A.dev->nspine;
+  const int32_t nranges = (A.dev->ncells + kRange - 1) / kRange;
+  const int32_t nslots = nranges * kSpineSlots;
   // a spine cell is ready at the start iff all its cell children were
   // finished in pass 1; decide that before any pass-2 arrival happens
-  for (int32_t i = threadIdx.x; i < ns; i += blockDim.x) {
-    const int32_t b = A.spine[i];
-    A.ready[i] = A.arrive[b] == (int32_t)A.ckc[b] ? 1 : 0;
+  for (int32_t i = threadIdx.x; i < nslots; i += blockDim.x) {
+    const int32_t r = i / kSpineSlots, k = i % kSpineSlots;
+    uint8_t rd = 0;
+    if (k < A.scount[r]) {
+      const int32_t b = A.spine[i];
+      rd = A.arrive[b] == (int32_t)A.ckc[b] ? 1 : 0;
+    }
+    A.ready[i] = rd;
   }
   __syncthreads();
-  for (int32_t i = threadIdx.x; i < ns; i += blockDim.x) {
+  for (int32_t i = threadIdx.x; i < nslots; i += blockDim.x) {
     if (!A.ready[i]) continue;
     int32_t cur = A.spine[i];
     while (true) {
@@ -187,8 +205,9 @@ __global__ __launch_bounds__(kSpineThreads) void com_spine_kernel(ComArgs A) {
 void launch_com(stellar_ctx* c) {
   if (c->n == 0 || c->cap == 0) return;
   Buffers& B = c->b;
-  ComArgs A{B.blk,  B.qblk,  B.cparent, B.cslot, B.ckc,   B.cend,
-            B.arrive, B.csum, B.ccom,   B.spine, B.ready, B.dev,
+  ComArgs A{B.blk,    B.qblk,  B.cparent, B.cslot,  B.ckc,
+            B.cend,   B.arrive, B.csum,   B.ccom,   B.spine,
+            B.scount, B.ready, B.dev,
             (c->p.flags & STELLAR_FLAG_QUADRUPOLE) ? 1 : 0};
   const unsigned grid = (unsigned)((c->cap + kRange - 1) / kRange);
   com_local_kernel<<<grid, kComThreads, 0, c->stream>>>(A);

@@ -22,6 +22,7 @@ enum : uint32_t {
 constexpr uint32_t kBuildErrors = kErrPool | kErrTieRun | kErrBucket;
 
 constexpr int kMaxLevel = STELLAR_MAX_LEVEL;
+constexpr int kBboxMaxBlocks = 1024;
 constexpr int kMaxTieRun = 256;
 constexpr int kWalkStack = 256;  // >= 4 + 3*(kMaxLevel-1)
 constexpr int kWalkBlock = 256;  // threads per walk workgroup (4 wavefronts)
@@ -41,9 +42,8 @@ struct alignas(16) ChildRec {
 
 // Device scalars, one 256-byte line.
 struct DevScalars {
-  uint32_t bbox_acc[4];  // ordered-int min x, max x, min y, max y
-  uint32_t bbox_ticket;  // blocks finished in the current bbox launch
   uint32_t err;          // sticky error bits
+  uint32_t pad0;
   int32_t ncells;        // cells of the current tree
   int32_t nspine;        // cells left to the second centre-of-mass pass
   float bbox[4];         // left, right, bottom, top
@@ -79,8 +79,10 @@ struct Buffers {
   float4* ccom = nullptr;     // per cell: com x, com y, mass, 0
   float4* qblk = nullptr;     // per child record: Qxx, Qxy, Qyy, 0
   int32_t* spine = nullptr;   // cells finished by the second CoM pass
+  int32_t* scount = nullptr;  // their count per centre-of-mass range
   uint8_t* ready = nullptr;
   float* out = nullptr;       // 2*(n+cap) interleaved output buffer
+  float4* bbox_part = nullptr;  // per-block bounding-box partials
   DevScalars* dev = nullptr;
   DevScalars* host_dev = nullptr;  // pinned mirror
 };
@@ -122,7 +124,7 @@ void launch_bbox(stellar_ctx* c);
 void launch_build(stellar_ctx* c);
 void launch_com(stellar_ctx* c);
 void launch_sort(stellar_ctx* c);
-void launch_forces(stellar_ctx* c);
+void launch_forces(stellar_ctx* c, bool zero_counter);
 void launch_update(stellar_ctx* c);
 void launch_output(stellar_ctx* c);
 void launch_zero_counter(stellar_ctx* c);
