@@ -449,6 +449,13 @@ static inline float depth0_of(const float* bbox, float theta, int opening) {
 
 static inline float rsqrt_ref(float r) { return (float)(1.0 / sqrt((double)r)); }
 
+/* r = dx*dx + dy*dy + eps2 (kernels.cu:403-405).  The reference leaves the
+ * rounding to nvcc's FMA contraction; this build fixes one contraction,
+ * dx*dx + (dy*dy + eps2) as two fused multiply-adds, on both sides. */
+static inline float dist2(float dx, float dy, float eps2) {
+  return fmaf(dx, dx, fmaf(dy, dy, eps2));
+}
+
 /* FP32 accumulation of one accepted node for one body. */
 static inline void acc_node(float* fx, float* fy, float dx, float dy, float r,
                             float cmv, const float* q) {
@@ -511,7 +518,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
         if (ch < n) {
           for (int32_t j = 0; j < gs; ++j) {
             const float dx = x[ch] - px[j], dy = y[ch] - py[j];
-            const float r = dx * dx + dy * dy + eps2;
+            const float r = dist2(dx, dy, eps2);
             acc_node(&fx[j], &fy[j], dx, dy, r, mass[ch], NULL);
           }
           ++cnt;
@@ -542,7 +549,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
           int all = 1;
           for (int32_t j = 0; j < gs; ++j) {
             const float dx = cxv - px[j], dy = cyv - py[j];
-            rr[j] = dx * dx + dy * dy + eps2;
+            rr[j] = dist2(dx, dy, eps2);
             if (!(dp <= rr[j])) all = 0;
           }
           if (ch < n || all) {
@@ -591,7 +598,7 @@ typedef struct recctx {
 static void rec_acc(recctx* R, float cxv, float cyv, float cmv, const float* q) {
   for (int32_t j = 0; j < R->gs; ++j) {
     const float dx = cxv - R->px[j], dy = cyv - R->py[j];
-    const float r = dx * dx + dy * dy + R->eps2;
+    const float r = dist2(dx, dy, R->eps2);
     const double rs = 1.0 / sqrt((double)r);
     const double X = dx, Y = dy;
     double gx = (double)cmv * rs * rs * rs * X, gy = (double)cmv * rs * rs * rs * Y;
@@ -622,7 +629,7 @@ static void rec_open(recctx* R, int32_t node, float depth) {
     int all = 1;
     for (int32_t j = 0; j < R->gs && all; ++j) {
       const float dx = R->ccx[c] - R->px[j], dy = R->ccy[c] - R->py[j];
-      const float r = dx * dx + dy * dy + R->eps2;
+      const float r = dist2(dx, dy, R->eps2);
       if (!(dp <= r)) all = 0;
     }
     if (all)

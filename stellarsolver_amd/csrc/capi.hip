@@ -196,7 +196,8 @@ int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
   c->p = *p;
   c->n = n;
   c->cap = node_capacity > 0 ? node_capacity : 2 * (int64_t)n + 12000;
-  if (c->cap >= (int64_t)INT32_MAX - 1) {
+  // the walk's stack words keep a block index in 26 bits
+  if (c->cap + 1 >= kMaxBlocks) {
     delete c;
     return STELLAR_ERR_ARG;
   }

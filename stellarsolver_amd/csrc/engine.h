@@ -23,6 +23,7 @@ constexpr uint32_t kBuildErrors = kErrPool | kErrTieRun | kErrBucket;
 
 constexpr int kMaxLevel = STELLAR_MAX_LEVEL;
 constexpr int kBboxMaxBlocks = 1024;
+constexpr int64_t kMaxBlocks = int64_t(1) << 26;  // root + cells
 constexpr int kMaxTieRun = 256;
 constexpr int kWalkStack = 256;  // >= 4 + 3*(kMaxLevel-1)
 constexpr int kWalkBlock = 256;  // threads per walk workgroup (4 wavefronts)

@@ -1,29 +1,32 @@
 // Tree-walk force kernel for gfx950 (compute_forces_kernel, kernels.cu:340-435).
 //
 // A walk group is G consecutive bodies in tree order (the reference uses warps
-// of 32).  The group shares one traversal: for each child of a popped node it
+// of 32).  The group shares one traversal: for each child of a visited node it
 // evaluates r = dx*dx + dy*dy + eps2 per body and accepts the child when it is
 // a body or when every body of the group passes dp <= r (the reference's
-// __all vote); otherwise the child is pushed.  Accepted children add
+// __all vote); otherwise the child is opened.  Accepted children add
 // f = m*rsqrt(r)^3 * (dx, dy) to every body of the group.
 //
-// gfx950 mapping (G = 64, one wavefront per group):
-//   * the node index is wavefront-uniform, so a node's four child records
-//     (one 64-byte block) arrive through scalar loads;
-//   * the vote is one 64-lane ballot;
-//   * the traversal stack lives in LDS, one region per wavefront, written by
-//     lane 0 and read back as a uniform value;
-//   * bodies are stored in tree order, so the group's positions and the
-//     outputs are contiguous 256-byte rows.
-// G = 32 and 16 run 2 or 4 independent groups per wavefront (ballot masks
-// per group), which reproduces the reference's warp-of-32 vote exactly.
+// gfx950 mapping (G = 64, walk64_kernel, one wavefront per group):
+//   * node indices are wavefront-uniform, so a node's four child records (one
+//     64-byte block) arrive through scalar loads, and each iteration pops up
+//     to KPOP nodes so that one memory round trip serves several nodes;
+//   * the vote is one 64-lane ballot; lanes past the last body repeat the
+//     last body, so their votes change nothing and need no mask;
+//   * the stack (in LDS, one region per wavefront, written by lane 0) holds
+//     node index and level in one word; the threshold dp of a level comes
+//     from a per-lane table with one v_readlane;
+//   * bodies are stored in tree order: the group's positions and outputs are
+//     contiguous 256-byte rows.
+// G = 32 and 16 (walk_kernel) run 2 or 4 independent groups per wavefront
+// with per-group ballot masks, reproducing the reference's warp-of-32 vote.
 //
-// Traversal order matches the reference (children in slot order, accepted
-// ones accumulated at once, opened ones pushed and popped LIFO), so the set
-// of interactions and the summation order are those of the CPU restatement.
-// One deliberate difference: a body alone in a root slot is accumulated; the
-// reference pushes it like a cell and then finds no children (kernels.cu:
-// 373-387, 397-398), so that body exerts no force.
+// Which children are accepted depends only on the child and the group, never
+// on the visiting order, so every variant makes the same decisions as the CPU
+// restatement (bit-exact interaction counts); the order of summation differs.
+// One deliberate difference from the reference: a body alone in a root slot
+// is accumulated; the reference pushes it like a cell and then finds no
+// children (kernels.cu:373-387, 397-398), so that body exerts no force.
 //
 // QUAD adds each accepted cell's quadrupole term (a second 64-byte block of
 // Qxx, Qxy, Qyy per node); bodies stay monopoles.
@@ -47,51 +50,31 @@ struct WalkArgs {
   int32_t* inter;
 };
 
-template <int G, bool QUAD>
-__global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs A) {
-  constexpr int kGroupsPerWave = 64 / G;
-  constexpr int kWaves = kWalkBlock / 64;
-  __shared__ int32_t stk_node[kWaves * kGroupsPerWave][kWalkStack];
-  __shared__ float stk_dep[kWaves * kGroupsPerWave][kWalkStack];
+// r = dx*dx + dy*dy + eps2 as dx*dx + (dy*dy + eps2), two fused
+// multiply-adds (the CPU restatement evaluates the same form).
+__device__ __forceinline__ float dist2(float dx, float dy, float eps2) {
+  return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, eps2));
+}
 
-  const ChildRec* __restrict__ blk = A.blk;
-  const float4* __restrict__ qblk = A.qblk;
-  const int32_t n = A.n;
-  const float eps2 = A.eps2;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int sub = lane / G;             // group within the wavefront
-  const int glane = lane & (G - 1);     // lane within the group
-  const int slot = wave * kGroupsPerWave + sub;
-  const int64_t s0 = ((int64_t)blockIdx.x * kWaves + wave) * 64 + sub * G;
-  if (((int64_t)blockIdx.x * kWaves + wave) * 64 >= n) return;
-  if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
-  const int64_t s = s0 + glane;
-  const bool valid = s < n;
-  const float px = valid ? A.x[s] : 0.f;
-  const float py = valid ? A.y[s] : 0.f;
-
-  // depth = radius^2/theta with radius = half the x extent (kernels.cu:349,376)
+// depth = radius^2/theta with radius = half the x extent (kernels.cu:349,376)
+__device__ __forceinline__ float root_depth(const WalkArgs& A) {
   const float bl = A.dev->bbox[0], br = A.dev->bbox[1];
   const float radius = (float)(0.5 * (double)(br - bl));
-  const float depth0 = (float)((double)(radius * radius) / (double)A.theta_div);
+  return (float)((double)(radius * radius) / (double)A.theta_div);
+}
 
-  const uint64_t gmask =
-      G == 64 ? ~0ull : (((1ull << G) - 1ull) << (sub * G));
-
+struct Acc {
   float fx = 0.f, fy = 0.f;
   int32_t cnt = 0;
-  int sp = 0;
-  bool overflow = false;
-
-  auto accumulate = [&](float cm, float r, float dx, float dy) {
+  __device__ __forceinline__ void mono(float cm, float r, float dx, float dy) {
     const float rs = __builtin_amdgcn_rsqf(r);
     const float f = cm * rs * rs * rs;
     fx = __builtin_fmaf(f, dx, fx);
     fy = __builtin_fmaf(f, dy, fy);
     ++cnt;
-  };
-  auto accumulate_q = [&](float cm, float r, float dx, float dy, float4 q) {
+  }
+  __device__ __forceinline__ void quad(float cm, float r, float dx, float dy,
+                                       float4 q) {
     const float rs = __builtin_amdgcn_rsqf(r);
     const float f = cm * rs * rs * rs;
     const float rs2 = rs * rs;
@@ -104,105 +87,27 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs A) {
     fx = __builtin_fmaf(-rs5, qdx, __builtin_fmaf(g, dx, fx));
     fy = __builtin_fmaf(-rs5, qdy, __builtin_fmaf(g, dy, fy));
     ++cnt;
-  };
+  }
+};
 
-  // root block: level-1 cells are always opened (kernels.cu:370-388)
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const ChildRec c = blk[k];
-    if (c.code == 0) continue;
-    if (c.code < 0) {
-      const float dx = c.x - px, dy = c.y - py;
-      const float r = dx * dx + dy * dy + eps2;
-      accumulate(c.m, r, dx, dy);
-    } else {
-      if (glane == 0) {
-        stk_node[slot][sp] = c.code;
-        stk_dep[slot][sp] = depth0;
-      }
-      ++sp;
-    }
-  }
+// ---------------------------------------------------------------- G = 64 --
+// Stack word: block index in the low 26 bits, (level - 1) in the top 6.
+constexpr int kIdxBits = 26;
+constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
+// Words per wavefront.  A one-at-a-time walk never holds more than
+// 1 + 3*kMaxLevel; popping KPOP at once can hold more, so pushes past the
+// end are dropped and the walk stops with kErrStack.
+constexpr int kStack64 = 1024;
 
-  while (sp > 0) {
-    --sp;
-    int32_t node = stk_node[slot][sp];
-    float depth = stk_dep[slot][sp];
-    if (G == 64) {
-      node = __builtin_amdgcn_readfirstlane(node);
-      depth = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(depth)));
-    }
-    const float dp = 0.25f * depth;
-    // the whole 64-byte block first (scalar loads for G = 64), then the four
-    // children from registers
-    const int4* B = reinterpret_cast<const int4*>(blk + 4 * (int64_t)node);
-    int4 q[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) q[k] = B[k];
-    float4 qm[4];
-    if (QUAD) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) qm[k] = qblk[4 * (int64_t)node + k];
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const ChildRec c{__int_as_float(q[k].x), __int_as_float(q[k].y),
-                       __int_as_float(q[k].z), q[k].w};
-      if (c.code == 0) continue;
-      const float dx = c.x - px, dy = c.y - py;
-      const float r = dx * dx + dy * dy + eps2;
-      if (c.code < 0) {
-        accumulate(c.m, r, dx, dy);
-        continue;
-      }
-      const uint64_t bad = __ballot(valid && !(dp <= r)) & gmask;
-      if (bad == 0) {
-        if (QUAD)
-          accumulate_q(c.m, r, dx, dy, qm[k]);
-        else
-          accumulate(c.m, r, dx, dy);
-      } else {
-        if (sp >= kWalkStack) {
-          overflow = true;
-          break;
-        }
-        if (glane == 0) {
-          stk_node[slot][sp] = c.code;
-          stk_dep[slot][sp] = dp;
-        }
-        ++sp;
-      }
-    }
-    if (overflow) break;
-  }
-  if (overflow && glane == 0) atomicOr(&A.dev->err, kErrStack);
-  if (valid) {
-    A.ax[s] = fx;
-    A.ay[s] = fy;
-    A.inter[s] = cnt;
-  }
-  if (glane == 0) {
-    const int64_t members = (n - s0) < G ? (n - s0) : G;
-    atomicAdd(&A.dev->interactions,
-              (unsigned long long)cnt * (unsigned long long)members);
-  }
-}
-
-// G = 64 with several nodes in flight: each iteration pops up to KPOP stack
-// entries and issues all their block loads before using any, so one memory
-// round trip serves KPOP nodes.  The set of interactions does not depend on
-// the order nodes are visited (a node is visited iff every ancestor was
-// opened, and each decision depends only on the node and the group), so this
-// walk makes exactly the decisions of walk_kernel<64>; only the order in
-// which accepted terms are summed differs.
+// The tree arrays are separate const __restrict__ arguments: that is what
+// lets the compiler read them through the scalar data cache.
 template <bool QUAD, int KPOP>
-__global__ __launch_bounds__(kWalkBlock) void walk64_kernel(WalkArgs A) {
+__global__ __launch_bounds__(kWalkBlock) void walk64_kernel(
+    const ChildRec* __restrict__ blk, const float4* __restrict__ qblk,
+    WalkArgs A) {
   constexpr int kWaves = kWalkBlock / 64;
-  __shared__ int32_t stk_node[kWaves][kWalkStack];
-  __shared__ float stk_dep[kWaves][kWalkStack];
+  __shared__ uint32_t stk[kWaves][kStack64];
 
-  const ChildRec* __restrict__ blk = A.blk;
-  const float4* __restrict__ qblk = A.qblk;
   const int32_t n = A.n;
   const float eps2 = A.eps2;
   const int lane = threadIdx.x & 63;
@@ -212,40 +117,20 @@ __global__ __launch_bounds__(kWalkBlock) void walk64_kernel(WalkArgs A) {
   if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
   const int64_t s = s0 + lane;
   const bool valid = s < n;
-  const float px = valid ? A.x[s] : 0.f;
-  const float py = valid ? A.y[s] : 0.f;
-  const float bl = A.dev->bbox[0], br = A.dev->bbox[1];
-  const float radius = (float)(0.5 * (double)(br - bl));
-  const float depth0 = (float)((double)(radius * radius) / (double)A.theta_div);
-  int32_t* snode = stk_node[wave];
-  float* sdep = stk_dep[wave];
+  const int64_t sl = valid ? s : n - 1;  // padding lanes repeat the last body
+  const float px = A.x[sl];
+  const float py = A.y[sl];
 
-  float fx = 0.f, fy = 0.f;
-  int32_t cnt = 0;
+  // thr(level) = threshold for the children of a node at `level`:
+  // 0.25*depth(level), depth(1) = depth0, depth(l+1) = 0.25*depth(l)
+  // (kernels.cu:394-395), one level per lane.
+  const float depth0 = root_depth(A);
+  float thr = depth0;
+  for (int i = 0; i <= lane; ++i) thr = 0.25f * thr;
+
+  uint32_t* st = stk[wave];
   int sp = 0;
-  bool overflow = false;
-
-  auto accumulate = [&](float cm, float r, float dx, float dy) {
-    const float rs = __builtin_amdgcn_rsqf(r);
-    const float f = cm * rs * rs * rs;
-    fx = __builtin_fmaf(f, dx, fx);
-    fy = __builtin_fmaf(f, dy, fy);
-    ++cnt;
-  };
-  auto accumulate_q = [&](float cm, float r, float dx, float dy, float4 q) {
-    const float rs = __builtin_amdgcn_rsqf(r);
-    const float f = cm * rs * rs * rs;
-    const float rs2 = rs * rs;
-    const float rs5 = rs2 * rs2 * rs;
-    const float rs7 = rs5 * rs2;
-    const float qdx = q.x * dx + q.y * dy;
-    const float qdy = q.y * dx + q.z * dy;
-    const float dqd = dx * qdx + dy * qdy;
-    const float g = f + 2.5f * dqd * rs7;
-    fx = __builtin_fmaf(-rs5, qdx, __builtin_fmaf(g, dx, fx));
-    fy = __builtin_fmaf(-rs5, qdy, __builtin_fmaf(g, dy, fy));
-    ++cnt;
-  };
+  Acc acc;
 
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -253,13 +138,9 @@ __global__ __launch_bounds__(kWalkBlock) void walk64_kernel(WalkArgs A) {
     if (c.code == 0) continue;
     if (c.code < 0) {
       const float dx = c.x - px, dy = c.y - py;
-      const float r = dx * dx + dy * dy + eps2;
-      accumulate(c.m, r, dx, dy);
+      acc.mono(c.m, dist2(dx, dy, eps2), dx, dy);
     } else {
-      if (lane == 0) {
-        snode[sp] = c.code;
-        sdep[sp] = depth0;
-      }
+      if (lane == 0) st[sp] = (uint32_t)c.code;  // level 1
       ++sp;
     }
   }
@@ -267,14 +148,18 @@ __global__ __launch_bounds__(kWalkBlock) void walk64_kernel(WalkArgs A) {
   while (sp > 0) {
     const int take = sp < KPOP ? sp : KPOP;
     sp -= take;
-    int32_t node[KPOP];
+    uint32_t node[KPOP];
     float dp[KPOP];
+    uint32_t lvl[KPOP];
 #pragma unroll
     for (int j = 0; j < KPOP; ++j) {
       // entries beyond `take` repeat the first one: a harmless extra load
-      const int e = sp + (j < take ? j : 0);
-      node[j] = __builtin_amdgcn_readfirstlane(snode[e]);
-      dp[j] = 0.25f * __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(sdep[e])));
+      const uint32_t w =
+          __builtin_amdgcn_readfirstlane(st[sp + (j < take ? j : 0)]);
+      node[j] = w & kIdxMask;
+      lvl[j] = w >> kIdxBits;  // level - 1
+      dp[j] = __int_as_float(
+          __builtin_amdgcn_readlane(__float_as_int(thr), (int)lvl[j]));
     }
     int4 q[KPOP][4];
     float4 qm[QUAD ? KPOP : 1][4];
@@ -291,50 +176,149 @@ __global__ __launch_bounds__(kWalkBlock) void walk64_kernel(WalkArgs A) {
 #pragma unroll
     for (int jj = KPOP - 1; jj >= 0; --jj) {
       if (jj >= take) continue;
+      const uint32_t child_lvl = (lvl[jj] + 1u) << kIdxBits;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int32_t code = q[jj][k].w;
         if (code == 0) continue;
-        const float cxv = __int_as_float(q[jj][k].x), cyv = __int_as_float(q[jj][k].y);
+        const float dx = __int_as_float(q[jj][k].x) - px;
+        const float dy = __int_as_float(q[jj][k].y) - py;
         const float cmv = __int_as_float(q[jj][k].z);
-        const float dx = cxv - px, dy = cyv - py;
-        const float r = dx * dx + dy * dy + eps2;
-        if (code < 0) {
-          accumulate(cmv, r, dx, dy);
-          continue;
-        }
-        const uint64_t bad = __ballot(valid && !(dp[jj] <= r));
-        if (bad == 0) {
-          if (QUAD)
-            accumulate_q(cmv, r, dx, dy, qm[QUAD ? jj : 0][k]);
+        const float r = dist2(dx, dy, eps2);
+        // one accumulate site for bodies and accepted cells (the vote is
+        // evaluated for bodies too; it costs one compare)
+        const uint64_t bad = __builtin_amdgcn_ballot_w64(!(dp[jj] <= r));
+        if (code < 0 || bad == 0) {
+          if (QUAD && code > 0)
+            acc.quad(cmv, r, dx, dy, qm[QUAD ? jj : 0][k]);
           else
-            accumulate(cmv, r, dx, dy);
+            acc.mono(cmv, r, dx, dy);
         } else {
-          if (sp >= kWalkStack) {
-            overflow = true;
-            break;
-          }
-          if (lane == 0) {
-            snode[sp] = code;
-            sdep[sp] = dp[jj];
-          }
+          if (lane == 0 && sp < kStack64) st[sp] = (uint32_t)code | child_lvl;
           ++sp;
         }
       }
-      if (overflow) break;
     }
-    if (overflow) break;
+    if (sp > kStack64) {
+      if (lane == 0) atomicOr(&A.dev->err, kErrStack);
+      break;
+    }
   }
-  if (overflow && lane == 0) atomicOr(&A.dev->err, kErrStack);
   if (valid) {
-    A.ax[s] = fx;
-    A.ay[s] = fy;
-    A.inter[s] = cnt;
+    A.ax[s] = acc.fx;
+    A.ay[s] = acc.fy;
+    A.inter[s] = acc.cnt;
   }
   if (lane == 0) {
     const int64_t members = (n - s0) < 64 ? (n - s0) : 64;
     atomicAdd(&A.dev->interactions,
-              (unsigned long long)cnt * (unsigned long long)members);
+              (unsigned long long)acc.cnt * (unsigned long long)members);
+  }
+}
+
+// ----------------------------------------------------------- G = 32, 16 --
+template <int G, bool QUAD>
+__global__ __launch_bounds__(kWalkBlock) void walk_kernel(
+    const ChildRec* __restrict__ blk, const float4* __restrict__ qblk,
+    WalkArgs A) {
+  constexpr int kGroupsPerWave = 64 / G;
+  constexpr int kWaves = kWalkBlock / 64;
+  __shared__ int32_t stk_node[kWaves * kGroupsPerWave][kWalkStack];
+  __shared__ float stk_dep[kWaves * kGroupsPerWave][kWalkStack];
+
+  const int32_t n = A.n;
+  const float eps2 = A.eps2;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int sub = lane / G;             // group within the wavefront
+  const int glane = lane & (G - 1);     // lane within the group
+  const int slot = wave * kGroupsPerWave + sub;
+  const int64_t s0 = ((int64_t)blockIdx.x * kWaves + wave) * 64 + sub * G;
+  if (((int64_t)blockIdx.x * kWaves + wave) * 64 >= n) return;
+  if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
+  const int64_t s = s0 + glane;
+  const bool valid = s < n;
+  const float px = valid ? A.x[s] : 0.f;
+  const float py = valid ? A.y[s] : 0.f;
+  const float depth0 = root_depth(A);
+  const uint64_t gmask = ((G == 64) ? ~0ull : ((1ull << G) - 1ull)) << (sub * G);
+
+  Acc acc;
+  int sp = 0;
+  bool overflow = false;
+
+  // root block: level-1 cells are always opened (kernels.cu:370-388)
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const ChildRec c = blk[k];
+    if (c.code == 0) continue;
+    if (c.code < 0) {
+      const float dx = c.x - px, dy = c.y - py;
+      acc.mono(c.m, dist2(dx, dy, eps2), dx, dy);
+    } else {
+      if (glane == 0) {
+        stk_node[slot][sp] = c.code;
+        stk_dep[slot][sp] = depth0;
+      }
+      ++sp;
+    }
+  }
+
+  while (sp > 0) {
+    --sp;
+    const int32_t node = stk_node[slot][sp];
+    const float dp = 0.25f * stk_dep[slot][sp];
+    const int4* B = reinterpret_cast<const int4*>(blk + 4 * (int64_t)node);
+    int4 q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = B[k];
+    float4 qm[4];
+    if (QUAD) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) qm[k] = qblk[4 * (int64_t)node + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int32_t code = q[k].w;
+      if (code == 0) continue;
+      const float dx = __int_as_float(q[k].x) - px;
+      const float dy = __int_as_float(q[k].y) - py;
+      const float cmv = __int_as_float(q[k].z);
+      const float r = dist2(dx, dy, eps2);
+      if (code < 0) {
+        acc.mono(cmv, r, dx, dy);
+        continue;
+      }
+      const uint64_t bad = __ballot(valid && !(dp <= r)) & gmask;
+      if (bad == 0) {
+        if (QUAD)
+          acc.quad(cmv, r, dx, dy, qm[k]);
+        else
+          acc.mono(cmv, r, dx, dy);
+      } else {
+        if (sp >= kWalkStack) {
+          overflow = true;
+          break;
+        }
+        if (glane == 0) {
+          stk_node[slot][sp] = code;
+          stk_dep[slot][sp] = dp;
+        }
+        ++sp;
+      }
+    }
+    if (overflow) break;
+  }
+  if (overflow && glane == 0) atomicOr(&A.dev->err, kErrStack);
+  if (valid) {
+    A.ax[s] = acc.fx;
+    A.ay[s] = acc.fy;
+    A.inter[s] = acc.cnt;
+  }
+  if (glane == 0) {
+    const int64_t members = (n - s0) < G ? (n - s0) : G;
+    atomicAdd(&A.dev->interactions,
+              (unsigned long long)acc.cnt * (unsigned long long)members);
   }
 }
 
@@ -343,15 +327,15 @@ void launch_g(stellar_ctx* c, unsigned grid, const WalkArgs& A) {
   const bool quad = (c->p.flags & STELLAR_FLAG_QUADRUPOLE) != 0;
   if (G == 64) {
     if (quad)
-      walk64_kernel<true, 2><<<grid, kWalkBlock, 0, c->stream>>>(A);
+      walk64_kernel<true, 2><<<grid, kWalkBlock, 0, c->stream>>>(A.blk, A.qblk, A);
     else
-      walk64_kernel<false, 4><<<grid, kWalkBlock, 0, c->stream>>>(A);
+      walk64_kernel<false, 4><<<grid, kWalkBlock, 0, c->stream>>>(A.blk, A.qblk, A);
     return;
   }
   if (quad)
-    walk_kernel<G, true><<<grid, kWalkBlock, 0, c->stream>>>(A);
+    walk_kernel<G, true><<<grid, kWalkBlock, 0, c->stream>>>(A.blk, A.qblk, A);
   else
-    walk_kernel<G, false><<<grid, kWalkBlock, 0, c->stream>>>(A);
+    walk_kernel<G, false><<<grid, kWalkBlock, 0, c->stream>>>(A.blk, A.qblk, A);
 }
 
 }  // namespace
