@@ -131,7 +131,6 @@ __global__ __launch_bounds__(kBboxThreads) void bbox_final_kernel(
 __global__ void init_scalars_kernel(DevScalars* dev) {
   dev->err = 0;
   dev->ncells = 0;
-  dev->nspine = 0;
   dev->bbox[0] = dev->bbox[1] = dev->bbox[2] = dev->bbox[3] = 0.f;
   dev->interactions = 0;
 }
@@ -281,9 +280,7 @@ struct LinkArgs {
   int32_t* cparent;
   uint8_t* clevel;
   uint8_t* cslot;
-  uint8_t* ckc;
   int32_t* cend;
-  int32_t* arrive;
   DevScalars* dev;
 };
 
@@ -335,7 +332,6 @@ __device__ void link_node(const LinkArgs& A, int32_t blkid, int32_t a,
   ChildRec rec[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) rec[k] = ChildRec{0.f, 0.f, 0.f, 0};
-  int kc = 0;
   if (lev == kMaxLevel) {
     // deepest level: keep up to four bodies by rank
     if (e - a + 1 > 4) {
@@ -363,7 +359,6 @@ __device__ void link_node(const LinkArgs& A, int32_t blkid, int32_t a,
           rec[d] = ChildRec{0.f, 0.f, 0.f, b2};
           A.cparent[b2] = blkid;
           A.cslot[b2] = (uint8_t)d;
-          ++kc;
         }
       }
       u = v + 1;
@@ -372,9 +367,7 @@ __device__ void link_node(const LinkArgs& A, int32_t blkid, int32_t a,
   ChildRec* dst = A.blk + 4 * (int64_t)blkid;
 #pragma unroll
   for (int k = 0; k < 4; ++k) dst[k] = rec[k];
-  A.ckc[blkid] = (uint8_t)kc;
   A.cend[blkid] = A.coff[e];  // pairs s..e-1 open the subtree's cells
-  A.arrive[blkid] = 0;
   A.cstart[blkid] = a;
   A.ccount[blkid] = e - a + 1;
   A.clevel[blkid] = (uint8_t)lev;
@@ -390,7 +383,6 @@ __global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
     const int64_t total = (int64_t)A.coff[s] + A.ncell[s];
     if (total > A.cap) atomicOr(&A.dev->err, kErrPool);
     A.dev->ncells = (int32_t)(total < A.cap ? total : A.cap);
-    A.dev->nspine = 0;
     link_node(A, 0, 0, A.n - 1, 0);
     return;
   }
@@ -464,7 +456,7 @@ void launch_build(stellar_ctx* c) {
   LinkArgs A{n,        c->cap,    B.key,     B.key2,    B.coff,
              B.ncell,  B.perm,    B.x,       B.y,       B.m,
              B.blk,    B.cstart,  B.ccount,  B.cparent, B.clevel,
-             B.cslot,  B.ckc,     B.cend,    B.arrive,  B.dev};
+             B.cslot,  B.cend,    B.dev};
   link_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(A);
 }
 

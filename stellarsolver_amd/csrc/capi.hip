@@ -70,10 +70,10 @@ void free_all(stellar_ctx* c) {
                   B.x2,    B.y2,     B.vx2,     B.vy2,    B.m2,     B.id2,
                   B.ax,    B.ay,     B.inter,   B.key,    B.key_alt, B.perm,
                   B.perm_alt, B.key2, B.ncell,  B.coff,   B.tmp,    B.blk,
-                  B.cstart, B.ccount, B.cparent, B.clevel, B.cslot, B.ckc,
-                  B.arrive, B.csum,  B.ccom,    B.out,    B.dev,
+                  B.cstart, B.ccount, B.cparent, B.clevel, B.cslot,
+                  B.csum,  B.ccom,    B.out,    B.dev,
                   B.bbox_part, B.scount,
-                  B.cend,  B.qblk,  B.spine,   B.ready};
+                  B.cend,  B.qblk,  B.spine,   B.clist, B.clen};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (B.host_dev) (void)hipHostFree(B.host_dev);
@@ -240,14 +240,14 @@ int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
   chk(dmalloc(&B.cparent, C + 1));
   chk(dmalloc(&B.clevel, C + 1));
   chk(dmalloc(&B.cslot, C + 1));
-  chk(dmalloc(&B.ckc, C + 1));
-  chk(dmalloc(&B.arrive, C + 1));
   chk(dmalloc(&B.csum, kSumStride * (C + 1)));
   chk(dmalloc(&B.cend, C + 1));
   chk(dmalloc(&B.qblk, 4 * (C + 1)));
-  chk(dmalloc(&B.spine, C / 64 + 2 * (size_t)kMaxLevel + 1024));
-  chk(dmalloc(&B.scount, C / 64 + 64));
-  chk(dmalloc(&B.ready, C / 64 + 2 * (size_t)kMaxLevel + 1024));
+  const size_t clcap = (size_t)com_list_capacity(c->cap);
+  chk(dmalloc(&B.spine, clcap));
+  chk(dmalloc(&B.scount, clcap / kMaxLevel + 64));
+  chk(dmalloc(&B.clist, 2 * clcap));
+  chk(dmalloc(&B.clen, 64));
   chk(dmalloc(&B.ccom, C + 1));
   chk(dmalloc(&B.out, 2 * (N + C)));
   chk(dmalloc(&B.dev, 1));

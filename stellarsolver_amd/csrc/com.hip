@@ -6,84 +6,90 @@
 // of the atomics, and so the rounding, changes from run to run, and every
 // level-1 cell takes about N/4 atomics.
 //
-// Here each cell is summed once from its four child records, in slot order,
-// in FP64, and the FP32 centre of mass is the FP64 quotient rounded once, so
-// the result does not depend on scheduling.
+// Here each cell is summed once from its four children, in slot order, in
+// FP64, and the FP32 centre of mass is the FP64 quotient rounded once, so the
+// result does not depend on scheduling.
 //
-// Scheduling without device-wide fences: on gfx950 a device-scope release
-// writes back the issuing XCD's whole L2, so a per-cell fence costs
-// milliseconds over a million cells.  Cells are numbered in DFS pre-order, so
-// every subtree is a contiguous id range [c, cend[c]).  Pass 1 gives each
-// workgroup a range of kRange cell ids and finishes every cell whose subtree
-// lies inside that range, with LDS arrival counters and workgroup-scope
-// ordering only.  The cells left over are ancestors of a range boundary
-// (at most one root-to-boundary path per range); pass 2 finishes them in a
-// single workgroup, again with workgroup-scope ordering only.  Results of
-// pass 1 reach pass 2 through the kernel boundary.
+// Scheduling.  Cells are numbered in DFS pre-order, so a cell's subtree is the
+// contiguous id range [c, cend[c]) and its children have larger ids.  A pass
+// takes a sorted list of unfinished cells in chunks of kChunk; a workgroup
+// stages its chunk's child records and child moments in LDS and finishes
+// every cell whose unfinished descendants all lie in the chunk, following
+// dependency chains with LDS arrival counters (no device-scope fences: on
+// gfx950 each one writes back the issuing XCD's L2).  The cells it cannot
+// finish are ancestors of the first cell after the chunk -- at most one per
+// level -- and form the next pass's list.  Pass 1 runs over all cells; after
+// a few passes the list is short and one workgroup finishes it, walking its
+// chunks from the last to the first so that every descendant is done before
+// its ancestors.
 #include "engine.h"
 
 namespace stellar {
 
 namespace {
 
-constexpr int kRange = 4096;      // cells per pass-1 workgroup
-constexpr int kComThreads = 1024;
-constexpr int kSpineThreads = 1024;
-// A range's leftover cells are ancestors of the first cell after the range,
-// so there are at most kMaxLevel of them: each range owns kMaxLevel slots.
-constexpr int kSpineSlots = kMaxLevel;
+constexpr int kChunk = 256;          // cells per workgroup and chunk
+constexpr int kComThreads = 256;
+constexpr int kSlots = kMaxLevel;    // leftover cells per chunk (one per level)
+constexpr int kMom = kSumStride;     // FP64 moments per child slot
 
 struct ComArgs {
   ChildRec* blk;
   float4* qblk;
   const int32_t* cparent;
   const uint8_t* cslot;
-  const uint8_t* ckc;
   const int32_t* cend;
-  int32_t* arrive;
   double* csum;  // kSumStride per block
   float4* ccom;
-  int32_t* spine;   // kSpineSlots per range
-  int32_t* scount;  // leftover cells per range
-  uint8_t* ready;
   DevScalars* dev;
   int quad;
 };
 
-// Moments of block b from its four child records, in slot order.  Writes the
-// FP64 sums, the FP32 centre of mass and the parent's child record.
-__device__ void cell_moments(const ComArgs& A, int32_t b) {
-  const ChildRec* rec = A.blk + 4 * (int64_t)b;
-  double cm[4], cx[4], cy[4];
-  const double* qq[4];
+struct ChunkLds {
+  int32_t ids[kChunk];          // block ids, ascending
+  int32_t pidx[kChunk];         // chunk index of the parent, -1 if none
+  int32_t pend[kChunk];         // unfinished children inside the chunk
+  uint8_t local[kChunk];        // every unfinished descendant is in the chunk
+  ChildRec rec[kChunk][4];
+  double mom[kChunk][4][kMom];  // child moments: body (m, x, y), cell sums
+  int32_t nout;
+};
+
+// Index of block id b among ids[lo, hi), or -1.
+__device__ __forceinline__ int find_id(const int32_t* ids, int lo, int hi, int32_t b) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const int32_t v = ids[mid];
+    if (v == b) return mid;
+    if (v < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return -1;
+}
+
+// Moments of chunk entry i from its staged child slots (slot order, FP64),
+// written to global memory and returned in out[kMom].
+__device__ void finish_cell(const ComArgs& A, ChunkLds& L, int i, double* out) {
+  const int32_t b = L.ids[i];
   double sm = 0.0, sx = 0.0, sy = 0.0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const ChildRec r = rec[k];
-    qq[k] = nullptr;
-    cm[k] = cx[k] = cy[k] = 0.0;
-    if (r.code == 0) continue;
-    if (r.code < 0) {
-      cm[k] = (double)r.m;
-      cx[k] = (double)r.x;
-      cy[k] = (double)r.y;
-      sm += cm[k];
-      sx += cm[k] * cx[k];
-      sy += cm[k] * cy[k];
+    const int32_t code = L.rec[i][k].code;
+    if (code == 0) continue;
+    const double* q = L.mom[i][k];
+    if (code < 0) {
+      const double m = q[0];
+      sm += m;
+      sx += m * q[1];
+      sy += m * q[2];
     } else {
-      const double* q = A.csum + kSumStride * (int64_t)r.code;
-      cm[k] = q[0];
       sm += q[0];
       sx += q[1];
       sy += q[2];
-      if (A.quad) {
-        cx[k] = q[0] != 0.0 ? q[1] / q[0] : 0.0;
-        cy[k] = q[0] != 0.0 ? q[2] / q[0] : 0.0;
-        qq[k] = q + 3;
-      }
     }
   }
-  double* out = A.csum + kSumStride * (int64_t)b;
   out[0] = sm;
   out[1] = sx;
   out[2] = sy;
@@ -97,121 +103,229 @@ __device__ void cell_moments(const ComArgs& A, int32_t b) {
   slot->x = fx;
   slot->y = fy;
   slot->m = fm;
+  double qxx = 0.0, qxy = 0.0, qyy = 0.0;
   if (A.quad) {
-    // traceless in-plane quadrupole about the FP64 centre of mass, with the
-    // parallel-axis shift of each child (same operation order as the CPU
+    // traceless in-plane quadrupole about the FP64 centre of mass with the
+    // parallel-axis shift of each child (operation order of the CPU
     // restatement, so the FP32 results agree bit for bit)
     const double ox = sm != 0.0 ? sx / sm : 0.0, oy = sm != 0.0 ? sy / sm : 0.0;
-    double qxx = 0.0, qxy = 0.0, qyy = 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      if (rec[k].code == 0) continue;
-      const double dx = cx[k] - ox, dy = cy[k] - oy;
+      const int32_t code = L.rec[i][k].code;
+      if (code == 0) continue;
+      const double* q = L.mom[i][k];
+      double cm, cx, cy;
+      if (code < 0) {
+        cm = q[0];
+        cx = q[1];
+        cy = q[2];
+      } else {
+        cm = q[0];
+        cx = q[0] != 0.0 ? q[1] / q[0] : 0.0;
+        cy = q[0] != 0.0 ? q[2] / q[0] : 0.0;
+      }
+      const double dx = cx - ox, dy = cy - oy;
       const double t1 = dx * dx, t2 = dy * dy;
-      double axx = cm[k] * (2.0 * t1 - t2);
-      double axy = cm[k] * (3.0 * (dx * dy));
-      double ayy = cm[k] * (2.0 * t2 - t1);
-      if (qq[k]) {
-        axx = qq[k][0] + axx;
-        axy = qq[k][1] + axy;
-        ayy = qq[k][2] + ayy;
+      double axx = cm * (2.0 * t1 - t2);
+      double axy = cm * (3.0 * (dx * dy));
+      double ayy = cm * (2.0 * t2 - t1);
+      if (code > 0) {
+        axx = q[3] + axx;
+        axy = q[4] + axy;
+        ayy = q[5] + ayy;
       }
       qxx += axx;
       qxy += axy;
       qyy += ayy;
     }
-    out[3] = qxx;
-    out[4] = qxy;
-    out[5] = qyy;
     A.qblk[4 * (int64_t)p + s] = make_float4((float)qxx, (float)qxy, (float)qyy, 0.f);
   }
+  out[3] = qxx;
+  out[4] = qxy;
+  out[5] = qyy;
+  double* g = A.csum + kSumStride * (int64_t)b;
+#pragma unroll
+  for (int t = 0; t < kMom; ++t) g[t] = out[t];
 }
 
-__global__ __launch_bounds__(kComThreads) void com_local_kernel(ComArgs A) {
-  __shared__ int32_t larr[kRange];
-  __shared__ int32_t lspine;
-  const int32_t ncells = A.dev->ncells;
-  const int32_t a = blockIdx.x * kRange;  // cell ids [a, hi)
-  if (a >= ncells) return;
-  if (A.dev->err & kBuildErrors) return;  // no complete tree to sum
-  const int32_t hi = (a + kRange < ncells) ? a + kRange : ncells;
-  for (int i = threadIdx.x; i < kRange; i += blockDim.x) larr[i] = 0;
-  if (threadIdx.x == 0) lspine = 0;
+// One chunk: list positions [p0, p1) of `list` (nullptr: block id = pos + 1),
+// `next` = block id at p1 (INT32_MAX if none or if every later cell is
+// already finished).  Leftover cells go to out_ids[0..*out_n).
+__device__ void process_chunk(const ComArgs& A, ChunkLds& L, const int32_t* list,
+                              int32_t p0, int32_t p1, int32_t next,
+                              int32_t* out_ids, int32_t* out_n) {
+  const int ne = p1 - p0;
+  if (threadIdx.x == 0) L.nout = 0;
+  for (int i = threadIdx.x; i < ne; i += blockDim.x) {
+    const int32_t b = list ? list[p0 + i] : p0 + i + 1;
+    L.ids[i] = b;
+    L.local[i] = (A.cend[b] + 1 <= next) ? 1 : 0;  // subtree blocks [b, cend+1)
+    const ChildRec* r = A.blk + 4 * (int64_t)b;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) L.rec[i][k] = r[k];
+  }
   __syncthreads();
-  int32_t* myspine = A.spine + (int64_t)blockIdx.x * kSpineSlots;
-  for (int32_t c = a + threadIdx.x; c < hi; c += blockDim.x) {
-    const int32_t b = c + 1;
-    if (A.cend[b] > hi) {
-      const int32_t k = atomicAdd(&lspine, 1);
-      if (k < kSpineSlots) myspine[k] = b;
-    }
-    if (A.ckc[b] != 0) continue;
-    int32_t cur = b;
-    while (true) {
-      cell_moments(A, cur);
-      const int32_t p = A.cparent[cur];
-      if (p == 0 || p >= cur) break;
-      const int32_t pc = p - 1;
-      if (pc < a || A.cend[p] > hi) {
-        // parent finished in pass 2; the kernel boundary publishes our sums
-        atomicAdd(&A.arrive[p], 1);
-        break;
+  for (int i = threadIdx.x; i < ne; i += blockDim.x) {
+    const int32_t b = L.ids[i];
+    L.pidx[i] = find_id(L.ids, 0, i, A.cparent[b]);
+    int pend = 0;
+    if (L.local[i]) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const ChildRec r = L.rec[i][k];
+        double* q = L.mom[i][k];
+        if (r.code == 0) continue;
+        if (r.code < 0) {
+          q[0] = (double)r.m;
+          q[1] = (double)r.x;
+          q[2] = (double)r.y;
+        } else if (find_id(L.ids, i + 1, ne, r.code) >= 0) {
+          ++pend;  // finished in this chunk
+        } else {
+          // finished by an earlier pass or a later chunk
+          const double* g = A.csum + kSumStride * (int64_t)r.code;
+#pragma unroll
+          for (int t = 0; t < kMom; ++t) q[t] = g[t];
+        }
       }
+    } else {
+      const int k = atomicAdd(&L.nout, 1);
+      if (k < kSlots) out_ids[k] = b;
+    }
+    L.pend[i] = pend;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ne; i += blockDim.x) {
+    if (!L.local[i] || L.pend[i] != 0) continue;
+    int cur = i;
+    while (true) {
+      double out[kMom];
+      finish_cell(A, L, cur, out);
+      const int p = L.pidx[cur];
+      if (p < 0 || !L.local[p]) break;  // the parent belongs to a later pass
+      const int s = A.cslot[L.ids[cur]];
+#pragma unroll
+      for (int t = 0; t < kMom; ++t) L.mom[p][s][t] = out[t];
       __threadfence_block();
-      const int32_t old = atomicAdd(&larr[pc - a], 1);
-      if (old != (int32_t)A.ckc[p] - 1) break;
+      if (atomicSub(&L.pend[p], 1) != 1) break;
       __threadfence_block();
       cur = p;
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) A.scount[blockIdx.x] = lspine < kSpineSlots ? lspine : kSpineSlots;
+  if (threadIdx.x == 0) *out_n = L.nout < kSlots ? L.nout : kSlots;
 }
 
-__global__ __launch_bounds__(kSpineThreads) void com_spine_kernel(ComArgs A) {
+// Pass over list positions: one chunk per workgroup.  list == nullptr: all
+// cells (length dev->ncells); else the list compacted by the previous pass.
+__global__ __launch_bounds__(kComThreads) void com_pass_kernel(
+    ComArgs A, const int32_t* __restrict__ list, const int32_t* __restrict__ len,
+    int32_t* __restrict__ out_slots, int32_t* __restrict__ out_counts) {
+  __shared__ ChunkLds L;
+  if (A.dev->err & kBuildErrors) return;  // no complete tree to sum
+  const int32_t m = list ? *len : A.dev->ncells;
+  const int32_t p0 = blockIdx.x * kChunk;
+  if (p0 >= m) return;
+  const int32_t p1 = p0 + kChunk < m ? p0 + kChunk : m;
+  const int32_t next = p1 < m ? (list ? list[p1] : p1 + 1) : INT32_MAX;
+  process_chunk(A, L, list, p0, p1, next, out_slots + (int64_t)blockIdx.x * kSlots,
+                out_counts + blockIdx.x);
+}
+
+// Compact a pass's leftover slots into a sorted dense list.  Chunks are in
+// ascending id order and so are the leftovers of each chunk (ancestors of one
+// cell in level order have ascending pre-order ids), but the slots of one
+// chunk were appended in any order: sort each chunk's few entries first.
+__global__ __launch_bounds__(1024) void com_compact_kernel(
+    const ComArgs A, const int32_t* __restrict__ len_in, int32_t nchunks_max,
+    int32_t* __restrict__ slots, const int32_t* __restrict__ counts,
+    int32_t* __restrict__ list, int32_t* __restrict__ len_out) {
+  __shared__ int32_t part[1024];
   if (A.dev->err & kBuildErrors) return;
-  const int32_t nranges = (A.dev->ncells + kRange - 1) / kRange;
-  const int32_t nslots = nranges * kSpineSlots;
-  // a spine cell is ready at the start iff all its cell children were
-  // finished in pass 1; decide that before any pass-2 arrival happens
-  for (int32_t i = threadIdx.x; i < nslots; i += blockDim.x) {
-    const int32_t r = i / kSpineSlots, k = i % kSpineSlots;
-    uint8_t rd = 0;
-    if (k < A.scount[r]) {
-      const int32_t b = A.spine[i];
-      rd = A.arrive[b] == (int32_t)A.ckc[b] ? 1 : 0;
+  const int32_t m = len_in ? *len_in : A.dev->ncells;
+  int32_t nchunks = (m + kChunk - 1) / kChunk;
+  if (nchunks > nchunks_max) nchunks = nchunks_max;
+  // per-thread contiguous run of chunks
+  const int32_t per = (nchunks + blockDim.x - 1) / blockDim.x;
+  const int32_t c0 = threadIdx.x * per;
+  const int32_t c1 = c0 + per < nchunks ? c0 + per : nchunks;
+  int32_t tot = 0;
+  for (int32_t c = c0; c < c1; ++c) {
+    const int32_t cnt = counts[c];
+    int32_t* s = slots + (int64_t)c * kSlots;
+    for (int a = 1; a < cnt; ++a) {  // insertion sort (<= kSlots entries)
+      const int32_t v = s[a];
+      int b = a - 1;
+      while (b >= 0 && s[b] > v) {
+        s[b + 1] = s[b];
+        --b;
+      }
+      s[b + 1] = v;
     }
-    A.ready[i] = rd;
+    tot += cnt;
   }
+  part[threadIdx.x] = tot;
   __syncthreads();
-  for (int32_t i = threadIdx.x; i < nslots; i += blockDim.x) {
-    if (!A.ready[i]) continue;
-    int32_t cur = A.spine[i];
-    while (true) {
-      cell_moments(A, cur);
-      const int32_t p = A.cparent[cur];
-      if (p == 0 || p >= cur) break;
-      __threadfence_block();
-      const int32_t old = atomicAdd(&A.arrive[p], 1);
-      if (old != (int32_t)A.ckc[p] - 1) break;
-      __threadfence_block();
-      cur = p;
-    }
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan
+    const int32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int32_t pos = part[threadIdx.x] - tot;
+  for (int32_t c = c0; c < c1; ++c) {
+    const int32_t cnt = counts[c];
+    const int32_t* s = slots + (int64_t)c * kSlots;
+    for (int a = 0; a < cnt; ++a) list[pos++] = s[a];
+  }
+  if (threadIdx.x == blockDim.x - 1) *len_out = part[threadIdx.x];
+}
+
+// Last pass: one workgroup finishes the whole list, chunks from last to
+// first (every descendant of a chunk's cells is then in the chunk or done).
+__global__ __launch_bounds__(kComThreads) void com_final_kernel(
+    ComArgs A, const int32_t* __restrict__ list, const int32_t* __restrict__ len,
+    int32_t* __restrict__ scratch_slots, int32_t* __restrict__ scratch_count) {
+  __shared__ ChunkLds L;
+  if (A.dev->err & kBuildErrors) return;
+  const int32_t m = *len;
+  const int32_t nch = (m + kChunk - 1) / kChunk;
+  for (int32_t c = nch - 1; c >= 0; --c) {
+    const int32_t p0 = c * kChunk;
+    const int32_t p1 = p0 + kChunk < m ? p0 + kChunk : m;
+    process_chunk(A, L, list, p0, p1, INT32_MAX, scratch_slots, scratch_count);
+    __syncthreads();
   }
 }
 
 }  // namespace
 
+// Worst-case list lengths: pass 1 leaves at most kSlots cells per chunk.
+int64_t com_list_capacity(int64_t cap) {
+  return ((cap + kChunk - 1) / kChunk) * (int64_t)kSlots + kChunk;
+}
+
 void launch_com(stellar_ctx* c) {
   if (c->n == 0 || c->cap == 0) return;
   Buffers& B = c->b;
-  ComArgs A{B.blk,    B.qblk,  B.cparent, B.cslot,  B.ckc,
-            B.cend,   B.arrive, B.csum,   B.ccom,   B.spine,
-            B.scount, B.ready, B.dev,
+  ComArgs A{B.blk,  B.qblk, B.cparent, B.cslot, B.cend, B.csum, B.ccom, B.dev,
             (c->p.flags & STELLAR_FLAG_QUADRUPOLE) ? 1 : 0};
-  const unsigned grid = (unsigned)((c->cap + kRange - 1) / kRange);
-  com_local_kernel<<<grid, kComThreads, 0, c->stream>>>(A);
-  com_spine_kernel<<<1, kSpineThreads, 0, c->stream>>>(A);
+  // pass 1 over all cells; each later list holds at most kSlots cells per
+  // chunk of the pass before, so grids are sized from that bound and chunks
+  // beyond the actual length exit at once
+  const int64_t ch1 = (c->cap + kChunk - 1) / kChunk;
+  const int64_t ch2 = (ch1 * kSlots + kChunk - 1) / kChunk;
+  const int64_t ch3 = (ch2 * kSlots + kChunk - 1) / kChunk;
+  int32_t* l0 = B.clist;
+  int32_t* l1 = B.clist + com_list_capacity(c->cap);
+  hipStream_t s = c->stream;
+  com_pass_kernel<<<(unsigned)ch1, kComThreads, 0, s>>>(A, nullptr, nullptr, B.spine, B.scount);
+  com_compact_kernel<<<1, 1024, 0, s>>>(A, nullptr, (int32_t)ch1, B.spine, B.scount, l0, B.clen + 0);
+  com_pass_kernel<<<(unsigned)ch2, kComThreads, 0, s>>>(A, l0, B.clen + 0, B.spine, B.scount);
+  com_compact_kernel<<<1, 1024, 0, s>>>(A, B.clen + 0, (int32_t)ch2, B.spine, B.scount, l1, B.clen + 1);
+  com_pass_kernel<<<(unsigned)ch3, kComThreads, 0, s>>>(A, l1, B.clen + 1, B.spine, B.scount);
+  com_compact_kernel<<<1, 1024, 0, s>>>(A, B.clen + 1, (int32_t)ch3, B.spine, B.scount, l0, B.clen + 2);
+  com_final_kernel<<<1, kComThreads, 0, s>>>(A, l0, B.clen + 2, B.spine, B.scount);
 }
 
 }  // namespace stellar

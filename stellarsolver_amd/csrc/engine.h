@@ -46,7 +46,7 @@ struct DevScalars {
   uint32_t err;          // sticky error bits
   uint32_t pad0;
   int32_t ncells;        // cells of the current tree
-  int32_t nspine;        // cells left to the second centre-of-mass pass
+  int32_t pad1;
   float bbox[4];         // left, right, bottom, top
   unsigned long long interactions;
   uint32_t pad[46];
@@ -73,15 +73,15 @@ struct Buffers {
   // cells
   ChildRec* blk = nullptr;    // (1+cap) blocks of 4 records
   int32_t *cstart = nullptr, *ccount = nullptr, *cparent = nullptr;
-  uint8_t *clevel = nullptr, *cslot = nullptr, *ckc = nullptr;
+  uint8_t *clevel = nullptr, *cslot = nullptr;
   int32_t* cend = nullptr;    // exclusive end (cell id) of each subtree
-  int32_t* arrive = nullptr;
   double* csum = nullptr;     // kSumStride FP64 moments per cell
   float4* ccom = nullptr;     // per cell: com x, com y, mass, 0
   float4* qblk = nullptr;     // per child record: Qxx, Qxy, Qyy, 0
-  int32_t* spine = nullptr;   // cells finished by the second CoM pass
-  int32_t* scount = nullptr;  // their count per centre-of-mass range
-  uint8_t* ready = nullptr;
+  int32_t* spine = nullptr;   // CoM pass leftovers, kMaxLevel slots per chunk
+  int32_t* scount = nullptr;  // leftovers per chunk
+  int32_t* clist = nullptr;   // two compacted CoM lists
+  int32_t* clen = nullptr;    // their lengths
   float* out = nullptr;       // 2*(n+cap) interleaved output buffer
   float4* bbox_part = nullptr;  // per-block bounding-box partials
   DevScalars* dev = nullptr;
@@ -136,5 +136,6 @@ void launch_direct_fp64(stellar_ctx* c, int32_t nt, const int32_t* dtargets,
 void launch_energy_fp64(stellar_ctx* c, double eps2, double* dpart,
                         int32_t nparts);
 int energy_parts(stellar_ctx* c);
+int64_t com_list_capacity(int64_t cap);
 
 }  // namespace stellar
