@@ -72,7 +72,7 @@ void free_all(stellar_ctx* c) {
                   B.perm_alt, B.key2, B.ncell,  B.coff,   B.tmp,    B.blk,
                   B.cstart, B.ccount, B.cparent, B.clevel, B.cslot,
                   B.csum,  B.ccom,    B.out,    B.dev,
-                  B.bbox_part, B.scount,
+                  B.bbox_part, B.scount, B.choff,
                   B.cend,  B.qblk,  B.spine,   B.clist, B.clen};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -246,6 +246,7 @@ int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
   const size_t clcap = (size_t)com_list_capacity(c->cap);
   chk(dmalloc(&B.spine, clcap));
   chk(dmalloc(&B.scount, clcap / kMaxLevel + 64));
+  chk(dmalloc(&B.choff, clcap / kMaxLevel + 64));
   chk(dmalloc(&B.clist, 2 * clcap));
   chk(dmalloc(&B.clen, 64));
   chk(dmalloc(&B.ccom, C + 1));

@@ -52,8 +52,10 @@ struct ChunkLds {
   uint8_t local[kChunk];        // every unfinished descendant is in the chunk
   ChildRec rec[kChunk][4];
   double mom[kChunk][4][kMom];  // child moments: body (m, x, y), cell sums
+  int32_t wcnt[kComThreads / 64];
   int32_t nout;
 };
+static_assert(kChunk == kComThreads, "one chunk entry per thread");
 
 // Index of block id b among ids[lo, hi), or -1.
 __device__ __forceinline__ int find_id(const int32_t* ids, int lo, int hi, int32_t b) {
@@ -155,7 +157,6 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds& L, const int32_t* list
                               int32_t p0, int32_t p1, int32_t next,
                               int32_t* out_ids, int32_t* out_n) {
   const int ne = p1 - p0;
-  if (threadIdx.x == 0) L.nout = 0;
   for (int i = threadIdx.x; i < ne; i += blockDim.x) {
     const int32_t b = list ? list[p0 + i] : p0 + i + 1;
     L.ids[i] = b;
@@ -165,34 +166,51 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds& L, const int32_t* list
     for (int k = 0; k < 4; ++k) L.rec[i][k] = r[k];
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < ne; i += blockDim.x) {
-    const int32_t b = L.ids[i];
-    L.pidx[i] = find_id(L.ids, 0, i, A.cparent[b]);
-    int pend = 0;
-    if (L.local[i]) {
+  {
+    const int i = threadIdx.x;  // one entry per thread (kChunk == blockDim)
+    const bool live = i < ne;
+    const int32_t b = live ? L.ids[i] : 0;
+    bool left = false;
+    if (live) {
+      L.pidx[i] = find_id(L.ids, 0, i, A.cparent[b]);
+      int pend = 0;
+      if (L.local[i]) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const ChildRec r = L.rec[i][k];
-        double* q = L.mom[i][k];
-        if (r.code == 0) continue;
-        if (r.code < 0) {
-          q[0] = (double)r.m;
-          q[1] = (double)r.x;
-          q[2] = (double)r.y;
-        } else if (find_id(L.ids, i + 1, ne, r.code) >= 0) {
-          ++pend;  // finished in this chunk
-        } else {
-          // finished by an earlier pass or a later chunk
-          const double* g = A.csum + kSumStride * (int64_t)r.code;
+        for (int k = 0; k < 4; ++k) {
+          const ChildRec r = L.rec[i][k];
+          double* q = L.mom[i][k];
+          if (r.code == 0) continue;
+          if (r.code < 0) {
+            q[0] = (double)r.m;
+            q[1] = (double)r.x;
+            q[2] = (double)r.y;
+          } else if (find_id(L.ids, i + 1, ne, r.code) >= 0) {
+            ++pend;  // finished in this chunk
+          } else {
+            // finished by an earlier pass or a later chunk
+            const double* g = A.csum + kSumStride * (int64_t)r.code;
 #pragma unroll
-          for (int t = 0; t < kMom; ++t) q[t] = g[t];
+            for (int t = 0; t < kMom; ++t) q[t] = g[t];
+          }
         }
+      } else {
+        left = true;
       }
-    } else {
-      const int k = atomicAdd(&L.nout, 1);
-      if (k < kSlots) out_ids[k] = b;
+      L.pend[i] = pend;
     }
-    L.pend[i] = pend;
+    // leftovers keep ascending id order: rank = leftovers before entry i
+    const uint64_t bal = __builtin_amdgcn_ballot_w64(left);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) L.wcnt[wv] = __popcll(bal);
+    __syncthreads();
+    int rank = __popcll(bal & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wv; ++w) rank += L.wcnt[w];
+    if (left && rank < kSlots) out_ids[rank] = b;
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int w = 0; w < kComThreads / 64; ++w) tot += L.wcnt[w];
+      L.nout = tot;
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < ne; i += blockDim.x) {
@@ -232,38 +250,24 @@ __global__ __launch_bounds__(kComThreads) void com_pass_kernel(
                 out_counts + blockIdx.x);
 }
 
-// Compact a pass's leftover slots into a sorted dense list.  Chunks are in
-// ascending id order and so are the leftovers of each chunk (ancestors of one
-// cell in level order have ascending pre-order ids), but the slots of one
-// chunk were appended in any order: sort each chunk's few entries first.
-__global__ __launch_bounds__(1024) void com_compact_kernel(
+// Compaction of a pass's leftovers into a dense list, in two launches:
+// offsets (one workgroup scans the per-chunk counts) and gather (one
+// wavefront per chunk).  Chunks are in ascending id order and each chunk's
+// leftovers were written in ascending order, so the list is sorted.
+__global__ __launch_bounds__(1024) void com_offsets_kernel(
     const ComArgs A, const int32_t* __restrict__ len_in, int32_t nchunks_max,
-    int32_t* __restrict__ slots, const int32_t* __restrict__ counts,
-    int32_t* __restrict__ list, int32_t* __restrict__ len_out) {
+    const int32_t* __restrict__ counts, int32_t* __restrict__ offs,
+    int32_t* __restrict__ len_out) {
   __shared__ int32_t part[1024];
   if (A.dev->err & kBuildErrors) return;
   const int32_t m = len_in ? *len_in : A.dev->ncells;
   int32_t nchunks = (m + kChunk - 1) / kChunk;
   if (nchunks > nchunks_max) nchunks = nchunks_max;
-  // per-thread contiguous run of chunks
   const int32_t per = (nchunks + blockDim.x - 1) / blockDim.x;
   const int32_t c0 = threadIdx.x * per;
   const int32_t c1 = c0 + per < nchunks ? c0 + per : nchunks;
   int32_t tot = 0;
-  for (int32_t c = c0; c < c1; ++c) {
-    const int32_t cnt = counts[c];
-    int32_t* s = slots + (int64_t)c * kSlots;
-    for (int a = 1; a < cnt; ++a) {  // insertion sort (<= kSlots entries)
-      const int32_t v = s[a];
-      int b = a - 1;
-      while (b >= 0 && s[b] > v) {
-        s[b + 1] = s[b];
-        --b;
-      }
-      s[b + 1] = v;
-    }
-    tot += cnt;
-  }
+  for (int32_t c = c0; c < c1; ++c) tot += counts[c];
   part[threadIdx.x] = tot;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan
@@ -274,11 +278,22 @@ __global__ __launch_bounds__(1024) void com_compact_kernel(
   }
   int32_t pos = part[threadIdx.x] - tot;
   for (int32_t c = c0; c < c1; ++c) {
-    const int32_t cnt = counts[c];
-    const int32_t* s = slots + (int64_t)c * kSlots;
-    for (int a = 0; a < cnt; ++a) list[pos++] = s[a];
+    offs[c] = pos;
+    pos += counts[c];
   }
   if (threadIdx.x == blockDim.x - 1) *len_out = part[threadIdx.x];
+}
+
+__global__ __launch_bounds__(64) void com_gather_kernel(
+    const ComArgs A, const int32_t* __restrict__ len_in,
+    const int32_t* __restrict__ slots, const int32_t* __restrict__ counts,
+    const int32_t* __restrict__ offs, int32_t* __restrict__ list) {
+  if (A.dev->err & kBuildErrors) return;
+  const int32_t m = len_in ? *len_in : A.dev->ncells;
+  const int32_t c = blockIdx.x;
+  if (c * kChunk >= m) return;
+  if ((int)threadIdx.x < counts[c])
+    list[offs[c] + threadIdx.x] = slots[(int64_t)c * kSlots + threadIdx.x];
 }
 
 // Last pass: one workgroup finishes the whole list, chunks from last to
@@ -319,12 +334,16 @@ void launch_com(stellar_ctx* c) {
   int32_t* l0 = B.clist;
   int32_t* l1 = B.clist + com_list_capacity(c->cap);
   hipStream_t s = c->stream;
+  auto compact = [&](const int32_t* len_in, int64_t ch, int32_t* list, int32_t* len_out) {
+    com_offsets_kernel<<<1, 1024, 0, s>>>(A, len_in, (int32_t)ch, B.scount, B.choff, len_out);
+    com_gather_kernel<<<(unsigned)ch, 64, 0, s>>>(A, len_in, B.spine, B.scount, B.choff, list);
+  };
   com_pass_kernel<<<(unsigned)ch1, kComThreads, 0, s>>>(A, nullptr, nullptr, B.spine, B.scount);
-  com_compact_kernel<<<1, 1024, 0, s>>>(A, nullptr, (int32_t)ch1, B.spine, B.scount, l0, B.clen + 0);
+  compact(nullptr, ch1, l0, B.clen + 0);
   com_pass_kernel<<<(unsigned)ch2, kComThreads, 0, s>>>(A, l0, B.clen + 0, B.spine, B.scount);
-  com_compact_kernel<<<1, 1024, 0, s>>>(A, B.clen + 0, (int32_t)ch2, B.spine, B.scount, l1, B.clen + 1);
+  compact(B.clen + 0, ch2, l1, B.clen + 1);
   com_pass_kernel<<<(unsigned)ch3, kComThreads, 0, s>>>(A, l1, B.clen + 1, B.spine, B.scount);
-  com_compact_kernel<<<1, 1024, 0, s>>>(A, B.clen + 1, (int32_t)ch3, B.spine, B.scount, l0, B.clen + 2);
+  compact(B.clen + 1, ch3, l0, B.clen + 2);
   com_final_kernel<<<1, kComThreads, 0, s>>>(A, l0, B.clen + 2, B.spine, B.scount);
 }
 

@@ -80,6 +80,7 @@ struct Buffers {
   float4* qblk = nullptr;     // per child record: Qxx, Qxy, Qyy, 0
   int32_t* spine = nullptr;   // CoM pass leftovers, kMaxLevel slots per chunk
   int32_t* scount = nullptr;  // leftovers per chunk
+  int32_t* choff = nullptr;   // their offsets in the compacted list
   int32_t* clist = nullptr;   // two compacted CoM lists
   int32_t* clen = nullptr;    // their lengths
   float* out = nullptr;       // 2*(n+cap) interleaved output buffer

@@ -185,10 +185,11 @@ __global__ __launch_bounds__(kWalkBlock) void walk64_kernel(
         const float dy = __int_as_float(q[jj][k].y) - py;
         const float cmv = __int_as_float(q[jj][k].z);
         const float r = dist2(dx, dy, eps2);
-        // one accumulate site for bodies and accepted cells (the vote is
-        // evaluated for bodies too; it costs one compare)
-        const uint64_t bad = __builtin_amdgcn_ballot_w64(!(dp[jj] <= r));
-        if (code < 0 || bad == 0) {
+        // one vote for bodies and cells: a body always passes (threshold -1
+        // against r >= eps2 > 0), so acceptance is a single scalar test
+        const int32_t ti = code < 0 ? __float_as_int(-1.0f) : __float_as_int(dp[jj]);
+        const float t = __int_as_float(__builtin_amdgcn_readfirstlane(ti));
+        if (__builtin_amdgcn_ballot_w64(!(t <= r)) == 0) {
           if (QUAD && code > 0)
             acc.quad(cmv, r, dx, dy, qm[QUAD ? jj : 0][k]);
           else
