@@ -1,0 +1,101 @@
+// stellar_app: the reference's command line (main.cpp:75-103) and headless
+// simulation loop (StellarSolverVisualizer::runSimulation,
+// StellarSolverVisualizer.cpp:199-206) over BarnesHutParticleSystem.
+//
+// Reference flags: -disk (default), -plummer, -colliding-disks, -opengl,
+// -debug, -benchmark, -fullscreen, -iterations=, -timestep=, -gravity=,
+// -dampening= (and -barnes-hut, which the README shows and which selects
+// nothing).  Added: -numbodies= (reference: 512*256), -theta=, -eps=,
+// -group=, -config= (seeded workload 0, 1, 2 or 4), -seed=, -quad,
+// -textbook.  As in the reference, a flag's first character is skipped
+// when matching, so "-x" and "--x"-style prefixes of one character both work.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <iostream>
+
+#include "BarnesHutParticleSystem.h"
+#include "SimulationParameters.h"
+
+namespace {
+
+bool checkCmdLineFlag(const int argc, char** argv, const char* string) {
+  for (int i = 1; i < argc; i++)
+    if (argv[i][0] != '\0' && std::strcmp(&argv[i][1], string) == 0) return true;
+  return false;
+}
+
+// Value text of "-name=value", or nullptr.
+const char* cmdLineValue(const int argc, char** argv, const char* name) {
+  const size_t len = std::strlen(name);
+  for (int i = 1; i < argc; i++) {
+    if (argv[i][0] == '\0') continue;
+    const char* str = &argv[i][1];
+    const char* eqs = std::strchr(str, '=');
+    if (eqs && (size_t)(eqs - str) == len && std::strncmp(str, name, len) == 0) return eqs + 1;
+  }
+  return nullptr;
+}
+
+int getCmdLineParameterInt(const int argc, char** argv, const char* s, const int def) {
+  const char* v = cmdLineValue(argc, argv, s);
+  return v ? std::atoi(v) : def;
+}
+
+float getCmdLineParameterFloat(const int argc, char** argv, const char* s, const float def) {
+  const char* v = cmdLineValue(argc, argv, s);
+  return v ? (float)std::atof(v) : def;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  int numbodies = getCmdLineParameterInt(argc, argv, "numbodies", 512 * 256);
+  SimulationParameters parameters;
+  if (checkCmdLineFlag(argc, argv, "plummer"))
+    parameters.model = plummer_model;
+  else if (checkCmdLineFlag(argc, argv, "colliding-disks"))
+    parameters.model = colliding_disk_model;
+  else
+    parameters.model = disk_model;
+  parameters.opengl = checkCmdLineFlag(argc, argv, "opengl");
+  parameters.debug = checkCmdLineFlag(argc, argv, "debug");
+  parameters.benchmark = checkCmdLineFlag(argc, argv, "benchmark");
+  parameters.fullscreen = checkCmdLineFlag(argc, argv, "fullscreen");
+  parameters.iterations = (float)getCmdLineParameterInt(argc, argv, "iterations", 50);
+  parameters.timestep = getCmdLineParameterFloat(argc, argv, "timestep", 0.001f);
+  parameters.gravity = getCmdLineParameterFloat(argc, argv, "gravity", 1.0f);
+  parameters.dampening = getCmdLineParameterFloat(argc, argv, "dampening", 1.0f);
+  parameters.theta = getCmdLineParameterFloat(argc, argv, "theta", 0.5f);
+  const char* eps = cmdLineValue(argc, argv, "eps");
+  parameters.eps2 = eps ? (float)(std::atof(eps) * std::atof(eps)) : 0.025f;
+  parameters.group = getCmdLineParameterInt(argc, argv, "group", 64);
+  parameters.workload = getCmdLineParameterInt(argc, argv, "config", -1);
+  const char* seed = cmdLineValue(argc, argv, "seed");
+  parameters.seed_given = seed != nullptr;
+  parameters.seed = seed ? (uint64_t)std::strtoull(seed, nullptr, 10) : 0;
+  parameters.quadrupole = checkCmdLineFlag(argc, argv, "quad");
+  parameters.textbook = checkCmdLineFlag(argc, argv, "textbook");
+  if (parameters.opengl)
+    std::cerr << "note: no renderer in this build; -opengl only fetches the output buffer "
+                 "every step" << std::endl;
+  try {
+    BarnesHutParticleSystem particles(parameters, numbodies);
+    particles.reset();
+    double total_ms = 0.0;
+    for (int i = 0; i < (int)parameters.iterations; i++) {
+      particles.update();
+      total_ms += particles.lastStepMilliseconds();
+    }
+    if (parameters.benchmark && parameters.iterations > 0)
+      std::cout << "Bodies: " << numbodies << "    Steps: " << (int)parameters.iterations
+                << "    Mean step: " << total_ms / parameters.iterations << " ms    "
+                << "Body-steps/s: "
+                << (double)numbodies * parameters.iterations / (total_ms * 1e-3) << std::endl;
+  } catch (const std::exception& e) {
+    std::cerr << "stellar_app: " << e.what() << std::endl;
+    return 1;
+  }
+  return 0;
+}
