@@ -248,19 +248,129 @@ __device__ __forceinline__ int digit_at(const uint64_t* __restrict__ key,
                    : (int)((key2[s] >> (2 * (64 - lev))) & 3u);
 }
 
-// Cells opened by pair (s, s+1): levels lambda(s-1)+1 .. lambda(s).
+// Minimum tree over lam: level 0 is lam itself (one byte per tree position,
+// lam[n-1] = 0), level k >= 1 holds the minimum of 64 entries of level k-1.
+// Every level is padded to whole 64-byte groups so that a group is read with
+// four 16-byte loads; entries past a level's length are never used.
+constexpr int kMinTreeLevels = 6;  // up to 64^6 positions
+struct MinTree {
+  const uint8_t* lv[kMinTreeLevels];
+  int32_t len[kMinTreeLevels];
+  int top;  // highest level, len[top] <= 64
+};
+
+MinTree mintree_layout(int32_t n, const uint8_t* lam, const uint8_t* lmin,
+                       int64_t* bytes) {
+  MinTree t{};
+  t.lv[0] = lam;
+  t.len[0] = n;
+  t.top = 0;
+  int64_t off = 0;
+  while (t.len[t.top] > 64 && t.top + 1 < kMinTreeLevels) {
+    const int32_t m = (t.len[t.top] + 63) / 64;
+    ++t.top;
+    t.lv[t.top] = lmin ? lmin + off : nullptr;
+    t.len[t.top] = m;
+    off += ((int64_t)m + 63) / 64 * 64;
+  }
+  if (bytes) *bytes = off + 64;
+  return t;
+}
+
+// Cells opened by pair (s, s+1): levels lam(s-1)+1 .. lam(s).  Also writes
+// lam and, per wavefront of 64 positions, level 1 of the minimum tree.
 __global__ __launch_bounds__(256) void lambda_kernel(
     int32_t n, const uint64_t* __restrict__ key,
-    const uint64_t* __restrict__ key2, int32_t* __restrict__ ncell) {
+    const uint64_t* __restrict__ key2, int32_t* __restrict__ ncell,
+    uint8_t* __restrict__ lam, uint8_t* __restrict__ lmin1) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  int v = 0;
+  int v = 0, lm = 0;
   if (s + 1 < n) {
-    const int lam = common_levels(key, key2, s, s + 1);
+    lm = common_levels(key, key2, s, s + 1);
     const int prev = s > 0 ? common_levels(key, key2, s - 1, s) : 0;
-    v = lam > prev ? lam - prev : 0;
+    v = lm > prev ? lm - prev : 0;
   }
-  ncell[s] = v;
+  if (s < n) {
+    ncell[s] = v;
+    lam[s] = (uint8_t)lm;
+  }
+  if (lmin1) {
+    int w = s < n ? lm : 255;
+    for (int off = 32; off > 0; off >>= 1) w = min(w, __shfl_xor(w, off));
+    if ((threadIdx.x & 63) == 0 && s < n) lmin1[s >> 6] = (uint8_t)w;
+  }
+}
+
+// Levels 2..top of the minimum tree (one workgroup).
+__global__ __launch_bounds__(1024) void mintree_kernel(MinTree t) {
+  for (int k = 2; k <= t.top; ++k) {
+    const uint8_t* src = t.lv[k - 1];
+    uint8_t* dst = const_cast<uint8_t*>(t.lv[k]);
+    for (int32_t e = threadIdx.x; e < t.len[k]; e += blockDim.x) {
+      const int32_t j0 = 64 * e;
+      const int32_t j1 = j0 + 64 < t.len[k - 1] ? j0 + 64 : t.len[k - 1];
+      int w = 255;
+      for (int32_t j = j0; j < j1; ++j) w = min(w, (int)src[j]);
+      dst[e] = (uint8_t)w;
+    }
+    __syncthreads();
+  }
+}
+
+// Bit 7 of byte i set when byte i of w is below v (bytes <= 255, v <= 65):
+// (b | 0x80) - v keeps bit 7 exactly when (b & 0x7f) >= v, and never borrows
+// from the next byte.  Entries are <= 64 or 255, so (b & 0x7f) < v <=> b < v.
+__device__ __forceinline__ uint64_t below_mask(uint64_t w, uint32_t v) {
+  const uint64_t hi = 0x8080808080808080ull;
+  return ~((w | hi) - 0x0101010101010101ull * v) & hi;
+}
+
+// Largest j < cnt with g[j] < v among the 64 entries at g (64-byte aligned),
+// or -1.
+__device__ __forceinline__ int last_below(const uint8_t* g, int cnt, uint32_t v) {
+  const uint4* q = reinterpret_cast<const uint4*>(g);
+  const uint4 a0 = q[0], a1 = q[1], a2 = q[2], a3 = q[3];
+  const uint64_t w[8] = {
+      (uint64_t)a0.x | ((uint64_t)a0.y << 32), (uint64_t)a0.z | ((uint64_t)a0.w << 32),
+      (uint64_t)a1.x | ((uint64_t)a1.y << 32), (uint64_t)a1.z | ((uint64_t)a1.w << 32),
+      (uint64_t)a2.x | ((uint64_t)a2.y << 32), (uint64_t)a2.z | ((uint64_t)a2.w << 32),
+      (uint64_t)a3.x | ((uint64_t)a3.y << 32), (uint64_t)a3.z | ((uint64_t)a3.w << 32)};
+  int res = -1;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    uint64_t m = below_mask(w[k], v);
+    const int lo = 8 * k;
+    if (cnt <= lo) m = 0;
+    else if (cnt < lo + 8) m &= (1ull << (8 * (cnt - lo))) - 1ull;
+    if (m) res = lo + ((63 - __clzll(m)) >> 3);
+  }
+  return res;
+}
+
+// Previous smaller value: the largest j < i with lam[j] < v, or -1.
+__device__ int32_t psv(const MinTree& t, int32_t i, uint32_t v) {
+  int k = 0;
+  int32_t pos = i;  // search level-k entries [pos & ~63, pos)
+  while (true) {
+    const int32_t g0 = pos & ~63;
+    const int j = pos > g0 ? last_below(t.lv[k] + g0, pos - g0, v) : -1;
+    if (j >= 0) {
+      pos = g0 + j;
+      break;
+    }
+    if (g0 == 0 || k == t.top) return -1;
+    pos = g0 >> 6;
+    ++k;
+  }
+  while (k > 0) {  // descend into the last group holding a smaller value
+    --k;
+    const int32_t g0 = pos << 6;
+    const int32_t cnt = t.len[k] - g0 < 64 ? t.len[k] - g0 : 64;
+    const int j = last_below(t.lv[k] + g0, cnt, v);
+    if (j < 0) return -1;  // only if the tree is inconsistent
+    pos = g0 + j;
+  }
+  return pos;
 }
 
 struct LinkArgs {
@@ -270,132 +380,119 @@ struct LinkArgs {
   const uint64_t* key2;
   const int32_t* coff;
   const int32_t* ncell;
+  const uint8_t* lam;
   const uint32_t* perm;
   const float* x;
   const float* y;
   const float* m;
   ChildRec* blk;
   int32_t* cstart;
-  int32_t* ccount;
   int32_t* cparent;
   uint8_t* clevel;
   uint8_t* cslot;
-  int32_t* cend;
   DevScalars* dev;
+  MinTree mt;
 };
 
-// Largest j in [s, n-1] sharing at least L levels with s.
-__device__ int32_t range_end(const LinkArgs& A, int32_t s, int L) {
-  int32_t lo = s, step = 1;
-  while (lo + step <= A.n - 1 && common_levels(A.key, A.key2, s, lo + step) >= L) {
-    lo += step;
-    step <<= 1;
-  }
-  int32_t hi = lo + step < A.n ? lo + step : A.n;
-  while (hi - lo > 1) {
-    const int32_t mid = lo + ((hi - lo) >> 1);
-    if (common_levels(A.key, A.key2, s, mid) >= L)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-// Largest v in [u, e] whose code at level lev is <= d (codes ascend there).
-__device__ int32_t digit_end(const LinkArgs& A, int32_t u, int32_t e, int lev,
-                             int d) {
-  int32_t lo = u, step = 1;
-  while (lo + step <= e && digit_at(A.key, A.key2, lo + step, lev) <= d) {
-    lo += step;
-    step <<= 1;
-  }
-  int32_t hi = lo + step <= e ? lo + step : e + 1;
-  while (hi - lo > 1) {
-    const int32_t mid = lo + ((hi - lo) >> 1);
-    if (digit_at(A.key, A.key2, mid, lev) <= d)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ int lam_before(const LinkArgs& A, int32_t u) {
-  return u > 0 ? common_levels(A.key, A.key2, u - 1, u) : 0;
-}
-
-// Fill block `blkid` for the node holding tree positions [a, e] whose
-// bodies share `lev` levels (lev = 0: the root).
-__device__ void link_node(const LinkArgs& A, int32_t blkid, int32_t a,
-                          int32_t e, int lev) {
-  ChildRec rec[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) rec[k] = ChildRec{0.f, 0.f, 0.f, 0};
-  if (lev == kMaxLevel) {
-    // deepest level: keep up to four bodies by rank
-    if (e - a + 1 > 4) {
-      atomicOr(&A.dev->err, kErrBucket);
-      e = a + 3;
-    }
-    for (int32_t u = a; u <= e; ++u) {
-      const uint32_t p = A.perm[u];
-      rec[u - a] = ChildRec{A.x[p], A.y[p], A.m[p], -1 - u};
-    }
-  } else {
-    int32_t u = a;
-    while (u <= e) {
-      const int d = digit_at(A.key, A.key2, u, lev + 1);
-      const int32_t v = digit_end(A, u, e, lev + 1, d);
-      if (u == v) {
-        const uint32_t p = A.perm[u];
-        rec[d] = ChildRec{A.x[p], A.y[p], A.m[p], -1 - u};
-      } else {
-        const int64_t c2 = (int64_t)A.coff[u] + (lev + 1 - lam_before(A, u) - 1);
-        // a child cell always follows its parent in pre-order; anything else
-        // means the keys were left inconsistent by an earlier error
-        if (c2 < A.cap && c2 + 1 > blkid) {
-          const int32_t b2 = (int32_t)(c2 + 1);
-          rec[d] = ChildRec{0.f, 0.f, 0.f, b2};
-          A.cparent[b2] = blkid;
-          A.cslot[b2] = (uint8_t)d;
-        }
-      }
-      u = v + 1;
-    }
-  }
-  ChildRec* dst = A.blk + 4 * (int64_t)blkid;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) dst[k] = rec[k];
-  A.cend[blkid] = A.coff[e];  // pairs s..e-1 open the subtree's cells
-  A.cstart[blkid] = a;
-  A.ccount[blkid] = e - a + 1;
-  A.clevel[blkid] = (uint8_t)lev;
-}
-
 // One thread per adjacent pair s: creates the cells that pair opens, at
-// levels lambda(s-1)+1 .. lambda(s), each with id coff[s] + offset.  The
-// last thread (s = n-1, which opens none) links the root block.
-__global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
+// levels lam(s-1)+1 .. lam(s), with ids coff[s] + k.  Each new block is
+// cleared; each cell but the first is linked into the one before it (its
+// parent).  The thread of position n-1 (no pair) clears the root block.
+__global__ __launch_bounds__(256) void cells_kernel(LinkArgs A) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= A.n) return;
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
   if (s == A.n - 1) {
     const int64_t total = (int64_t)A.coff[s] + A.ncell[s];
     if (total > A.cap) atomicOr(&A.dev->err, kErrPool);
     A.dev->ncells = (int32_t)(total < A.cap ? total : A.cap);
-    link_node(A, 0, 0, A.n - 1, 0);
+    float4* d = reinterpret_cast<float4*>(A.blk);
+    d[0] = zero;
+    d[1] = zero;
+    d[2] = zero;
+    d[3] = zero;
+    A.cstart[0] = 0;
+    A.clevel[0] = 0;
+    A.cparent[0] = -1;
     return;
   }
   const int nc = A.ncell[s];
   if (nc == 0) return;
-  const int lam = common_levels(A.key, A.key2, s, s + 1);
-  const int first = lam - nc + 1;
-  for (int L = lam; L >= first; --L) {
-    const int64_t c = (int64_t)A.coff[s] + (L - first);
-    if (c >= A.cap) continue;
-    const int32_t e = range_end(A, s, L);
-    link_node(A, (int32_t)(c + 1), s, e, L);
+  const int lp = s > 0 ? A.lam[s - 1] : 0;
+  const int64_t base = A.coff[s];
+  for (int k = 0; k < nc; ++k) {
+    const int64_t c = base + k;
+    if (c >= A.cap) break;
+    const int32_t b = (int32_t)(c + 1);
+    const int L = lp + 1 + k;
+    float4* d = reinterpret_cast<float4*>(A.blk + 4 * (int64_t)b);
+    d[0] = zero;
+    d[1] = zero;
+    d[2] = zero;
+    d[3] = zero;
+    A.cstart[b] = s;
+    A.clevel[b] = (uint8_t)L;
+    const int dg = digit_at(A.key, A.key2, s, L);
+    A.cslot[b] = (uint8_t)dg;
+    if (k > 0) {
+      A.cparent[b] = b - 1;
+      A.blk[4 * (int64_t)(b - 1) + dg] = ChildRec{0.f, 0.f, 0.f, b};
+    }
   }
+}
+
+// One thread per tree position s: links body s, and the first cell s opens,
+// into their parents.  The node at level lam(s-1) holding s-1 and s starts
+// at a = psv(s-1) + 1 and was opened by pair a.  Body s sits in the deepest
+// node holding another body: s's own last cell when lam(s) > lam(s-1), else
+// that node; its slot is its code one level down, or its rank in a level-64
+// cell.
+__global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= A.n) return;
+  const int lp = s > 0 ? A.lam[s - 1] : 0;
+  const int nc = s + 1 < A.n ? A.ncell[s] : 0;
+  int64_t pid = -1;  // cell id of the level-lp node (-1: the root)
+  int32_t a = 0;
+  if (lp > 0) {
+    a = psv(A.mt, s - 1, (uint32_t)lp) + 1;
+    const int la = a > 0 ? A.lam[a - 1] : 0;
+    pid = (int64_t)A.coff[a] + (lp - la - 1);
+  }
+  const bool pok = pid < A.cap;
+  const int32_t pblk = (int32_t)(pid + 1);
+  if (nc > 0) {
+    const int64_t c0 = A.coff[s];
+    if (c0 < A.cap && pok) {
+      const int32_t b = (int32_t)(c0 + 1);
+      A.cparent[b] = pblk;
+      const int dg = digit_at(A.key, A.key2, s, lp + 1);
+      A.blk[4 * (int64_t)pblk + dg] = ChildRec{0.f, 0.f, 0.f, b};
+    }
+  }
+  int64_t bid;  // block of body s's parent
+  int slot;
+  if (nc > 0) {
+    const int64_t c = (int64_t)A.coff[s] + nc - 1;
+    if (c >= A.cap) return;
+    bid = c + 1;
+    const int lev = lp + nc;
+    slot = lev < kMaxLevel ? digit_at(A.key, A.key2, s, lev + 1) : 0;
+  } else {
+    if (!pok) return;
+    bid = pblk;
+    if (lp < kMaxLevel) {
+      slot = digit_at(A.key, A.key2, s, lp + 1);
+    } else {
+      slot = s - a;
+      if (slot > 3) {
+        atomicOr(&A.dev->err, kErrBucket);
+        return;
+      }
+    }
+  }
+  const uint32_t p = A.perm[s];
+  A.blk[4 * bid + slot] = ChildRec{A.x[p], A.y[p], A.m[p], -1 - s};
 }
 
 inline unsigned grid_for(int64_t items, int threads) {
@@ -448,16 +545,25 @@ void launch_build(stellar_ctx* c) {
   if (vb.current() != B.perm) std::swap(B.perm, B.perm_alt);
   ties_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, B.id, n, B.key,
                                                         B.perm, B.key2, B.dev);
-  lambda_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, B.key, B.key2,
-                                                          B.ncell);
+  int64_t mtb = 0;
+  const MinTree mt = mintree_layout(n, B.lam, B.lmin, &mtb);
+  lambda_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(
+      n, B.key, B.key2, B.ncell, B.lam, mt.top >= 1 ? const_cast<uint8_t*>(mt.lv[1]) : nullptr);
+  if (mt.top >= 2) mintree_kernel<<<1, 1024, 0, c->stream>>>(mt);
   bytes = B.tmp_bytes;
   (void)rocprim::exclusive_scan(B.tmp, bytes, B.ncell, B.coff, 0, (size_t)n,
                                 rocprim::plus<int32_t>(), c->stream);
-  LinkArgs A{n,        c->cap,    B.key,     B.key2,    B.coff,
-             B.ncell,  B.perm,    B.x,       B.y,       B.m,
-             B.blk,    B.cstart,  B.ccount,  B.cparent, B.clevel,
-             B.cslot,  B.cend,    B.dev};
+  LinkArgs A{n,      c->cap,   B.key,   B.key2,   B.coff,    B.ncell,
+             B.lam,  B.perm,   B.x,     B.y,      B.m,       B.blk,
+             B.cstart, B.cparent, B.clevel, B.cslot, B.dev,   mt};
+  cells_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(A);
   link_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(A);
+}
+
+size_t mintree_bytes(int32_t n) {
+  int64_t b = 0;
+  (void)mintree_layout(n, nullptr, nullptr, &b);
+  return (size_t)b;
 }
 
 }  // namespace stellar

@@ -70,10 +70,10 @@ void free_all(stellar_ctx* c) {
                   B.x2,    B.y2,     B.vx2,     B.vy2,    B.m2,     B.id2,
                   B.ax,    B.ay,     B.inter,   B.key,    B.key_alt, B.perm,
                   B.perm_alt, B.key2, B.ncell,  B.coff,   B.tmp,    B.blk,
-                  B.cstart, B.ccount, B.cparent, B.clevel, B.cslot,
+                  B.cstart, B.lam, B.cparent, B.clevel, B.cslot,
                   B.csum,  B.ccom,    B.out,    B.dev,
                   B.bbox_part, B.scount, B.choff,
-                  B.cend,  B.qblk,  B.spine,   B.clist, B.clen};
+                  B.lmin,  B.qblk,  B.spine,   B.clist, B.clen};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (B.host_dev) (void)hipHostFree(B.host_dev);
@@ -232,16 +232,16 @@ int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
   chk(dmalloc(&B.key2, N));
   chk(dmalloc(&B.ncell, N));
   chk(dmalloc(&B.coff, N));
+  chk(dmalloc(&B.lam, (size_t)N + 64));
+  chk(dmalloc(&B.lmin, mintree_bytes((int32_t)N)));
   B.tmp_bytes = build_scratch_bytes((int32_t)N);
   chk(hipMalloc(&B.tmp, B.tmp_bytes));
   chk(dmalloc(&B.blk, 4 * (C + 1)));
   chk(dmalloc(&B.cstart, C + 1));
-  chk(dmalloc(&B.ccount, C + 1));
   chk(dmalloc(&B.cparent, C + 1));
   chk(dmalloc(&B.clevel, C + 1));
   chk(dmalloc(&B.cslot, C + 1));
   chk(dmalloc(&B.csum, kSumStride * (C + 1)));
-  chk(dmalloc(&B.cend, C + 1));
   chk(dmalloc(&B.qblk, 4 * (C + 1)));
   const size_t clcap = (size_t)com_list_capacity(c->cap);
   chk(dmalloc(&B.spine, clcap));
@@ -536,8 +536,16 @@ int stellar_export_tree(stellar_ctx* c, int32_t* root4, int32_t* child,
     std::vector<int32_t> tmp((size_t)(C + 1));
     std::vector<uint8_t> lv((size_t)(C + 1));
     if (count) {
-      HIP_TRY(c, hipMemcpy(tmp.data(), B.ccount, (size_t)(C + 1) * 4, hipMemcpyDeviceToHost));
-      std::memcpy(count, tmp.data() + 1, (size_t)C * 4);
+      // bodies below each cell, summed up the tree (children have larger ids)
+      for (int64_t q = C - 1; q >= 0; --q) {
+        int32_t t = 0;
+        for (int k = 0; k < 4; ++k) {
+          const int32_t code = blk[(size_t)(4 * (q + 1) + k)].code;
+          if (code < 0) t += 1;
+          else if (code > 0 && code - 1 > q && code - 1 < C) t += count[code - 1];
+        }
+        count[q] = t;
+      }
     }
     if (start) {
       HIP_TRY(c, hipMemcpy(tmp.data(), B.cstart, (size_t)(C + 1) * 4, hipMemcpyDeviceToHost));

@@ -10,8 +10,9 @@
 // FP64, and the FP32 centre of mass is the FP64 quotient rounded once, so the
 // result does not depend on scheduling.
 //
-// Scheduling.  Cells are numbered in DFS pre-order, so a cell's subtree is the
-// contiguous id range [c, cend[c]) and its children have larger ids.  A pass
+// Scheduling.  Cells are numbered in DFS pre-order, so a cell's subtree is a
+// contiguous id range that ends at the next cell of the same or a lower level,
+// and its children have larger ids.  A pass
 // takes a sorted list of unfinished cells in chunks of kChunk; a workgroup
 // stages its chunk's child records and child moments in LDS and finishes
 // every cell whose unfinished descendants all lie in the chunk, following
@@ -38,7 +39,7 @@ struct ComArgs {
   float4* qblk;
   const int32_t* cparent;
   const uint8_t* cslot;
-  const int32_t* cend;
+  const uint8_t* clevel;
   double* csum;  // kSumStride per block
   float4* ccom;
   DevScalars* dev;
@@ -53,6 +54,7 @@ struct ChunkLds {
   ChildRec rec[kChunk][4];
   double mom[kChunk][4][kMom];  // child moments: body (m, x, y), cell sums
   int32_t wcnt[kComThreads / 64];
+  int32_t wmin[kComThreads / 64];
   int32_t nout;
 };
 static_assert(kChunk == kComThreads, "one chunk entry per thread");
@@ -153,17 +155,43 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds& L, int i, double* out) {
 // One chunk: list positions [p0, p1) of `list` (nullptr: block id = pos + 1),
 // `next` = block id at p1 (INT32_MAX if none or if every later cell is
 // already finished).  Leftover cells go to out_ids[0..*out_n).
+//
+// Every list is closed under taking parents (a cell is unfinished while any
+// descendant is), and in such a subsequence of pre-order the first entry
+// after a cell's descendants has a level no deeper than the cell's.  So the
+// unfinished descendants of entry i all lie in the chunk exactly when some
+// later entry of the chunk, or `next`, has level <= level(i).
 __device__ void process_chunk(const ComArgs& A, ChunkLds& L, const int32_t* list,
                               int32_t p0, int32_t p1, int32_t next,
                               int32_t* out_ids, int32_t* out_n) {
   const int ne = p1 - p0;
-  for (int i = threadIdx.x; i < ne; i += blockDim.x) {
-    const int32_t b = list ? list[p0 + i] : p0 + i + 1;
-    L.ids[i] = b;
-    L.local[i] = (A.cend[b] + 1 <= next) ? 1 : 0;  // subtree blocks [b, cend+1)
-    const ChildRec* r = A.blk + 4 * (int64_t)b;
+  {
+    // one entry per thread (kChunk == blockDim)
+    const int i = threadIdx.x;
+    int lev = 255;
+    if (i < ne) {
+      const int32_t b = list ? list[p0 + i] : p0 + i + 1;
+      L.ids[i] = b;
+      lev = A.clevel[b];
+      const ChildRec* r = A.blk + 4 * (int64_t)b;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) L.rec[i][k] = r[k];
+      for (int k = 0; k < 4; ++k) L.rec[i][k] = r[k];
+    }
+    // suffix minimum of the levels: within the wavefront, then across
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int suf = lev;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_down(suf, off);
+      if (lane + off < 64) suf = min(suf, o);
+    }
+    if (lane == 0) L.wmin[wv] = suf;
+    int after = __shfl_down(suf, 1);  // entries i+1 .. end of wavefront
+    if (lane == 63) after = 255;
+    __syncthreads();
+    for (int w = wv + 1; w < kComThreads / 64; ++w) after = min(after, L.wmin[w]);
+    const int nlev = next == INT32_MAX ? -1 : (int)A.clevel[next];
+    after = min(after, nlev);
+    if (i < ne) L.local[i] = after <= lev ? 1 : 0;
   }
   __syncthreads();
   {
@@ -323,7 +351,7 @@ int64_t com_list_capacity(int64_t cap) {
 void launch_com(stellar_ctx* c) {
   if (c->n == 0 || c->cap == 0) return;
   Buffers& B = c->b;
-  ComArgs A{B.blk,  B.qblk, B.cparent, B.cslot, B.cend, B.csum, B.ccom, B.dev,
+  ComArgs A{B.blk,  B.qblk, B.cparent, B.cslot, B.clevel, B.csum, B.ccom, B.dev,
             (c->p.flags & STELLAR_FLAG_QUADRUPOLE) ? 1 : 0};
   // pass 1 over all cells; each later list holds at most kSlots cells per
   // chunk of the pass before, so grids are sized from that bound and chunks

@@ -68,13 +68,14 @@ struct Buffers {
   uint64_t* key2 = nullptr;   // level 33..64 codes (valid inside tie runs)
   int32_t* ncell = nullptr;   // cells opened by each adjacent pair
   int32_t* coff = nullptr;    // exclusive scan of ncell
+  uint8_t* lam = nullptr;     // common-prefix depth of each adjacent pair
+  uint8_t* lmin = nullptr;    // minimum tree over lam (levels 1..top)
   void* tmp = nullptr;        // rocprim scratch
   size_t tmp_bytes = 0;
   // cells
   ChildRec* blk = nullptr;    // (1+cap) blocks of 4 records
-  int32_t *cstart = nullptr, *ccount = nullptr, *cparent = nullptr;
+  int32_t *cstart = nullptr, *cparent = nullptr;
   uint8_t *clevel = nullptr, *cslot = nullptr;
-  int32_t* cend = nullptr;    // exclusive end (cell id) of each subtree
   double* csum = nullptr;     // kSumStride FP64 moments per cell
   float4* ccom = nullptr;     // per cell: com x, com y, mass, 0
   float4* qblk = nullptr;     // per child record: Qxx, Qxy, Qyy, 0
@@ -132,6 +133,7 @@ void launch_output(stellar_ctx* c);
 void launch_zero_counter(stellar_ctx* c);
 void launch_init_scalars(stellar_ctx* c);
 size_t build_scratch_bytes(int32_t n);
+size_t mintree_bytes(int32_t n);
 void launch_direct_fp64(stellar_ctx* c, int32_t nt, const int32_t* dtargets,
                         double eps2, double* dax, double* day);
 void launch_energy_fp64(stellar_ctx* c, double eps2, double* dpart,
