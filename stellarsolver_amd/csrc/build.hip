@@ -278,11 +278,14 @@ MinTree mintree_layout(int32_t n, const uint8_t* lam, const uint8_t* lmin,
 }
 
 // Cells opened by pair (s, s+1): levels lam(s-1)+1 .. lam(s).  Also writes
-// lam and, per wavefront of 64 positions, level 1 of the minimum tree.
+// lam and, per wavefront of 64 positions, level 1 of the minimum tree, and
+// clears tree blocks [0, nclear) with coalesced 16-byte stores (the cells
+// kernel clears any block from n on).
 __global__ __launch_bounds__(256) void lambda_kernel(
     int32_t n, const uint64_t* __restrict__ key,
     const uint64_t* __restrict__ key2, int32_t* __restrict__ ncell,
-    uint8_t* __restrict__ lam, uint8_t* __restrict__ lmin1) {
+    uint8_t* __restrict__ lam, uint8_t* __restrict__ lmin1,
+    float4* __restrict__ blk, int64_t nclear) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   int v = 0, lm = 0;
   if (s + 1 < n) {
@@ -294,6 +297,12 @@ __global__ __launch_bounds__(256) void lambda_kernel(
     ncell[s] = v;
     lam[s] = (uint8_t)lm;
   }
+  const int64_t r0 = 4 * (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int64_t q = r0 + (int64_t)r * blockDim.x;
+    if (q < 4 * nclear) blk[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   if (lmin1) {
     int w = s < n ? lm : 255;
     for (int off = 32; off > 0; off >>= 1) w = min(w, __shfl_xor(w, off));
@@ -301,16 +310,28 @@ __global__ __launch_bounds__(256) void lambda_kernel(
   }
 }
 
-// Levels 2..top of the minimum tree (one workgroup).
+// Levels 2..top of the minimum tree (one workgroup; each entry reads its 64
+// children as four 16-byte loads).
 __global__ __launch_bounds__(1024) void mintree_kernel(MinTree t) {
   for (int k = 2; k <= t.top; ++k) {
     const uint8_t* src = t.lv[k - 1];
     uint8_t* dst = const_cast<uint8_t*>(t.lv[k]);
     for (int32_t e = threadIdx.x; e < t.len[k]; e += blockDim.x) {
       const int32_t j0 = 64 * e;
-      const int32_t j1 = j0 + 64 < t.len[k - 1] ? j0 + 64 : t.len[k - 1];
+      const int cnt = t.len[k - 1] - j0 < 64 ? t.len[k - 1] - j0 : 64;
+      const uint4* q = reinterpret_cast<const uint4*>(src + j0);
+      const uint4 a[4] = {q[0], q[1], q[2], q[3]};
       int w = 255;
-      for (int32_t j = j0; j < j1; ++j) w = min(w, (int)src[j]);
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const uint32_t word = u % 4 == 0 ? a[u / 4].x
+                              : u % 4 == 1 ? a[u / 4].y
+                              : u % 4 == 2 ? a[u / 4].z
+                                           : a[u / 4].w;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (4 * u + b < cnt) w = min(w, (int)((word >> (8 * b)) & 255u));
+      }
       dst[e] = (uint8_t)w;
     }
     __syncthreads();
@@ -395,9 +416,9 @@ struct LinkArgs {
 };
 
 // One thread per adjacent pair s: creates the cells that pair opens, at
-// levels lam(s-1)+1 .. lam(s), with ids coff[s] + k.  Each new block is
-// cleared; each cell but the first is linked into the one before it (its
-// parent).  The thread of position n-1 (no pair) clears the root block.
+// levels lam(s-1)+1 .. lam(s), with ids coff[s] + k, and links each cell but
+// the first into the one before it (its parent).  The thread of position n-1
+// (no pair) records the cell count and the root's metadata.
 __global__ __launch_bounds__(256) void cells_kernel(LinkArgs A) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= A.n) return;
@@ -406,11 +427,6 @@ __global__ __launch_bounds__(256) void cells_kernel(LinkArgs A) {
     const int64_t total = (int64_t)A.coff[s] + A.ncell[s];
     if (total > A.cap) atomicOr(&A.dev->err, kErrPool);
     A.dev->ncells = (int32_t)(total < A.cap ? total : A.cap);
-    float4* d = reinterpret_cast<float4*>(A.blk);
-    d[0] = zero;
-    d[1] = zero;
-    d[2] = zero;
-    d[3] = zero;
     A.cstart[0] = 0;
     A.clevel[0] = 0;
     A.cparent[0] = -1;
@@ -425,11 +441,13 @@ __global__ __launch_bounds__(256) void cells_kernel(LinkArgs A) {
     if (c >= A.cap) break;
     const int32_t b = (int32_t)(c + 1);
     const int L = lp + 1 + k;
-    float4* d = reinterpret_cast<float4*>(A.blk + 4 * (int64_t)b);
-    d[0] = zero;
-    d[1] = zero;
-    d[2] = zero;
-    d[3] = zero;
+    if (b >= A.n) {  // blocks below n were cleared by the lambda kernel
+      float4* d = reinterpret_cast<float4*>(A.blk + 4 * (int64_t)b);
+      d[0] = zero;
+      d[1] = zero;
+      d[2] = zero;
+      d[3] = zero;
+    }
     A.cstart[b] = s;
     A.clevel[b] = (uint8_t)L;
     const int dg = digit_at(A.key, A.key2, s, L);
@@ -547,8 +565,10 @@ void launch_build(stellar_ctx* c) {
                                                         B.perm, B.key2, B.dev);
   int64_t mtb = 0;
   const MinTree mt = mintree_layout(n, B.lam, B.lmin, &mtb);
+  const int64_t nclear = (int64_t)n < c->cap + 1 ? (int64_t)n : c->cap + 1;
   lambda_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(
-      n, B.key, B.key2, B.ncell, B.lam, mt.top >= 1 ? const_cast<uint8_t*>(mt.lv[1]) : nullptr);
+      n, B.key, B.key2, B.ncell, B.lam, mt.top >= 1 ? const_cast<uint8_t*>(mt.lv[1]) : nullptr,
+      reinterpret_cast<float4*>(B.blk), nclear);
   if (mt.top >= 2) mintree_kernel<<<1, 1024, 0, c->stream>>>(mt);
   bytes = B.tmp_bytes;
   (void)rocprim::exclusive_scan(B.tmp, bytes, B.ncell, B.coff, 0, (size_t)n,

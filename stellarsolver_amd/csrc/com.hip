@@ -46,18 +46,25 @@ struct ComArgs {
   int quad;
 };
 
+template <bool EXT>
 struct ChunkLds {
   int32_t ids[kChunk];          // block ids, ascending
   int32_t pidx[kChunk];         // chunk index of the parent, -1 if none
   int32_t pend[kChunk];         // unfinished children inside the chunk
+  int32_t pblk[kChunk];         // parent block
+  uint8_t slot[kChunk];         // slot in the parent block
   uint8_t local[kChunk];        // every unfinished descendant is in the chunk
+  int16_t cidx[kChunk][4];      // chunk index of each cell child, -1 if none
   ChildRec rec[kChunk][4];
-  double mom[kChunk][4][kMom];  // child moments: body (m, x, y), cell sums
+  double mom[kChunk][kMom];     // moments of the entry once finished
+  // moments of cell children finished before this chunk (EXT only)
+  double ext[EXT ? kChunk : 1][4][kMom];
   int32_t wcnt[kComThreads / 64];
   int32_t wmin[kComThreads / 64];
   int32_t nout;
 };
 static_assert(kChunk == kComThreads, "one chunk entry per thread");
+static_assert(kChunk <= 32767, "chunk indices are int16");
 
 // Index of block id b among ids[lo, hi), or -1.
 __device__ __forceinline__ int find_id(const int32_t* ids, int lo, int hi, int32_t b) {
@@ -73,16 +80,44 @@ __device__ __forceinline__ int find_id(const int32_t* ids, int lo, int hi, int32
   return -1;
 }
 
-// Moments of chunk entry i from its staged child slots (slot order, FP64),
-// written to global memory and returned in out[kMom].
-__device__ void finish_cell(const ComArgs& A, ChunkLds& L, int i, double* out) {
+// Moments of child slot k of entry i into registers: a body's (m, x, y), or
+// a cell's sums, finished in this chunk or, staged in L.ext, before it (never
+// in the first pass, EXT = false, whose local cells have their whole subtree
+// in the chunk).  NM = 3 or kMom values.
+template <int NM, bool EXT>
+__device__ __forceinline__ void child_mom(const ChunkLds<EXT>& L, int i, int k,
+                                          int32_t code, double (&q)[kMom]) {
+  if (code < 0) {
+    const ChildRec& r = L.rec[i][k];
+    q[0] = (double)r.m;
+    q[1] = (double)r.x;
+    q[2] = (double)r.y;
+#pragma unroll
+    for (int t = 3; t < NM; ++t) q[t] = 0.0;
+    return;
+  }
+  const int ci = L.cidx[i][k];
+  if (!EXT || ci >= 0) {
+#pragma unroll
+    for (int t = 0; t < NM; ++t) q[t] = L.mom[ci < 0 ? 0 : ci][t];
+  } else {
+#pragma unroll
+    for (int t = 0; t < NM; ++t) q[t] = L.ext[EXT ? i : 0][k][t];
+  }
+}
+
+// Moments of chunk entry i from its child slots (slot order, FP64), kept in
+// L.mom[i] and written to global memory.
+template <bool EXT>
+__device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
   const int32_t b = L.ids[i];
   double sm = 0.0, sx = 0.0, sy = 0.0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int32_t code = L.rec[i][k].code;
     if (code == 0) continue;
-    const double* q = L.mom[i][k];
+    double q[kMom];
+    child_mom<3, EXT>(L, i, k, code, q);
     if (code < 0) {
       const double m = q[0];
       sm += m;
@@ -94,6 +129,7 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds& L, int i, double* out) {
       sy += q[2];
     }
   }
+  double out[kMom];
   out[0] = sm;
   out[1] = sx;
   out[2] = sy;
@@ -101,12 +137,9 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds& L, int i, double* out) {
   const float fx = sm != 0.0 ? (float)(sx / sm) : 0.f;
   const float fy = sm != 0.0 ? (float)(sy / sm) : 0.f;
   A.ccom[b] = make_float4(fx, fy, fm, 0.f);
-  const int32_t p = A.cparent[b];
-  const int s = A.cslot[b];
-  ChildRec* slot = A.blk + 4 * (int64_t)p + s;
-  slot->x = fx;
-  slot->y = fy;
-  slot->m = fm;
+  const int32_t p = L.pblk[i];
+  const int s = L.slot[i];
+  A.blk[4 * (int64_t)p + s] = ChildRec{fx, fy, fm, b};
   double qxx = 0.0, qxy = 0.0, qyy = 0.0;
   if (A.quad) {
     // traceless in-plane quadrupole about the FP64 centre of mass with the
@@ -117,7 +150,8 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds& L, int i, double* out) {
     for (int k = 0; k < 4; ++k) {
       const int32_t code = L.rec[i][k].code;
       if (code == 0) continue;
-      const double* q = L.mom[i][k];
+      double q[kMom];
+      child_mom<kMom, EXT>(L, i, k, code, q);
       double cm, cx, cy;
       if (code < 0) {
         cm = q[0];
@@ -149,7 +183,10 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds& L, int i, double* out) {
   out[5] = qyy;
   double* g = A.csum + kSumStride * (int64_t)b;
 #pragma unroll
-  for (int t = 0; t < kMom; ++t) g[t] = out[t];
+  for (int t = 0; t < kMom; ++t) {
+    g[t] = out[t];
+    L.mom[i][t] = out[t];
+  }
 }
 
 // One chunk: list positions [p0, p1) of `list` (nullptr: block id = pos + 1),
@@ -161,24 +198,29 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds& L, int i, double* out) {
 // after a cell's descendants has a level no deeper than the cell's.  So the
 // unfinished descendants of entry i all lie in the chunk exactly when some
 // later entry of the chunk, or `next`, has level <= level(i).
-__device__ void process_chunk(const ComArgs& A, ChunkLds& L, const int32_t* list,
+//
+// Everything a dependency chain needs is staged in LDS first, so the chains
+// issue global stores only and never wait on global memory.
+template <bool EXT>
+__device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t* list,
                               int32_t p0, int32_t p1, int32_t next,
                               int32_t* out_ids, int32_t* out_n) {
   const int ne = p1 - p0;
+  const int i = threadIdx.x;  // one entry per thread (kChunk == blockDim)
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   {
-    // one entry per thread (kChunk == blockDim)
-    const int i = threadIdx.x;
     int lev = 255;
     if (i < ne) {
       const int32_t b = list ? list[p0 + i] : p0 + i + 1;
       L.ids[i] = b;
       lev = A.clevel[b];
+      L.pblk[i] = A.cparent[b];
+      L.slot[i] = A.cslot[b];
       const ChildRec* r = A.blk + 4 * (int64_t)b;
 #pragma unroll
       for (int k = 0; k < 4; ++k) L.rec[i][k] = r[k];
     }
     // suffix minimum of the levels: within the wavefront, then across
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int suf = lev;
     for (int off = 1; off < 64; off <<= 1) {
       const int o = __shfl_down(suf, off);
@@ -195,31 +237,29 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds& L, const int32_t* list
   }
   __syncthreads();
   {
-    const int i = threadIdx.x;  // one entry per thread (kChunk == blockDim)
     const bool live = i < ne;
     const int32_t b = live ? L.ids[i] : 0;
     bool left = false;
     if (live) {
-      L.pidx[i] = find_id(L.ids, 0, i, A.cparent[b]);
+      L.pidx[i] = find_id(L.ids, 0, i, L.pblk[i]);
       int pend = 0;
       if (L.local[i]) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const ChildRec r = L.rec[i][k];
-          double* q = L.mom[i][k];
-          if (r.code == 0) continue;
-          if (r.code < 0) {
-            q[0] = (double)r.m;
-            q[1] = (double)r.x;
-            q[2] = (double)r.y;
-          } else if (find_id(L.ids, i + 1, ne, r.code) >= 0) {
-            ++pend;  // finished in this chunk
-          } else {
-            // finished by an earlier pass or a later chunk
-            const double* g = A.csum + kSumStride * (int64_t)r.code;
+          const int32_t code = L.rec[i][k].code;
+          int ci = -1;
+          if (code > 0) {
+            ci = find_id(L.ids, i + 1, ne, code);
+            if (ci >= 0) {
+              ++pend;  // finished in this chunk
+            } else if (EXT) {
+              // finished by an earlier pass or a later chunk
+              const double* g = A.csum + kSumStride * (int64_t)code;
 #pragma unroll
-            for (int t = 0; t < kMom; ++t) q[t] = g[t];
+              for (int t = 0; t < kMom; ++t) L.ext[EXT ? i : 0][k][t] = g[t];
+            }
           }
+          L.cidx[i][k] = (int16_t)ci;
         }
       } else {
         left = true;
@@ -228,7 +268,6 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds& L, const int32_t* list
     }
     // leftovers keep ascending id order: rank = leftovers before entry i
     const uint64_t bal = __builtin_amdgcn_ballot_w64(left);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) L.wcnt[wv] = __popcll(bal);
     __syncthreads();
     int rank = __popcll(bal & ((1ull << lane) - 1ull));
@@ -241,17 +280,12 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds& L, const int32_t* list
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < ne; i += blockDim.x) {
-    if (!L.local[i] || L.pend[i] != 0) continue;
+  if (i < ne && L.local[i] && L.pend[i] == 0) {
     int cur = i;
     while (true) {
-      double out[kMom];
-      finish_cell(A, L, cur, out);
+      finish_cell<EXT>(A, L, cur);
       const int p = L.pidx[cur];
       if (p < 0 || !L.local[p]) break;  // the parent belongs to a later pass
-      const int s = A.cslot[L.ids[cur]];
-#pragma unroll
-      for (int t = 0; t < kMom; ++t) L.mom[p][s][t] = out[t];
       __threadfence_block();
       if (atomicSub(&L.pend[p], 1) != 1) break;
       __threadfence_block();
@@ -264,17 +298,18 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds& L, const int32_t* list
 
 // Pass over list positions: one chunk per workgroup.  list == nullptr: all
 // cells (length dev->ncells); else the list compacted by the previous pass.
+template <bool EXT>
 __global__ __launch_bounds__(kComThreads) void com_pass_kernel(
     ComArgs A, const int32_t* __restrict__ list, const int32_t* __restrict__ len,
     int32_t* __restrict__ out_slots, int32_t* __restrict__ out_counts) {
-  __shared__ ChunkLds L;
+  __shared__ ChunkLds<EXT> L;
   if (A.dev->err & kBuildErrors) return;  // no complete tree to sum
   const int32_t m = list ? *len : A.dev->ncells;
   const int32_t p0 = blockIdx.x * kChunk;
   if (p0 >= m) return;
   const int32_t p1 = p0 + kChunk < m ? p0 + kChunk : m;
   const int32_t next = p1 < m ? (list ? list[p1] : p1 + 1) : INT32_MAX;
-  process_chunk(A, L, list, p0, p1, next, out_slots + (int64_t)blockIdx.x * kSlots,
+  process_chunk<EXT>(A, L, list, p0, p1, next, out_slots + (int64_t)blockIdx.x * kSlots,
                 out_counts + blockIdx.x);
 }
 
@@ -329,14 +364,14 @@ __global__ __launch_bounds__(64) void com_gather_kernel(
 __global__ __launch_bounds__(kComThreads) void com_final_kernel(
     ComArgs A, const int32_t* __restrict__ list, const int32_t* __restrict__ len,
     int32_t* __restrict__ scratch_slots, int32_t* __restrict__ scratch_count) {
-  __shared__ ChunkLds L;
+  __shared__ ChunkLds<true> L;
   if (A.dev->err & kBuildErrors) return;
   const int32_t m = *len;
   const int32_t nch = (m + kChunk - 1) / kChunk;
   for (int32_t c = nch - 1; c >= 0; --c) {
     const int32_t p0 = c * kChunk;
     const int32_t p1 = p0 + kChunk < m ? p0 + kChunk : m;
-    process_chunk(A, L, list, p0, p1, INT32_MAX, scratch_slots, scratch_count);
+    process_chunk<true>(A, L, list, p0, p1, INT32_MAX, scratch_slots, scratch_count);
     __syncthreads();
   }
 }
@@ -366,11 +401,11 @@ void launch_com(stellar_ctx* c) {
     com_offsets_kernel<<<1, 1024, 0, s>>>(A, len_in, (int32_t)ch, B.scount, B.choff, len_out);
     com_gather_kernel<<<(unsigned)ch, 64, 0, s>>>(A, len_in, B.spine, B.scount, B.choff, list);
   };
-  com_pass_kernel<<<(unsigned)ch1, kComThreads, 0, s>>>(A, nullptr, nullptr, B.spine, B.scount);
+  com_pass_kernel<false><<<(unsigned)ch1, kComThreads, 0, s>>>(A, nullptr, nullptr, B.spine, B.scount);
   compact(nullptr, ch1, l0, B.clen + 0);
-  com_pass_kernel<<<(unsigned)ch2, kComThreads, 0, s>>>(A, l0, B.clen + 0, B.spine, B.scount);
+  com_pass_kernel<true><<<(unsigned)ch2, kComThreads, 0, s>>>(A, l0, B.clen + 0, B.spine, B.scount);
   compact(B.clen + 0, ch2, l1, B.clen + 1);
-  com_pass_kernel<<<(unsigned)ch3, kComThreads, 0, s>>>(A, l1, B.clen + 1, B.spine, B.scount);
+  com_pass_kernel<true><<<(unsigned)ch3, kComThreads, 0, s>>>(A, l1, B.clen + 1, B.spine, B.scount);
   compact(B.clen + 1, ch3, l0, B.clen + 2);
   com_final_kernel<<<1, kComThreads, 0, s>>>(A, l0, B.clen + 2, B.spine, B.scount);
 }
