@@ -91,6 +91,20 @@ void stellar_default_params(stellar_params* p);
  * of BarnesHutParticleSystem.cpp:54-83. */
 int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
                    stellar_ctx** out);
+/* Create a context for an ensemble of ninst independent simulations of
+ * counts[i] >= 1 bodies each (no counterpart in the reference, which runs one
+ * simulation per process).  Instance i owns body ids and positions
+ * [offsets[i], offsets[i+1]) with offsets the running sum of counts; every
+ * stage advances all instances in one launch (per-instance boxes, roots and
+ * node pools; the sort keeps each instance in its own range).  Body arrays
+ * crossing the ABI are the instances' arrays concatenated.  node_capacity
+ * <= 0 selects sum(2*counts[i] + 12000).  ninst <= 65536. */
+int stellar_batch_create(const stellar_params* p, int32_t ninst,
+                         const int32_t* counts, int64_t node_capacity,
+                         stellar_ctx** out);
+int32_t stellar_num_instances(const stellar_ctx* ctx);
+/* ninst + 1 offsets. */
+int stellar_instance_offsets(const stellar_ctx* ctx, int32_t* offsets);
 int stellar_destroy(stellar_ctx* ctx);
 int stellar_set_params(stellar_ctx* ctx, const stellar_params* p);
 int stellar_get_params(const stellar_ctx* ctx, stellar_params* p);
@@ -125,9 +139,11 @@ int stellar_run(stellar_ctx* ctx, int32_t steps);
 int stellar_sync(stellar_ctx* ctx);
 
 /* ---- inspection --------------------------------------------------------- */
-/* Bounding box {left, right, bottom, top} of the last stellar_bbox. */
+/* Bounding box {left, right, bottom, top} of the last stellar_bbox
+ * (instance 0 / instance inst). */
 int stellar_get_bbox(stellar_ctx* ctx, float* bbox4);
-/* Number of cells of the last stellar_build. */
+int stellar_get_bbox_instance(stellar_ctx* ctx, int32_t inst, float* bbox4);
+/* Number of cells of the last stellar_build (all instances). */
 int stellar_num_cells(stellar_ctx* ctx, int64_t* ncells);
 
 /* Canonical tree of the last build: cells renumbered in DFS pre-order over
@@ -139,6 +155,13 @@ int stellar_export_tree(stellar_ctx* ctx, int32_t* root4, int32_t* child,
                         int32_t* count, int32_t* start, int32_t* level,
                         int32_t* sorted, float* com_x, float* com_y,
                         float* com_m, int64_t capacity, int64_t* ncells);
+/* The same for instance inst of an ensemble, with body ids and positions
+ * local to the instance (0 .. counts[inst]-1). */
+int stellar_export_tree_instance(stellar_ctx* ctx, int32_t inst, int32_t* root4,
+                                 int32_t* child, int32_t* count, int32_t* start,
+                                 int32_t* level, int32_t* sorted, float* com_x,
+                                 float* com_y, float* com_m, int64_t capacity,
+                                 int64_t* ncells);
 
 /* Per-body interaction counts and accelerations of the last stellar_forces,
  * body-id order.  Either pointer may be NULL. */
@@ -159,13 +182,18 @@ const char* stellar_status_string(int status);
 
 /* ---- FP64 diagnostics on the device ------------------------------------ */
 /* Direct-sum accelerations in FP64 for the listed body ids (all if targets
- * is NULL, then ntargets must be n), softening eps2 as given. */
+ * is NULL, then ntargets must be n), softening eps2 as given; sources are
+ * the bodies of the target's instance. */
 int stellar_direct_fp64(stellar_ctx* ctx, int32_t ntargets,
                         const int32_t* targets, double eps2, double* ax,
                         double* ay);
-/* Kinetic and Plummer-softened potential energy, linear momentum. */
+/* Kinetic and Plummer-softened potential energy, linear momentum (summed
+ * over the instances of an ensemble, each with its own bodies only). */
 int stellar_energy_fp64(stellar_ctx* ctx, double eps2, double* kinetic,
                         double* potential, double* px, double* py);
+/* The same per instance: arrays of ninst values, any pointer may be NULL. */
+int stellar_energy_fp64_instances(stellar_ctx* ctx, double eps2, double* kinetic,
+                                  double* potential, double* px, double* py);
 
 /* ---- initial conditions (host) ------------------------------------------ */
 #define STELLAR_IC_DISK 0          /* central mass + uniform-radius disk     */

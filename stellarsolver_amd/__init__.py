@@ -87,6 +87,9 @@ _I64 = C.POINTER(C.c_int64)
 SIGNATURES = {
     "stellar_default_params": (None, [C.POINTER(StellarParams)]),
     "stellar_create": (C.c_int, [C.POINTER(StellarParams), C.c_int32, C.c_int64, C.POINTER(_P)]),
+    "stellar_batch_create": (C.c_int, [C.POINTER(StellarParams), C.c_int32, _I, C.c_int64, C.POINTER(_P)]),
+    "stellar_num_instances": (C.c_int32, [_P]),
+    "stellar_instance_offsets": (C.c_int, [_P, _I]),
     "stellar_destroy": (C.c_int, [_P]),
     "stellar_set_params": (C.c_int, [_P, C.POINTER(StellarParams)]),
     "stellar_get_params": (C.c_int, [_P, C.POINTER(StellarParams)]),
@@ -104,8 +107,10 @@ SIGNATURES = {
     "stellar_run": (C.c_int, [_P, C.c_int32]),
     "stellar_sync": (C.c_int, [_P]),
     "stellar_get_bbox": (C.c_int, [_P, _F]),
+    "stellar_get_bbox_instance": (C.c_int, [_P, C.c_int32, _F]),
     "stellar_num_cells": (C.c_int, [_P, _I64]),
     "stellar_export_tree": (C.c_int, [_P, _I, _I, _I, _I, _I, _I, _F, _F, _F, C.c_int64, _I64]),
+    "stellar_export_tree_instance": (C.c_int, [_P, C.c_int32, _I, _I, _I, _I, _I, _I, _F, _F, _F, C.c_int64, _I64]),
     "stellar_export_forces": (C.c_int, [_P, _F, _F, _I]),
     "stellar_last_interactions": (C.c_int, [_P, _I64]),
     "stellar_stage_times": (C.c_int, [_P, _F]),
@@ -113,6 +118,7 @@ SIGNATURES = {
     "stellar_status_string": (C.c_char_p, [C.c_int]),
     "stellar_direct_fp64": (C.c_int, [_P, C.c_int32, _I, C.c_double, _D, _D]),
     "stellar_energy_fp64": (C.c_int, [_P, C.c_double, _D, _D, _D, _D]),
+    "stellar_energy_fp64_instances": (C.c_int, [_P, C.c_double, _D, _D, _D, _D]),
     "stellar_ic_defaults": (C.c_int, [C.c_int32, C.POINTER(StellarICParams)]),
     "stellar_generate_ic": (C.c_int, [C.POINTER(StellarICParams), C.c_int32, _F, _F, _F, _F, _F]),
 }
@@ -191,17 +197,30 @@ def generate_ic(p: StellarICParams, n: int) -> dict[str, np.ndarray]:
 
 
 class Engine:
-    """One simulation on the current HIP device (a stellar_ctx)."""
+    """One simulation, or an ensemble of independent simulations (``counts``),
+    on the current HIP device (a stellar_ctx)."""
 
-    def __init__(self, n: int, params: StellarParams | None = None, node_capacity: int = 0):
+    def __init__(self, n: int | None = None, params: StellarParams | None = None,
+                 node_capacity: int = 0, counts=None):
         self._lib = load_library()
-        self.n = int(n)
         self.params = params if params is not None else default_params()
         h = C.c_void_p()
-        rc = self._lib.stellar_create(C.byref(self.params), self.n, int(node_capacity), C.byref(h))
+        if counts is None:
+            self.n = int(n)
+            self.offsets = np.array([0, self.n], np.int64)
+            rc = self._lib.stellar_create(C.byref(self.params), self.n, int(node_capacity), C.byref(h))
+        else:
+            cnt = np.ascontiguousarray(counts, dtype=np.int32)
+            self.offsets = np.concatenate([[0], np.cumsum(cnt, dtype=np.int64)])
+            self.n = int(self.offsets[-1])
+            if n is not None and int(n) != self.n:
+                raise ValueError("n must equal sum(counts)")
+            rc = self._lib.stellar_batch_create(C.byref(self.params), cnt.size, _ip(cnt),
+                                                int(node_capacity), C.byref(h))
         if rc:
             raise StellarError(rc, self._lib.stellar_status_string(rc).decode())
         self._h = h
+        self.ninst = len(self.offsets) - 1
 
     # -- lifetime -------------------------------------------------------
     def close(self) -> None:
@@ -287,9 +306,9 @@ class Engine:
         self._check(self._lib.stellar_sync(self._h), "sync")
 
     # -- inspection -----------------------------------------------------
-    def get_bbox(self) -> np.ndarray:
+    def get_bbox(self, instance: int = 0) -> np.ndarray:
         b = np.zeros(4, np.float32)
-        self._check(self._lib.stellar_get_bbox(self._h, _fp(b)), "get_bbox")
+        self._check(self._lib.stellar_get_bbox_instance(self._h, int(instance), _fp(b)), "get_bbox")
         return b
 
     def num_cells(self) -> int:
@@ -297,31 +316,32 @@ class Engine:
         self._check(self._lib.stellar_num_cells(self._h, C.byref(c)), "num_cells")
         return c.value
 
-    def export_tree(self, with_com: bool = True) -> dict[str, np.ndarray]:
+    def export_tree(self, with_com: bool = True, instance: int = 0) -> dict[str, np.ndarray]:
         ncell = self.num_cells()
         cap = max(ncell, 1)
+        ni = int(self.offsets[instance + 1] - self.offsets[instance])
         t = {
             "root": np.zeros(4, np.int32),
             "child": np.zeros(4 * cap, np.int32),
             "count": np.zeros(cap, np.int32),
             "start": np.zeros(cap, np.int32),
             "level": np.zeros(cap, np.int32),
-            "sorted": np.zeros(max(self.n, 1), np.int32),
+            "sorted": np.zeros(max(ni, 1), np.int32),
         }
         if with_com:
             for k in ("com_x", "com_y", "com_m"):
                 t[k] = np.zeros(cap, np.float32)
         got = C.c_int64()
         self._check(
-            self._lib.stellar_export_tree(
-                self._h, _ip(t["root"]), _ip(t["child"]), _ip(t["count"]), _ip(t["start"]),
+            self._lib.stellar_export_tree_instance(
+                self._h, int(instance), _ip(t["root"]), _ip(t["child"]), _ip(t["count"]), _ip(t["start"]),
                 _ip(t["level"]), _ip(t["sorted"]), _fp(t.get("com_x")), _fp(t.get("com_y")),
                 _fp(t.get("com_m")), cap, C.byref(got)),
             "export_tree",
         )
         c = got.value
         out = {"root": t["root"], "child": t["child"][: 4 * c], "count": t["count"][:c],
-               "start": t["start"][:c], "level": t["level"][:c], "sorted": t["sorted"][: self.n],
+               "start": t["start"][:c], "level": t["level"][:c], "sorted": t["sorted"][:ni],
                "ncells": c}
         if with_com:
             for k in ("com_x", "com_y", "com_m"):
@@ -364,6 +384,13 @@ class Engine:
         v = [C.c_double() for _ in range(4)]
         self._check(self._lib.stellar_energy_fp64(self._h, e2, *(C.byref(q) for q in v)), "energy_fp64")
         return {"kinetic": v[0].value, "potential": v[1].value, "px": v[2].value, "py": v[3].value}
+
+    def energy_fp64_instances(self, eps2: float | None = None) -> dict[str, np.ndarray]:
+        e2 = float(self.params.eps2) if eps2 is None else float(eps2)
+        v = {k: np.zeros(self.ninst, np.float64) for k in ("kinetic", "potential", "px", "py")}
+        self._check(self._lib.stellar_energy_fp64_instances(
+            self._h, e2, *(_dp(v[k]) for k in ("kinetic", "potential", "px", "py"))), "energy_fp64_instances")
+        return v
 
 
 __all__ = [

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void update_kernel(
 // copy_kernel (kernels.cu:541-553): interleaved positions in body-id order,
 // followed by the cell centres of mass.
 __global__ __launch_bounds__(256) void output_kernel(
-    int32_t n, int64_t cap, const float* __restrict__ x,
+    int32_t n, int64_t cap, int32_t nroot, const float* __restrict__ x,
     const float* __restrict__ y, const int32_t* __restrict__ id,
     const float4* __restrict__ ccom, const DevScalars* __restrict__ dev,
     float* __restrict__ out) {
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void output_kernel(
   }
   const int64_t c = i - n;
   if (c < dev->ncells && c < cap) {
-    const float4 q = ccom[c + 1];
+    const float4 q = ccom[c + nroot];
     reinterpret_cast<float2*>(out)[n + c] = make_float2(q.x, q.y);
   }
 }
@@ -116,25 +116,41 @@ __global__ __launch_bounds__(256) void by_id_kernel(
 
 constexpr int kTile = 256;
 
-// FP64 direct sum for a list of targets over all sources (sources staged
-// through LDS in tiles).
+// FP64 direct sum for a list of targets (body ids) over the sources of each
+// target's instance (sources staged through LDS in tiles over the union of
+// the block's instance ranges).
 __global__ __launch_bounds__(kTile) void direct_fp64_kernel(
-    int32_t n, int32_t nt, const int32_t* __restrict__ targets,
+    int32_t nt, const int32_t* __restrict__ targets,
+    const int32_t* __restrict__ inst_of, const int32_t* __restrict__ ioff,
     const float* __restrict__ xid, const float* __restrict__ yid,
     const float* __restrict__ x, const float* __restrict__ y,
     const float* __restrict__ m, double eps2, double* __restrict__ ax,
     double* __restrict__ ay) {
   __shared__ float sx[kTile], sy[kTile], sm[kTile];
+  __shared__ int32_t range[2];
   const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = q < nt;
   const int32_t b = active ? (targets ? targets[q] : q) : 0;
+  const int32_t inst = inst_of[b];
+  const int32_t lo = ioff[inst], hi = ioff[inst + 1];
+  if (threadIdx.x == 0) {
+    range[0] = INT32_MAX;
+    range[1] = 0;
+  }
+  __syncthreads();
+  if (active) {
+    atomicMin(&range[0], lo);
+    atomicMax(&range[1], hi);
+  }
+  __syncthreads();
+  const int32_t blo = range[0], bhi = range[1];
   const double xi = active ? (double)xid[b] : 0.0;
   const double yi = active ? (double)yid[b] : 0.0;
   double fx = 0.0, fy = 0.0;
-  for (int32_t base = 0; base < n; base += kTile) {
+  for (int32_t base = blo; base < bhi; base += kTile) {
     const int32_t j = base + threadIdx.x;
     __syncthreads();
-    if (j < n) {
+    if (j < bhi) {
       sx[threadIdx.x] = x[j];
       sy[threadIdx.x] = y[j];
       sm[threadIdx.x] = m[j];
@@ -144,8 +160,9 @@ __global__ __launch_bounds__(kTile) void direct_fp64_kernel(
       sm[threadIdx.x] = 0.f;
     }
     __syncthreads();
-    const int32_t cnt = (n - base) < kTile ? (n - base) : kTile;
-    for (int32_t k = 0; k < cnt; ++k) {
+    const int32_t k0 = lo > base ? lo - base : 0;
+    const int32_t k1 = (hi - base) < kTile ? (hi - base) : kTile;
+    for (int32_t k = k0; k < k1; ++k) {
       const double dx = (double)sx[k] - xi, dy = (double)sy[k] - yi;
       const double r2 = dx * dx + dy * dy + eps2;
       const double inv = 1.0 / (r2 * sqrt(r2));
@@ -159,41 +176,56 @@ __global__ __launch_bounds__(kTile) void direct_fp64_kernel(
   }
 }
 
-// Plummer-softened potential: per block, sum over its targets i of
-// m_i * sum_{j != i} m_j / sqrt(r^2 + eps2); kinetic energy and momentum of
-// the block's targets.  part[4*block + {0..3}] = pot, kin, px, py.
+// Plummer-softened potential, kinetic energy and momentum of kTile targets
+// (slice blockIdx.y of tile blockIdx.x, positions of one instance) summed
+// over the instance's sources: part[4*(t*kSlices + slice) + {0..3}] = pot,
+// kin, px, py with pot = -1/2 sum_i m_i sum_{j != i} m_j / sqrt(r^2 + eps2).
+constexpr int kSlices = kEnergySlices;
+static_assert(kSlices * kTile == kBboxTile, "slices cover a tile");
 __global__ __launch_bounds__(kTile) void energy_fp64_kernel(
-    int32_t n, const float* __restrict__ x, const float* __restrict__ y,
-    const float* __restrict__ vx, const float* __restrict__ vy,
-    const float* __restrict__ m, double eps2, double* __restrict__ part) {
+    const int32_t* __restrict__ tstart, const int32_t* __restrict__ inst_of,
+    const int32_t* __restrict__ ioff, const float* __restrict__ x,
+    const float* __restrict__ y, const float* __restrict__ vx,
+    const float* __restrict__ vy, const float* __restrict__ m, double eps2,
+    double* __restrict__ part) {
   __shared__ float sx[kTile], sy[kTile], sm[kTile];
   __shared__ double red[4][kTile / 64];
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i < n;
-  const double xi = active ? (double)x[i] : 0.0;
-  const double yi = active ? (double)y[i] : 0.0;
-  double pot = 0.0;
-  for (int32_t base = 0; base < n; base += kTile) {
-    const int32_t j = base + threadIdx.x;
-    __syncthreads();
-    sx[threadIdx.x] = j < n ? x[j] : 0.f;
-    sy[threadIdx.x] = j < n ? y[j] : 0.f;
-    sm[threadIdx.x] = j < n ? m[j] : 0.f;
-    __syncthreads();
-    const int32_t cnt = (n - base) < kTile ? (n - base) : kTile;
-    for (int32_t k = 0; k < cnt; ++k) {
-      if (base + k == i) continue;
-      const double dx = (double)sx[k] - xi, dy = (double)sy[k] - yi;
-      pot += (double)sm[k] / sqrt(dx * dx + dy * dy + eps2);
-    }
-  }
+  const int32_t t0 = tstart[blockIdx.x], t1 = tstart[blockIdx.x + 1];
+  const int32_t inst = inst_of[t0];
+  const int32_t lo = ioff[inst], hi = ioff[inst + 1];
   double vals[4] = {0.0, 0.0, 0.0, 0.0};
-  if (active) {
-    const double mi = m[i], u = vx[i], v = vy[i];
-    vals[0] = -0.5 * mi * pot;
-    vals[1] = 0.5 * mi * (u * u + v * v);
-    vals[2] = mi * u;
-    vals[3] = mi * v;
+  const int32_t tb = t0 + (int32_t)blockIdx.y * kTile;
+  if (tb >= t1) {  // a short tile: this slice has no targets
+    if (threadIdx.x < 4) part[4 * ((int64_t)blockIdx.x * kSlices + blockIdx.y) + threadIdx.x] = 0.0;
+    return;
+  }
+  {
+    const int32_t i = tb + threadIdx.x;
+    const bool active = i < t1;
+    const double xi = active ? (double)x[i] : 0.0;
+    const double yi = active ? (double)y[i] : 0.0;
+    double pot = 0.0;
+    for (int32_t base = lo; base < hi; base += kTile) {
+      const int32_t j = base + threadIdx.x;
+      __syncthreads();
+      sx[threadIdx.x] = j < hi ? x[j] : 0.f;
+      sy[threadIdx.x] = j < hi ? y[j] : 0.f;
+      sm[threadIdx.x] = j < hi ? m[j] : 0.f;
+      __syncthreads();
+      const int32_t cnt = (hi - base) < kTile ? (hi - base) : kTile;
+      for (int32_t k = 0; k < cnt; ++k) {
+        if (base + k == i) continue;
+        const double dx = (double)sx[k] - xi, dy = (double)sy[k] - yi;
+        pot += (double)sm[k] / sqrt(dx * dx + dy * dy + eps2);
+      }
+    }
+    if (active) {
+      const double mi = m[i], u = vx[i], v = vy[i];
+      vals[0] += -0.5 * mi * pot;
+      vals[1] += 0.5 * mi * (u * u + v * v);
+      vals[2] += mi * u;
+      vals[3] += mi * v;
+    }
   }
   for (int k = 0; k < 4; ++k)
     for (int off = 32; off > 0; off >>= 1) vals[k] += __shfl_xor(vals[k], off);
@@ -205,7 +237,7 @@ __global__ __launch_bounds__(kTile) void energy_fp64_kernel(
     for (int k = 0; k < 4; ++k) {
       double s = 0.0;
       for (int q = 0; q < kTile / 64; ++q) s += red[k][q];
-      part[4 * (int64_t)blockIdx.x + k] = s;
+      part[4 * ((int64_t)blockIdx.x * kSlices + blockIdx.y) + k] = s;
     }
   }
 }
@@ -241,7 +273,7 @@ void launch_output(stellar_ctx* c) {
   if (n == 0) return;
   Buffers& B = c->b;
   output_kernel<<<grid_for(n + c->cap, 256), 256, 0, c->stream>>>(
-      n, c->cap, B.x, B.y, B.id, B.ccom, B.dev, B.out);
+      n, c->cap, c->ninst, B.x, B.y, B.id, B.ccom, B.dev, B.out);
 }
 
 void launch_zero_counter(stellar_ctx* c) {
@@ -256,17 +288,17 @@ void launch_direct_fp64(stellar_ctx* c, int32_t nt, const int32_t* dtargets,
   by_id_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, B.x, B.y, B.id,
                                                          B.x2, B.y2);
   direct_fp64_kernel<<<grid_for(nt, kTile), kTile, 0, c->stream>>>(
-      n, nt, dtargets, B.x2, B.y2, B.x, B.y, B.m, eps2, dax, day);
+      nt, dtargets, B.inst_of, B.ioff, B.x2, B.y2, B.x, B.y, B.m, eps2, dax, day);
 }
 
-int energy_parts(stellar_ctx* c) { return (int)grid_for(c->n, kTile); }
+int energy_parts(stellar_ctx* c) { return c->ntiles * kSlices; }
 
 void launch_energy_fp64(stellar_ctx* c, double eps2, double* dpart,
                         int32_t nparts) {
   (void)nparts;
   Buffers& B = c->b;
-  energy_fp64_kernel<<<grid_for(c->n, kTile), kTile, 0, c->stream>>>(
-      c->n, B.x, B.y, B.vx, B.vy, B.m, eps2, dpart);
+  energy_fp64_kernel<<<dim3((unsigned)c->ntiles, kSlices), kTile, 0, c->stream>>>(
+      B.tstart, B.inst_of, B.ioff, B.x, B.y, B.vx, B.vy, B.m, eps2, dpart);
 }
 
 }  // namespace stellar

@@ -35,31 +35,22 @@ constexpr int kBboxThreads = 256;
 __device__ __forceinline__ float mn(float a, float v) { return v < a ? v : a; }
 __device__ __forceinline__ float mx(float a, float v) { return v > a ? v : a; }
 
-// Pass 1: per-block min/max into part[blockIdx]; pass 2 (one block) folds
-// the partials.  Two launches instead of a last-block ticket: the ticket
-// pattern needs a device-scope release per block, which on gfx950 writes
-// back the issuing XCD's L2.
+// Pass 1: min/max of one tile (kBboxTile positions of one instance) into
+// part[tile]; pass 2: one wavefront per instance folds its tiles.  Two
+// launches instead of a last-block ticket: the ticket pattern needs a
+// device-scope release per block, which on gfx950 writes back the issuing
+// XCD's L2.
 __global__ __launch_bounds__(kBboxThreads) void bbox_kernel(
-    const float* __restrict__ x, const float* __restrict__ y, int32_t n,
-    float4* __restrict__ part) {
+    const float* __restrict__ x, const float* __restrict__ y,
+    const int32_t* __restrict__ tstart, float4* __restrict__ part) {
   float l = 0.f, r = 0.f, b = 0.f, t = 0.f;
-  const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int32_t stride = gridDim.x * blockDim.x;
-  const int32_t n4 = n >> 2;
-  const float4* x4 = reinterpret_cast<const float4*>(x);
-  const float4* y4 = reinterpret_cast<const float4*>(y);
-  for (int32_t i = tid; i < n4; i += stride) {
-    const float4 a = x4[i], c = y4[i];
-    l = mn(mn(mn(mn(l, a.x), a.y), a.z), a.w);
-    r = mx(mx(mx(mx(r, a.x), a.y), a.z), a.w);
-    b = mn(mn(mn(mn(b, c.x), c.y), c.z), c.w);
-    t = mx(mx(mx(mx(t, c.x), c.y), c.z), c.w);
-  }
-  for (int32_t i = 4 * n4 + tid; i < n; i += stride) {
-    l = mn(l, x[i]);
-    r = mx(r, x[i]);
-    b = mn(b, y[i]);
-    t = mx(t, y[i]);
+  const int32_t lo = tstart[blockIdx.x], hi = tstart[blockIdx.x + 1];
+  for (int32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const float px = x[i], py = y[i];
+    l = mn(l, px);
+    r = mx(r, px);
+    b = mn(b, py);
+    t = mx(t, py);
   }
   // wavefront reduction
   for (int off = 32; off > 0; off >>= 1) {
@@ -89,10 +80,13 @@ __global__ __launch_bounds__(kBboxThreads) void bbox_kernel(
 }
 
 __global__ __launch_bounds__(kBboxThreads) void bbox_final_kernel(
-    const float4* __restrict__ part, int32_t nparts,
-    DevScalars* __restrict__ dev) {
+    const float4* __restrict__ part, const int32_t* __restrict__ itile,
+    int32_t ninst, float4* __restrict__ ibbox) {
+  const int32_t inst = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (inst >= ninst) return;  // whole wavefronts
   float l = 0.f, r = 0.f, b = 0.f, t = 0.f;
-  for (int32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+  for (int32_t i = itile[inst] + lane; i < itile[inst + 1]; i += 64) {
     const float4 q = part[i];
     l = mn(l, q.x);
     r = mx(r, q.y);
@@ -105,27 +99,7 @@ __global__ __launch_bounds__(kBboxThreads) void bbox_final_kernel(
     b = mn(b, __shfl_xor(b, off));
     t = mx(t, __shfl_xor(t, off));
   }
-  __shared__ float red[4][kBboxThreads / 64];
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    red[0][w] = l;
-    red[1][w] = r;
-    red[2][w] = b;
-    red[3][w] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < kBboxThreads / 64; ++k) {
-      l = mn(l, red[0][k]);
-      r = mx(r, red[1][k]);
-      b = mn(b, red[2][k]);
-      t = mx(t, red[3][k]);
-    }
-    dev->bbox[0] = l;
-    dev->bbox[1] = r;
-    dev->bbox[2] = b;
-    dev->bbox[3] = t;
-  }
+  if (lane == 0) ibbox[inst] = make_float4(l, r, b, t);
 }
 
 __global__ void init_scalars_kernel(DevScalars* dev) {
@@ -166,41 +140,63 @@ __device__ __forceinline__ uint32_t split2(float px, float py, float& l,
   return cx | (cy << 1);
 }
 
+// Sort keys.  With one instance a key holds slot codes 1..32.  With several,
+// its top ibits bits hold the instance index, so that one sort orders
+// (instance, path) and every instance keeps its own position range, and the
+// codes below hold levels 1..l1 = 32 - ibits/2.  Levels past l1 are computed
+// only inside runs of equal keys (ties_kernel): l1+1..l1+32 in key2, the
+// last ibits/2 in key3 (left-aligned).
+struct KeyView {
+  const uint64_t* key;
+  const uint64_t* key2;
+  const uint32_t* key3;
+  int l1;
+  int ibits;
+};
+
 __global__ __launch_bounds__(256) void keys_kernel(
     const float* __restrict__ x, const float* __restrict__ y, int32_t n,
-    const DevScalars* __restrict__ dev, uint64_t* __restrict__ key,
-    uint32_t* __restrict__ perm) {
+    const int32_t* __restrict__ inst_of, const float4* __restrict__ ibbox,
+    int l1, int ibits, uint64_t* __restrict__ key, uint32_t* __restrict__ perm) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float l = dev->bbox[0], r = dev->bbox[1], b = dev->bbox[2], t = dev->bbox[3];
+  const int32_t inst = inst_of[i];
+  const float4 bb = ibbox[inst];
+  float l = bb.x, r = bb.y, b = bb.z, t = bb.w;
   const float px = x[i], py = y[i];
   uint64_t k = 0;
 #pragma unroll 4
-  for (int lev = 0; lev < 32; ++lev) k = (k << 2) | split2(px, py, l, r, b, t);
+  for (int lev = 0; lev < l1; ++lev) k = (k << 2) | split2(px, py, l, r, b, t);
+  if (ibits > 0) k |= (uint64_t)inst << (64 - ibits);
   key[i] = k;
   perm[i] = (uint32_t)i;
 }
 
-// Levels 33..64 of one body (its path is replayed from the root).
-__device__ uint64_t deep_key(float px, float py, const float* bbox) {
-  float l = bbox[0], r = bbox[1], b = bbox[2], t = bbox[3];
-  for (int lev = 0; lev < 32; ++lev) (void)split2(px, py, l, r, b, t);
-  uint64_t k = 0;
-  for (int lev = 0; lev < 32; ++lev) k = (k << 2) | split2(px, py, l, r, b, t);
-  return k;
+// Levels l1+1..l1+32 (k2) and l1+33..64 (k3, left-aligned) of one body
+// (its path is replayed from the root).
+__device__ void deep_key(float px, float py, float4 bb, int l1, uint64_t& k2,
+                         uint32_t& k3) {
+  float l = bb.x, r = bb.y, b = bb.z, t = bb.w;
+  for (int lev = 0; lev < l1; ++lev) (void)split2(px, py, l, r, b, t);
+  k2 = 0;
+  for (int lev = 0; lev < 32; ++lev) k2 = (k2 << 2) | split2(px, py, l, r, b, t);
+  k3 = 0;
+  const int rest = kMaxLevel - l1 - 32;
+  for (int lev = 0; lev < rest; ++lev) k3 |= split2(px, py, l, r, b, t) << (30 - 2 * lev);
 }
 
-// Runs of equal 32-level keys (bodies closer than a level-32 cell): order
-// each run by (levels 33..64, body id) with an in-place insertion sort.
-// Such runs are rare and short; longer than kMaxTieRun is an error.
+// Runs of equal keys (bodies of one instance closer than a level-l1 cell):
+// order each run by (deeper levels, body id) with an in-place insertion
+// sort.  Such runs are rare and short; longer than kMaxTieRun is an error.
 __global__ __launch_bounds__(256) void ties_kernel(
     const float* __restrict__ x, const float* __restrict__ y,
     const int32_t* __restrict__ id, int32_t n, const uint64_t* __restrict__ key,
+    const int32_t* __restrict__ inst_of, const float4* __restrict__ ibbox, int l1,
     uint32_t* __restrict__ perm, uint64_t* __restrict__ key2,
-    DevScalars* __restrict__ dev) {
+    uint32_t* __restrict__ key3, DevScalars* __restrict__ dev) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s + 1 >= n) return;
-  const uint64_t k = key[s];
+  const uint64_t k = key[s];  // the instance is part of the key
   if (key[s + 1] != k || (s > 0 && key[s - 1] == k)) return;
   int32_t e = s + 1;
   while (e + 1 < n && key[e + 1] == k && e - s + 1 < kMaxTieRun) ++e;
@@ -208,44 +204,47 @@ __global__ __launch_bounds__(256) void ties_kernel(
     atomicOr(&dev->err, kErrTieRun);
     return;
   }
-  float bbox[4] = {dev->bbox[0], dev->bbox[1], dev->bbox[2], dev->bbox[3]};
+  const float4 bb = ibbox[inst_of[s]];
   for (int32_t j = s; j <= e; ++j) {
     const uint32_t p = perm[j];
-    key2[j] = deep_key(x[p], y[p], bbox);
+    deep_key(x[p], y[p], bb, l1, key2[j], key3[j]);
   }
   for (int32_t j = s + 1; j <= e; ++j) {
     const uint64_t k2 = key2[j];
+    const uint32_t k3 = key3[j];
     const uint32_t p = perm[j];
     const int32_t b = id[p];
     int32_t q = j - 1;
-    while (q >= s &&
-           (key2[q] > k2 || (key2[q] == k2 && id[perm[q]] > b))) {
+    while (q >= s && (key2[q] > k2 ||
+                      (key2[q] == k2 && (key3[q] > k3 || (key3[q] == k3 && id[perm[q]] > b))))) {
       key2[q + 1] = key2[q];
+      key3[q + 1] = key3[q];
       perm[q + 1] = perm[q];
       --q;
     }
     key2[q + 1] = k2;
+    key3[q + 1] = k3;
     perm[q + 1] = p;
   }
 }
 
-// Number of leading slot codes shared by tree positions s and t (s < t).
-// Equal 32-level keys only occur inside a tie run, where key2 is valid.
-__device__ __forceinline__ int common_levels(const uint64_t* __restrict__ key,
-                                             const uint64_t* __restrict__ key2,
-                                             int32_t s, int32_t t) {
-  const uint64_t a = key[s] ^ key[t];
-  if (a) return __clzll(a) >> 1;
-  const uint64_t c = key2[s] ^ key2[t];
-  if (c) return 32 + (__clzll(c) >> 1);
+// Number of leading slot codes shared by tree positions s and t (s < t) of
+// one instance.  Equal keys only occur inside a tie run, where key2 and key3
+// are valid.
+__device__ __forceinline__ int common_levels(const KeyView& K, int32_t s, int32_t t) {
+  const uint64_t a = K.key[s] ^ K.key[t];
+  if (a) return (__clzll(a) - K.ibits) >> 1;
+  const uint64_t c = K.key2[s] ^ K.key2[t];
+  if (c) return K.l1 + (__clzll(c) >> 1);
+  const uint32_t d = K.key3[s] ^ K.key3[t];
+  if (d) return K.l1 + 32 + (__clz(d) >> 1);
   return kMaxLevel;
 }
 
-__device__ __forceinline__ int digit_at(const uint64_t* __restrict__ key,
-                                        const uint64_t* __restrict__ key2,
-                                        int32_t s, int lev) {
-  return lev <= 32 ? (int)((key[s] >> (2 * (32 - lev))) & 3u)
-                   : (int)((key2[s] >> (2 * (64 - lev))) & 3u);
+__device__ __forceinline__ int digit_at(const KeyView& K, int32_t s, int lev) {
+  if (lev <= K.l1) return (int)((K.key[s] >> (2 * (K.l1 - lev))) & 3u);
+  if (lev <= K.l1 + 32) return (int)((K.key2[s] >> (2 * (K.l1 + 32 - lev))) & 3u);
+  return (int)((K.key3[s] >> (30 - 2 * (lev - K.l1 - 33))) & 3u);
 }
 
 // Minimum tree over lam: level 0 is lam itself (one byte per tree position,
@@ -282,15 +281,17 @@ MinTree mintree_layout(int32_t n, const uint8_t* lam, const uint8_t* lmin,
 // clears tree blocks [0, nclear) with coalesced 16-byte stores (the cells
 // kernel clears any block from n on).
 __global__ __launch_bounds__(256) void lambda_kernel(
-    int32_t n, const uint64_t* __restrict__ key,
-    const uint64_t* __restrict__ key2, int32_t* __restrict__ ncell,
-    uint8_t* __restrict__ lam, uint8_t* __restrict__ lmin1,
-    float4* __restrict__ blk, int64_t nclear) {
+    int32_t n, KeyView K, const int32_t* __restrict__ inst_of,
+    int32_t* __restrict__ ncell, uint8_t* __restrict__ lam,
+    uint8_t* __restrict__ lmin1, float4* __restrict__ blk, int64_t nclear) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   int v = 0, lm = 0;
   if (s + 1 < n) {
-    lm = common_levels(key, key2, s, s + 1);
-    const int prev = s > 0 ? common_levels(key, key2, s - 1, s) : 0;
+    // a pair that straddles two instances shares no level (lam = 0, like the
+    // end of the last instance)
+    const int32_t is = inst_of[s];
+    lm = inst_of[s + 1] == is ? common_levels(K, s, s + 1) : 0;
+    const int prev = s > 0 && inst_of[s - 1] == is ? common_levels(K, s - 1, s) : 0;
     v = lm > prev ? lm - prev : 0;
   }
   if (s < n) {
@@ -397,8 +398,7 @@ __device__ int32_t psv(const MinTree& t, int32_t i, uint32_t v) {
 struct LinkArgs {
   int32_t n;
   int64_t cap;
-  const uint64_t* key;
-  const uint64_t* key2;
+  KeyView K;
   const int32_t* coff;
   const int32_t* ncell;
   const uint8_t* lam;
@@ -412,24 +412,31 @@ struct LinkArgs {
   uint8_t* clevel;
   uint8_t* cslot;
   DevScalars* dev;
+  const int32_t* inst_of;
+  const int32_t* ioff;
+  int32_t nroot;  // number of instances = root blocks; cell c is block nroot + c
   MinTree mt;
 };
 
 // One thread per adjacent pair s: creates the cells that pair opens, at
 // levels lam(s-1)+1 .. lam(s), with ids coff[s] + k, and links each cell but
-// the first into the one before it (its parent).  The thread of position n-1
-// (no pair) records the cell count and the root's metadata.
+// the first into the one before it (its parent).  The thread of the last
+// position of an instance (no pair) records the root's metadata; that of
+// position n-1 also the cell count.
 __global__ __launch_bounds__(256) void cells_kernel(LinkArgs A) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= A.n) return;
   const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (s == A.n - 1) {
-    const int64_t total = (int64_t)A.coff[s] + A.ncell[s];
-    if (total > A.cap) atomicOr(&A.dev->err, kErrPool);
-    A.dev->ncells = (int32_t)(total < A.cap ? total : A.cap);
-    A.cstart[0] = 0;
-    A.clevel[0] = 0;
-    A.cparent[0] = -1;
+  const int32_t inst = A.inst_of[s];
+  if (s == A.n - 1 || A.inst_of[s + 1] != inst) {
+    if (s == A.n - 1) {
+      const int64_t total = (int64_t)A.coff[s] + A.ncell[s];
+      if (total > A.cap) atomicOr(&A.dev->err, kErrPool);
+      A.dev->ncells = (int32_t)(total < A.cap ? total : A.cap);
+    }
+    A.cstart[inst] = A.ioff[inst];
+    A.clevel[inst] = 0;
+    A.cparent[inst] = -1;
     return;
   }
   const int nc = A.ncell[s];
@@ -439,7 +446,7 @@ __global__ __launch_bounds__(256) void cells_kernel(LinkArgs A) {
   for (int k = 0; k < nc; ++k) {
     const int64_t c = base + k;
     if (c >= A.cap) break;
-    const int32_t b = (int32_t)(c + 1);
+    const int32_t b = (int32_t)(c + A.nroot);
     const int L = lp + 1 + k;
     if (b >= A.n) {  // blocks below n were cleared by the lambda kernel
       float4* d = reinterpret_cast<float4*>(A.blk + 4 * (int64_t)b);
@@ -450,7 +457,7 @@ __global__ __launch_bounds__(256) void cells_kernel(LinkArgs A) {
     }
     A.cstart[b] = s;
     A.clevel[b] = (uint8_t)L;
-    const int dg = digit_at(A.key, A.key2, s, L);
+    const int dg = digit_at(A.K, s, L);
     A.cslot[b] = (uint8_t)dg;
     if (k > 0) {
       A.cparent[b] = b - 1;
@@ -468,9 +475,9 @@ __global__ __launch_bounds__(256) void cells_kernel(LinkArgs A) {
 __global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= A.n) return;
-  const int lp = s > 0 ? A.lam[s - 1] : 0;
+  const int lp = s > 0 ? A.lam[s - 1] : 0;  // 0 at an instance's first position
   const int nc = s + 1 < A.n ? A.ncell[s] : 0;
-  int64_t pid = -1;  // cell id of the level-lp node (-1: the root)
+  int64_t pid = -1;  // cell id of the level-lp node (-1: the instance's root)
   int32_t a = 0;
   if (lp > 0) {
     a = psv(A.mt, s - 1, (uint32_t)lp) + 1;
@@ -478,13 +485,13 @@ __global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
     pid = (int64_t)A.coff[a] + (lp - la - 1);
   }
   const bool pok = pid < A.cap;
-  const int32_t pblk = (int32_t)(pid + 1);
+  const int32_t pblk = pid < 0 ? A.inst_of[s] : (int32_t)(pid + A.nroot);
   if (nc > 0) {
     const int64_t c0 = A.coff[s];
     if (c0 < A.cap && pok) {
-      const int32_t b = (int32_t)(c0 + 1);
+      const int32_t b = (int32_t)(c0 + A.nroot);
       A.cparent[b] = pblk;
-      const int dg = digit_at(A.key, A.key2, s, lp + 1);
+      const int dg = digit_at(A.K, s, lp + 1);
       A.blk[4 * (int64_t)pblk + dg] = ChildRec{0.f, 0.f, 0.f, b};
     }
   }
@@ -493,14 +500,14 @@ __global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
   if (nc > 0) {
     const int64_t c = (int64_t)A.coff[s] + nc - 1;
     if (c >= A.cap) return;
-    bid = c + 1;
+    bid = c + A.nroot;
     const int lev = lp + nc;
-    slot = lev < kMaxLevel ? digit_at(A.key, A.key2, s, lev + 1) : 0;
+    slot = lev < kMaxLevel ? digit_at(A.K, s, lev + 1) : 0;
   } else {
     if (!pok) return;
     bid = pblk;
     if (lp < kMaxLevel) {
-      slot = digit_at(A.key, A.key2, s, lp + 1);
+      slot = digit_at(A.K, s, lp + 1);
     } else {
       slot = s - a;
       if (slot > 3) {
@@ -537,22 +544,30 @@ void launch_init_scalars(stellar_ctx* c) {
 }
 
 void launch_bbox(stellar_ctx* c) {
-  const int32_t n = c->n;
-  int blocks = (int)grid_for((n + 3) / 4, kBboxThreads * 4);
-  if (blocks < 1) blocks = 1;
-  if (blocks > kBboxMaxBlocks) blocks = kBboxMaxBlocks;
-  bbox_kernel<<<blocks, kBboxThreads, 0, c->stream>>>(c->b.x, c->b.y, n,
-                                                       c->b.bbox_part);
-  bbox_final_kernel<<<1, kBboxThreads, 0, c->stream>>>(c->b.bbox_part, blocks,
-                                                        c->b.dev);
+  if (c->n == 0) return;
+  Buffers& B = c->b;
+  bbox_kernel<<<(unsigned)c->ntiles, kBboxThreads, 0, c->stream>>>(B.x, B.y, B.tstart,
+                                                                    B.bbox_part);
+  const unsigned waves_per_block = kBboxThreads / 64;
+  bbox_final_kernel<<<(unsigned)((c->ninst + waves_per_block - 1) / waves_per_block),
+                      kBboxThreads, 0, c->stream>>>(B.bbox_part, B.itile, c->ninst, B.ibbox);
+}
+
+// Instance bits of the sort key (even, so that slot codes stay 2-bit aligned).
+int key_instance_bits(int32_t ninst) {
+  int b = 0;
+  while (b < 32 && ((int64_t)1 << b) < ninst) b += 2;
+  return ninst > 1 ? b : 0;
 }
 
 void launch_build(stellar_ctx* c) {
   Buffers& B = c->b;
   const int32_t n = c->n;
   if (n == 0) return;
-  keys_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, n, B.dev,
-                                                        B.key, B.perm);
+  const int ibits = key_instance_bits(c->ninst);
+  const int l1 = 32 - ibits / 2;
+  keys_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, n, B.inst_of, B.ibbox,
+                                                        l1, ibits, B.key, B.perm);
   rocprim::double_buffer<uint64_t> kb(B.key, B.key_alt);
   rocprim::double_buffer<uint32_t> vb(B.perm, B.perm_alt);
   size_t bytes = B.tmp_bytes;
@@ -561,21 +576,24 @@ void launch_build(stellar_ctx* c) {
   // keep the sorted data in the primary buffers
   if (kb.current() != B.key) std::swap(B.key, B.key_alt);
   if (vb.current() != B.perm) std::swap(B.perm, B.perm_alt);
-  ties_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, B.id, n, B.key,
-                                                        B.perm, B.key2, B.dev);
+  ties_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, B.id, n, B.key, B.inst_of,
+                                                        B.ibbox, l1, B.perm, B.key2, B.key3,
+                                                        B.dev);
+  const KeyView K{B.key, B.key2, B.key3, l1, ibits};
   int64_t mtb = 0;
   const MinTree mt = mintree_layout(n, B.lam, B.lmin, &mtb);
-  const int64_t nclear = (int64_t)n < c->cap + 1 ? (int64_t)n : c->cap + 1;
+  const int64_t nclear = (int64_t)n < c->cap + c->ninst ? (int64_t)n : c->cap + c->ninst;
   lambda_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(
-      n, B.key, B.key2, B.ncell, B.lam, mt.top >= 1 ? const_cast<uint8_t*>(mt.lv[1]) : nullptr,
+      n, K, B.inst_of, B.ncell, B.lam, mt.top >= 1 ? const_cast<uint8_t*>(mt.lv[1]) : nullptr,
       reinterpret_cast<float4*>(B.blk), nclear);
   if (mt.top >= 2) mintree_kernel<<<1, 1024, 0, c->stream>>>(mt);
   bytes = B.tmp_bytes;
   (void)rocprim::exclusive_scan(B.tmp, bytes, B.ncell, B.coff, 0, (size_t)n,
                                 rocprim::plus<int32_t>(), c->stream);
-  LinkArgs A{n,      c->cap,   B.key,   B.key2,   B.coff,    B.ncell,
-             B.lam,  B.perm,   B.x,     B.y,      B.m,       B.blk,
-             B.cstart, B.cparent, B.clevel, B.cslot, B.dev,   mt};
+  LinkArgs A{n,         c->cap,   K,       B.coff,    B.ncell,  B.lam,
+             B.perm,    B.x,      B.y,     B.m,       B.blk,    B.cstart,
+             B.cparent, B.clevel, B.cslot, B.dev,     B.inst_of, B.ioff,
+             c->ninst,  mt};
   cells_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(A);
   link_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(A);
 }

@@ -73,7 +73,9 @@ void free_all(stellar_ctx* c) {
                   B.cstart, B.lam, B.cparent, B.clevel, B.cslot,
                   B.csum,  B.ccom,    B.out,    B.dev,
                   B.bbox_part, B.scount, B.choff,
-                  B.lmin,  B.qblk,  B.spine,   B.clist, B.clen};
+                  B.lmin,  B.qblk,  B.spine,   B.clist, B.clen,
+                  B.key3,  B.ioff,  B.inst_of, B.ibbox, B.tstart, B.itile,
+                  B.gtab[0], B.gtab[1], B.gtab[2]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (B.host_dev) (void)hipHostFree(B.host_dev);
@@ -187,24 +189,70 @@ const char* stellar_last_error(const stellar_ctx* ctx) {
   return ctx ? ctx->err.c_str() : "null context";
 }
 
-int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
-                   stellar_ctx** out) {
-  if (!out || n < 0 || !params_ok(p)) return STELLAR_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// Host tables of a context's instance layout: positions of instance i are
+// [ioff[i], ioff[i+1]); bounding-box tiles of at most kBboxTile positions and
+// walk groups of G positions never straddle two instances.
+struct Layout {
+  std::vector<int32_t> inst_of, tstart, itile, gtab[3];
+};
+
+Layout make_layout(const std::vector<int32_t>& ioff) {
+  Layout L;
+  const int32_t ninst = (int32_t)ioff.size() - 1;
+  const int32_t n = ioff[(size_t)ninst];
+  L.inst_of.resize((size_t)(n > 0 ? n : 1), 0);
+  const int sizes[3] = {64, 32, 16};
+  for (int32_t i = 0; i < ninst; ++i) {
+    const int32_t lo = ioff[(size_t)i], hi = ioff[(size_t)i + 1];
+    for (int32_t s = lo; s < hi; ++s) L.inst_of[(size_t)s] = i;
+    L.itile.push_back((int32_t)L.tstart.size());
+    for (int32_t s = lo; s < hi; s += kBboxTile) L.tstart.push_back(s);
+    for (int k = 0; k < 3; ++k)
+      for (int32_t s = lo; s < hi; s += sizes[k]) L.gtab[k].push_back(s);
+  }
+  L.itile.push_back((int32_t)L.tstart.size());
+  L.tstart.push_back(n);
+  for (int k = 0; k < 3; ++k) L.gtab[k].push_back(n);
+  return L;
+}
+
+int create_ctx(const stellar_params* p, int32_t ninst, const int32_t* counts,
+               int64_t node_capacity, stellar_ctx** out) {
+  if (!out || ninst < 1 || ninst > 65536 || !counts || !params_ok(p)) return STELLAR_ERR_ARG;
   *out = nullptr;
+  std::vector<int32_t> ioff(1, 0);
+  for (int32_t i = 0; i < ninst; ++i) {
+    if (counts[i] < (ninst > 1 ? 1 : 0)) return STELLAR_ERR_ARG;
+    const int64_t next = (int64_t)ioff.back() + counts[i];
+    if (next > INT32_MAX / 2) return STELLAR_ERR_ARG;
+    ioff.push_back((int32_t)next);
+  }
+  const int32_t n = ioff.back();
   stellar_ctx* c = new (std::nothrow) stellar_ctx();
   if (!c) return STELLAR_ERR_OOM;
   c->p = *p;
   c->n = n;
-  c->cap = node_capacity > 0 ? node_capacity : 2 * (int64_t)n + 12000;
+  c->ninst = ninst;
+  c->ioff = ioff;
+  c->cap = node_capacity > 0 ? node_capacity : 2 * (int64_t)n + 12000 * (int64_t)ninst;
   // the walk's stack words keep a block index in 26 bits
-  if (c->cap + 1 >= kMaxBlocks) {
+  if (c->cap + ninst >= kMaxBlocks) {
     delete c;
     return STELLAR_ERR_ARG;
   }
+  const Layout lay = make_layout(ioff);
+  c->ntiles = (int32_t)lay.tstart.size() - 1;
+  c->itile = lay.itile;
+  for (int k = 0; k < 3; ++k) c->ngroups[k] = (int64_t)lay.gtab[k].size() - 1;
   (void)hipGetDevice(&c->device);
   Buffers& B = c->b;
   const size_t N = (size_t)(n > 0 ? n : 1);
   const size_t C = (size_t)c->cap;
+  const size_t NB = C + (size_t)ninst;  // root blocks + cells
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) {
     if (e == hipSuccess && r != hipSuccess) e = r;
@@ -230,29 +278,36 @@ int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
   chk(dmalloc(&B.perm, N));
   chk(dmalloc(&B.perm_alt, N));
   chk(dmalloc(&B.key2, N));
+  chk(dmalloc(&B.key3, N));
   chk(dmalloc(&B.ncell, N));
   chk(dmalloc(&B.coff, N));
   chk(dmalloc(&B.lam, (size_t)N + 64));
   chk(dmalloc(&B.lmin, mintree_bytes((int32_t)N)));
   B.tmp_bytes = build_scratch_bytes((int32_t)N);
   chk(hipMalloc(&B.tmp, B.tmp_bytes));
-  chk(dmalloc(&B.blk, 4 * (C + 1)));
-  chk(dmalloc(&B.cstart, C + 1));
-  chk(dmalloc(&B.cparent, C + 1));
-  chk(dmalloc(&B.clevel, C + 1));
-  chk(dmalloc(&B.cslot, C + 1));
-  chk(dmalloc(&B.csum, kSumStride * (C + 1)));
-  chk(dmalloc(&B.qblk, 4 * (C + 1)));
+  chk(dmalloc(&B.blk, 4 * NB));
+  chk(dmalloc(&B.cstart, NB));
+  chk(dmalloc(&B.cparent, NB));
+  chk(dmalloc(&B.clevel, NB));
+  chk(dmalloc(&B.cslot, NB));
+  chk(dmalloc(&B.csum, kSumStride * NB));
+  chk(dmalloc(&B.qblk, 4 * NB));
   const size_t clcap = (size_t)com_list_capacity(c->cap);
   chk(dmalloc(&B.spine, clcap));
   chk(dmalloc(&B.scount, clcap / kMaxLevel + 64));
   chk(dmalloc(&B.choff, clcap / kMaxLevel + 64));
   chk(dmalloc(&B.clist, 2 * clcap));
   chk(dmalloc(&B.clen, 64));
-  chk(dmalloc(&B.ccom, C + 1));
+  chk(dmalloc(&B.ccom, NB));
   chk(dmalloc(&B.out, 2 * (N + C)));
   chk(dmalloc(&B.dev, 1));
-  chk(dmalloc(&B.bbox_part, kBboxMaxBlocks));
+  chk(dmalloc(&B.bbox_part, (size_t)c->ntiles));
+  chk(dmalloc(&B.ioff, (size_t)ninst + 1));
+  chk(dmalloc(&B.inst_of, N));
+  chk(dmalloc(&B.ibbox, (size_t)ninst));
+  chk(dmalloc(&B.tstart, lay.tstart.size()));
+  chk(dmalloc(&B.itile, lay.itile.size()));
+  for (int k = 0; k < 3; ++k) chk(dmalloc(&B.gtab[k], lay.gtab[k].size()));
   chk(hipHostMalloc(reinterpret_cast<void**>(&B.host_dev), sizeof(DevScalars),
                     hipHostMallocDefault));
   for (int k = 0; k < 8; ++k) chk(hipEventCreate(&c->ev[k]));
@@ -265,10 +320,19 @@ int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
     (void)hipGetLastError();
     return e == hipErrorOutOfMemory ? STELLAR_ERR_OOM : STELLAR_ERR_DEVICE;
   }
-  // clear the output buffer and the cell records once
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    chk(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  };
+  up(B.ioff, ioff.data(), ioff.size() * 4);
+  up(B.inst_of, lay.inst_of.data(), lay.inst_of.size() * 4);
+  up(B.tstart, lay.tstart.data(), lay.tstart.size() * 4);
+  up(B.itile, lay.itile.data(), lay.itile.size() * 4);
+  for (int k = 0; k < 3; ++k) up(B.gtab[k], lay.gtab[k].data(), lay.gtab[k].size() * 4);
+  // clear the output buffer, the tree records and the boxes once
   chk(hipMemsetAsync(B.out, 0, 2 * (N + C) * sizeof(float), c->stream));
-  chk(hipMemsetAsync(B.blk, 0, 4 * (C + 1) * sizeof(ChildRec), c->stream));
-  chk(hipMemsetAsync(B.qblk, 0, 4 * (C + 1) * sizeof(float4), c->stream));
+  chk(hipMemsetAsync(B.blk, 0, 4 * NB * sizeof(ChildRec), c->stream));
+  chk(hipMemsetAsync(B.qblk, 0, 4 * NB * sizeof(float4), c->stream));
+  chk(hipMemsetAsync(B.ibbox, 0, (size_t)ninst * sizeof(float4), c->stream));
   chk(hipMemsetAsync(B.ax, 0, N * sizeof(float), c->stream));
   chk(hipMemsetAsync(B.ay, 0, N * sizeof(float), c->stream));
   launch_init_scalars(c);
@@ -280,6 +344,29 @@ int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
     return rc;
   }
   *out = c;
+  return STELLAR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int stellar_create(const stellar_params* p, int32_t n, int64_t node_capacity,
+                   stellar_ctx** out) {
+  if (n < 0) return STELLAR_ERR_ARG;
+  return create_ctx(p, 1, &n, node_capacity, out);
+}
+
+int stellar_batch_create(const stellar_params* p, int32_t ninst, const int32_t* counts,
+                         int64_t node_capacity, stellar_ctx** out) {
+  return create_ctx(p, ninst, counts, node_capacity, out);
+}
+
+int32_t stellar_num_instances(const stellar_ctx* c) { return c ? c->ninst : -1; }
+
+int stellar_instance_offsets(const stellar_ctx* c, int32_t* offsets) {
+  if (!c || !offsets) return STELLAR_ERR_ARG;
+  std::memcpy(offsets, c->ioff.data(), c->ioff.size() * sizeof(int32_t));
   return STELLAR_OK;
 }
 
@@ -462,12 +549,21 @@ int stellar_sync(stellar_ctx* c) {
   return check_device(c);
 }
 
-int stellar_get_bbox(stellar_ctx* c, float* bbox4) {
-  if (!c || !bbox4) return STELLAR_ERR_ARG;
+int stellar_get_bbox_instance(stellar_ctx* c, int32_t inst, float* bbox4) {
+  if (!c || !bbox4 || inst < 0 || inst >= c->ninst) return STELLAR_ERR_ARG;
   int rc = check_device(c);
   if (rc) return rc;
-  std::memcpy(bbox4, c->b.host_dev->bbox, 4 * sizeof(float));
+  float4 b;
+  HIP_TRY(c, hipMemcpy(&b, c->b.ibbox + inst, sizeof(float4), hipMemcpyDeviceToHost));
+  bbox4[0] = b.x;
+  bbox4[1] = b.y;
+  bbox4[2] = b.z;
+  bbox4[3] = b.w;
   return STELLAR_OK;
+}
+
+int stellar_get_bbox(stellar_ctx* c, float* bbox4) {
+  return stellar_get_bbox_instance(c, 0, bbox4);
 }
 
 int stellar_num_cells(stellar_ctx* c, int64_t* ncells) {
@@ -492,82 +588,116 @@ int stellar_stage_times(stellar_ctx* c, float* ms7) {
   return STELLAR_OK;
 }
 
-int stellar_export_tree(stellar_ctx* c, int32_t* root4, int32_t* child,
-                        int32_t* count, int32_t* start, int32_t* level,
-                        int32_t* sorted, float* com_x, float* com_y,
-                        float* com_m, int64_t capacity, int64_t* ncells) {
-  if (!c) return STELLAR_ERR_ARG;
+int stellar_export_tree_instance(stellar_ctx* c, int32_t inst, int32_t* root4,
+                                 int32_t* child, int32_t* count, int32_t* start,
+                                 int32_t* level, int32_t* sorted, float* com_x,
+                                 float* com_y, float* com_m, int64_t capacity,
+                                 int64_t* ncells) {
+  if (!c || inst < 0 || inst >= c->ninst) return STELLAR_ERR_ARG;
   if (c->stage < kBuildDone) return fail(c, STELLAR_ERR_STATE, "no tree built");
   int rc = check_device(c);
   if (rc) return rc;
-  const int32_t n = c->n;
-  const int64_t C = c->b.host_dev->ncells;
+  Buffers& B = c->b;
+  const int32_t lo = c->ioff[(size_t)inst], hi = c->ioff[(size_t)inst + 1];
+  const int32_t n = hi - lo;  // bodies of this instance
+  const int64_t ctot = c->b.host_dev->ncells;
+  // cells of the instance: those opened by its positions, ids [c0, c1)
+  int64_t c0 = 0, c1 = ctot;
+  if (n > 0) {
+    int32_t v = 0;
+    HIP_TRY(c, hipMemcpy(&v, B.coff + lo, 4, hipMemcpyDeviceToHost));
+    c0 = v;
+    if (inst + 1 < c->ninst) {
+      HIP_TRY(c, hipMemcpy(&v, B.coff + hi, 4, hipMemcpyDeviceToHost));
+      c1 = v;
+    }
+  }
+  if (c0 > ctot) c0 = ctot;
+  if (c1 > ctot) c1 = ctot;
+  const int64_t C = c1 - c0;
   if (ncells) *ncells = C;
   if (capacity < C && (child || count || start || level || com_x || com_y || com_m))
     return fail(c, STELLAR_ERR_ARG, "export capacity below cell count");
-  Buffers& B = c->b;
-  std::vector<int32_t> id((size_t)(n > 0 ? n : 1)), perm((size_t)(n > 0 ? n : 1));
-  std::vector<int32_t> srt((size_t)(n > 0 ? n : 1));
+  const int32_t nr = c->ninst;  // block of cell q of this instance: nr + c0 + q
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  std::vector<int32_t> id(nn), perm(nn), srt(nn);
   if (n > 0) {
-    HIP_TRY(c, hipMemcpy(id.data(), B.id, (size_t)n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(id.data(), B.id + lo, (size_t)n * 4, hipMemcpyDeviceToHost));
     if (c->sorted_storage) {
-      for (int32_t s = 0; s < n; ++s) srt[(size_t)s] = id[(size_t)s];
+      for (int32_t s = 0; s < n; ++s) srt[(size_t)s] = id[(size_t)s] - lo;
     } else {
-      HIP_TRY(c, hipMemcpy(perm.data(), B.perm, (size_t)n * 4, hipMemcpyDeviceToHost));
-      for (int32_t s = 0; s < n; ++s) srt[(size_t)s] = id[(size_t)perm[(size_t)s]];
+      std::vector<int32_t> idall((size_t)c->n);
+      HIP_TRY(c, hipMemcpy(idall.data(), B.id, (size_t)c->n * 4, hipMemcpyDeviceToHost));
+      HIP_TRY(c, hipMemcpy(perm.data(), B.perm + lo, (size_t)n * 4, hipMemcpyDeviceToHost));
+      for (int32_t s = 0; s < n; ++s) srt[(size_t)s] = idall[(size_t)perm[(size_t)s]] - lo;
     }
   }
   if (sorted && n > 0) std::memcpy(sorted, srt.data(), (size_t)n * 4);
-  std::vector<ChildRec> blk((size_t)(4 * (C + 1)));
-  HIP_TRY(c, hipMemcpy(blk.data(), B.blk, blk.size() * sizeof(ChildRec),
+  std::vector<ChildRec> rootb(4), blk((size_t)(4 * (C > 0 ? C : 1)));
+  HIP_TRY(c, hipMemcpy(rootb.data(), B.blk + 4 * (int64_t)inst, 4 * sizeof(ChildRec),
                        hipMemcpyDeviceToHost));
+  if (C > 0)
+    HIP_TRY(c, hipMemcpy(blk.data(), B.blk + 4 * (nr + c0), (size_t)(4 * C) * sizeof(ChildRec),
+                         hipMemcpyDeviceToHost));
   auto conv = [&](int32_t code) -> int32_t {
     if (code == 0) return -1;
-    if (code < 0) return srt[(size_t)(-1 - code)];
-    return n + (code - 1);
+    if (code < 0) {
+      const int64_t s = -1 - (int64_t)code - lo;
+      return s >= 0 && s < n ? srt[(size_t)s] : -2;
+    }
+    const int64_t q = (int64_t)code - nr - c0;
+    return q >= 0 && q < C ? (int32_t)(n + q) : -2;
   };
   if (root4)
-    for (int k = 0; k < 4; ++k) root4[k] = conv(blk[(size_t)k].code);
+    for (int k = 0; k < 4; ++k) root4[k] = conv(rootb[(size_t)k].code);
   if (child)
     for (int64_t q = 0; q < C; ++q)
-      for (int k = 0; k < 4; ++k)
-        child[4 * q + k] = conv(blk[(size_t)(4 * (q + 1) + k)].code);
-  if (C > 0 && (count || start || level)) {
-    std::vector<int32_t> tmp((size_t)(C + 1));
-    std::vector<uint8_t> lv((size_t)(C + 1));
-    if (count) {
-      // bodies below each cell, summed up the tree (children have larger ids)
-      for (int64_t q = C - 1; q >= 0; --q) {
-        int32_t t = 0;
-        for (int k = 0; k < 4; ++k) {
-          const int32_t code = blk[(size_t)(4 * (q + 1) + k)].code;
-          if (code < 0) t += 1;
-          else if (code > 0 && code - 1 > q && code - 1 < C) t += count[code - 1];
-        }
-        count[q] = t;
+      for (int k = 0; k < 4; ++k) child[4 * q + k] = conv(blk[(size_t)(4 * q + k)].code);
+  if (C > 0 && count) {
+    // bodies below each cell, summed up the tree (children have larger ids)
+    for (int64_t q = C - 1; q >= 0; --q) {
+      int32_t t = 0;
+      for (int k = 0; k < 4; ++k) {
+        const int32_t code = blk[(size_t)(4 * q + k)].code;
+        const int64_t cq = (int64_t)code - nr - c0;
+        if (code < 0) t += 1;
+        else if (code > 0 && cq > q && cq < C) t += count[cq];
       }
+      count[q] = t;
     }
+  }
+  if (C > 0 && (start || level)) {
     if (start) {
-      HIP_TRY(c, hipMemcpy(tmp.data(), B.cstart, (size_t)(C + 1) * 4, hipMemcpyDeviceToHost));
-      std::memcpy(start, tmp.data() + 1, (size_t)C * 4);
+      std::vector<int32_t> tmp((size_t)C);
+      HIP_TRY(c, hipMemcpy(tmp.data(), B.cstart + nr + c0, (size_t)C * 4, hipMemcpyDeviceToHost));
+      for (int64_t q = 0; q < C; ++q) start[q] = tmp[(size_t)q] - lo;
     }
     if (level) {
-      HIP_TRY(c, hipMemcpy(lv.data(), B.clevel, (size_t)(C + 1), hipMemcpyDeviceToHost));
-      for (int64_t q = 0; q < C; ++q) level[q] = lv[(size_t)(q + 1)];
+      std::vector<uint8_t> lv((size_t)C);
+      HIP_TRY(c, hipMemcpy(lv.data(), B.clevel + nr + c0, (size_t)C, hipMemcpyDeviceToHost));
+      for (int64_t q = 0; q < C; ++q) level[q] = lv[(size_t)q];
     }
   }
   if (C > 0 && (com_x || com_y || com_m)) {
     if (c->stage < kComDone) return fail(c, STELLAR_ERR_STATE, "no centre of mass yet");
-    std::vector<float4> com((size_t)(C + 1));
-    HIP_TRY(c, hipMemcpy(com.data(), B.ccom, (size_t)(C + 1) * sizeof(float4),
+    std::vector<float4> com((size_t)C);
+    HIP_TRY(c, hipMemcpy(com.data(), B.ccom + nr + c0, (size_t)C * sizeof(float4),
                          hipMemcpyDeviceToHost));
     for (int64_t q = 0; q < C; ++q) {
-      if (com_x) com_x[q] = com[(size_t)(q + 1)].x;
-      if (com_y) com_y[q] = com[(size_t)(q + 1)].y;
-      if (com_m) com_m[q] = com[(size_t)(q + 1)].z;
+      if (com_x) com_x[q] = com[(size_t)q].x;
+      if (com_y) com_y[q] = com[(size_t)q].y;
+      if (com_m) com_m[q] = com[(size_t)q].z;
     }
   }
   return STELLAR_OK;
+}
+
+int stellar_export_tree(stellar_ctx* c, int32_t* root4, int32_t* child,
+                        int32_t* count, int32_t* start, int32_t* level,
+                        int32_t* sorted, float* com_x, float* com_y,
+                        float* com_m, int64_t capacity, int64_t* ncells) {
+  return stellar_export_tree_instance(c, 0, root4, child, count, start, level, sorted,
+                                      com_x, com_y, com_m, capacity, ncells);
 }
 
 int stellar_export_forces(stellar_ctx* c, float* ax, float* ay,
@@ -638,11 +768,13 @@ int stellar_direct_fp64(stellar_ctx* c, int32_t ntargets, const int32_t* targets
   return STELLAR_OK;
 }
 
-int stellar_energy_fp64(stellar_ctx* c, double eps2, double* kinetic,
-                        double* potential, double* px, double* py) {
+int stellar_energy_fp64_instances(stellar_ctx* c, double eps2, double* kinetic,
+                                  double* potential, double* px, double* py) {
   if (!c) return STELLAR_ERR_ARG;
   if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
-  double k = 0, u = 0, mx = 0, my = 0;
+  const int32_t ni = c->ninst;
+  std::vector<double> k((size_t)ni, 0.0), u((size_t)ni, 0.0), mx((size_t)ni, 0.0),
+      my((size_t)ni, 0.0);
   if (c->n > 0) {
     const int parts = energy_parts(c);
     double* dpart = nullptr;
@@ -655,17 +787,43 @@ int stellar_energy_fp64(stellar_ctx* c, double eps2, double* kinetic,
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(dpart);
     if (e != hipSuccess) return hip_fail(c, e, "stellar_energy_fp64");
-    for (int q = 0; q < parts; ++q) {
-      u += h[(size_t)(4 * q + 0)];
-      k += h[(size_t)(4 * q + 1)];
-      mx += h[(size_t)(4 * q + 2)];
-      my += h[(size_t)(4 * q + 3)];
-    }
+    // fixed summation order: tiles of an instance in position order
+    for (int32_t i = 0; i < ni; ++i)
+      for (int64_t q = (int64_t)c->itile[(size_t)i] * kEnergySlices;
+           q < (int64_t)c->itile[(size_t)i + 1] * kEnergySlices; ++q) {
+        u[(size_t)i] += h[(size_t)(4 * q + 0)];
+        k[(size_t)i] += h[(size_t)(4 * q + 1)];
+        mx[(size_t)i] += h[(size_t)(4 * q + 2)];
+        my[(size_t)i] += h[(size_t)(4 * q + 3)];
+      }
   }
-  if (kinetic) *kinetic = k;
-  if (potential) *potential = u;
-  if (px) *px = mx;
-  if (py) *py = my;
+  for (int32_t i = 0; i < ni; ++i) {
+    if (kinetic) kinetic[i] = k[(size_t)i];
+    if (potential) potential[i] = u[(size_t)i];
+    if (px) px[i] = mx[(size_t)i];
+    if (py) py[i] = my[(size_t)i];
+  }
+  return STELLAR_OK;
+}
+
+int stellar_energy_fp64(stellar_ctx* c, double eps2, double* kinetic,
+                        double* potential, double* px, double* py) {
+  if (!c) return STELLAR_ERR_ARG;
+  std::vector<double> k((size_t)c->ninst), u((size_t)c->ninst), mx((size_t)c->ninst),
+      my((size_t)c->ninst);
+  const int rc = stellar_energy_fp64_instances(c, eps2, k.data(), u.data(), mx.data(), my.data());
+  if (rc) return rc;
+  double tk = 0, tu = 0, tx = 0, ty = 0;
+  for (int32_t i = 0; i < c->ninst; ++i) {
+    tk += k[(size_t)i];
+    tu += u[(size_t)i];
+    tx += mx[(size_t)i];
+    ty += my[(size_t)i];
+  }
+  if (kinetic) *kinetic = tk;
+  if (potential) *potential = tu;
+  if (px) *px = tx;
+  if (py) *py = ty;
   return STELLAR_OK;
 }
 

@@ -44,6 +44,7 @@ struct ComArgs {
   float4* ccom;
   DevScalars* dev;
   int quad;
+  int32_t nroot;  // root blocks (instances); cell c is block nroot + c
 };
 
 template <bool EXT>
@@ -189,7 +190,7 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
   }
 }
 
-// One chunk: list positions [p0, p1) of `list` (nullptr: block id = pos + 1),
+// One chunk: list positions [p0, p1) of `list` (nullptr: block id = pos + nroot),
 // `next` = block id at p1 (INT32_MAX if none or if every later cell is
 // already finished).  Leftover cells go to out_ids[0..*out_n).
 //
@@ -211,7 +212,7 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
   {
     int lev = 255;
     if (i < ne) {
-      const int32_t b = list ? list[p0 + i] : p0 + i + 1;
+      const int32_t b = list ? list[p0 + i] : p0 + i + A.nroot;
       L.ids[i] = b;
       lev = A.clevel[b];
       L.pblk[i] = A.cparent[b];
@@ -308,7 +309,7 @@ __global__ __launch_bounds__(kComThreads) void com_pass_kernel(
   const int32_t p0 = blockIdx.x * kChunk;
   if (p0 >= m) return;
   const int32_t p1 = p0 + kChunk < m ? p0 + kChunk : m;
-  const int32_t next = p1 < m ? (list ? list[p1] : p1 + 1) : INT32_MAX;
+  const int32_t next = p1 < m ? (list ? list[p1] : p1 + A.nroot) : INT32_MAX;
   process_chunk<EXT>(A, L, list, p0, p1, next, out_slots + (int64_t)blockIdx.x * kSlots,
                 out_counts + blockIdx.x);
 }
@@ -387,7 +388,7 @@ void launch_com(stellar_ctx* c) {
   if (c->n == 0 || c->cap == 0) return;
   Buffers& B = c->b;
   ComArgs A{B.blk,  B.qblk, B.cparent, B.cslot, B.clevel, B.csum, B.ccom, B.dev,
-            (c->p.flags & STELLAR_FLAG_QUADRUPOLE) ? 1 : 0};
+            (c->p.flags & STELLAR_FLAG_QUADRUPOLE) ? 1 : 0, c->ninst};
   // pass 1 over all cells; each later list holds at most kSlots cells per
   // chunk of the pass before, so grids are sized from that bound and chunks
   // beyond the actual length exit at once

@@ -45,12 +45,18 @@ struct alignas(16) ChildRec {
 struct DevScalars {
   uint32_t err;          // sticky error bits
   uint32_t pad0;
-  int32_t ncells;        // cells of the current tree
+  int32_t ncells;        // cells of the current tree (all instances)
   int32_t pad1;
-  float bbox[4];         // left, right, bottom, top
+  float bbox[4];         // unused (per-instance boxes live in Buffers::ibbox)
   unsigned long long interactions;
   uint32_t pad[46];
 };
+
+// Positions per bounding-box tile (one workgroup; tiles never straddle two
+// instances).
+constexpr int kBboxTile = 4096;
+// FP64 energy: slices of 256 targets per bounding-box tile.
+constexpr int kEnergySlices = kBboxTile / 256;
 
 struct Buffers {
   // body state in tree order of the last sort (double-buffered for the sort)
@@ -65,7 +71,8 @@ struct Buffers {
   // build
   uint64_t *key = nullptr, *key_alt = nullptr;  // level 1..32 codes
   uint32_t *perm = nullptr, *perm_alt = nullptr;
-  uint64_t* key2 = nullptr;   // level 33..64 codes (valid inside tie runs)
+  uint64_t* key2 = nullptr;   // 32 deeper levels (valid inside tie runs)
+  uint32_t* key3 = nullptr;   // the deepest levels when the key holds instances
   int32_t* ncell = nullptr;   // cells opened by each adjacent pair
   int32_t* coff = nullptr;    // exclusive scan of ncell
   uint8_t* lam = nullptr;     // common-prefix depth of each adjacent pair
@@ -85,7 +92,16 @@ struct Buffers {
   int32_t* clist = nullptr;   // two compacted CoM lists
   int32_t* clen = nullptr;    // their lengths
   float* out = nullptr;       // 2*(n+cap) interleaved output buffer
-  float4* bbox_part = nullptr;  // per-block bounding-box partials
+  float4* bbox_part = nullptr;  // per-tile bounding-box partials
+  // instances: a context advances ninst independent simulations whose
+  // bodies occupy consecutive position ranges [ioff[i], ioff[i+1]); block i
+  // is the root of instance i and block ninst + c is cell c.
+  int32_t* ioff = nullptr;      // ninst + 1 offsets
+  int32_t* inst_of = nullptr;   // instance of each position (and body id)
+  float4* ibbox = nullptr;      // per instance: left, right, bottom, top
+  int32_t* tstart = nullptr;    // bbox tiles: ntiles + 1 starts
+  int32_t* itile = nullptr;     // first tile of each instance, ninst + 1
+  int32_t* gtab[3] = {};        // walk groups of 64, 32, 16: ngroups + 1 starts
   DevScalars* dev = nullptr;
   DevScalars* host_dev = nullptr;  // pinned mirror
 };
@@ -106,6 +122,11 @@ struct stellar_ctx {
   stellar_params p;
   int32_t n = 0;
   int64_t cap = 0;
+  int32_t ninst = 1;
+  std::vector<int32_t> ioff;    // host copy of the instance offsets
+  std::vector<int32_t> itile;   // host copy of each instance's first tile
+  int32_t ntiles = 0;
+  int64_t ngroups[3] = {};      // walk groups of 64, 32, 16
   int device = 0;
   hipStream_t stream = nullptr;
   stellar::Buffers b;

@@ -36,6 +36,8 @@ namespace stellar {
 
 namespace {
 
+// Walk groups are G consecutive tree positions of one instance: group g is
+// [gtab[g], gtab[g+1]) (64, 32 or 16 positions, fewer at an instance's end).
 struct WalkArgs {
   const float* x;
   const float* y;
@@ -48,6 +50,10 @@ struct WalkArgs {
   float* ax;
   float* ay;
   int32_t* inter;
+  const int32_t* gtab;
+  int64_t ngroups;
+  const int32_t* inst_of;
+  const float4* ibbox;
 };
 
 // r = dx*dx + dy*dy + eps2 as dx*dx + (dy*dy + eps2), two fused
@@ -56,9 +62,11 @@ __device__ __forceinline__ float dist2(float dx, float dy, float eps2) {
   return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, eps2));
 }
 
-// depth = radius^2/theta with radius = half the x extent (kernels.cu:349,376)
-__device__ __forceinline__ float root_depth(const WalkArgs& A) {
-  const float bl = A.dev->bbox[0], br = A.dev->bbox[1];
+// depth = radius^2/theta with radius = half the x extent of the instance's
+// box (kernels.cu:349,376)
+__device__ __forceinline__ float root_depth(const WalkArgs& A, int32_t inst) {
+  const float4 bb = A.ibbox[inst];
+  const float bl = bb.x, br = bb.y;
   const float radius = (float)(0.5 * (double)(br - bl));
   return (float)((double)(radius * radius) / (double)A.theta_div);
 }
@@ -108,23 +116,25 @@ __global__ __launch_bounds__(kWalkBlock) void walk64_kernel(
   constexpr int kWaves = kWalkBlock / 64;
   __shared__ uint32_t stk[kWaves][kStack64];
 
-  const int32_t n = A.n;
   const float eps2 = A.eps2;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int64_t s0 = ((int64_t)blockIdx.x * kWaves + wave) * 64;
-  if (s0 >= n) return;
+  const int64_t g = (int64_t)blockIdx.x * kWaves + wave;
+  if (g >= A.ngroups) return;
   if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
-  const int64_t s = s0 + lane;
-  const bool valid = s < n;
-  const int64_t sl = valid ? s : n - 1;  // padding lanes repeat the last body
+  const int32_t s0 = __builtin_amdgcn_readfirstlane(A.gtab[g]);
+  const int32_t cnt = __builtin_amdgcn_readfirstlane(A.gtab[g + 1]) - s0;
+  const int32_t inst = __builtin_amdgcn_readfirstlane(A.inst_of[s0]);
+  const int32_t s = s0 + lane;
+  const bool valid = lane < cnt;
+  const int32_t sl = valid ? s : s0 + cnt - 1;  // padding lanes repeat the last body
   const float px = A.x[sl];
   const float py = A.y[sl];
 
   // thr(level) = threshold for the children of a node at `level`:
   // 0.25*depth(level), depth(1) = depth0, depth(l+1) = 0.25*depth(l)
   // (kernels.cu:394-395), one level per lane.
-  const float depth0 = root_depth(A);
+  const float depth0 = root_depth(A, inst);
   float thr = depth0;
   for (int i = 0; i <= lane; ++i) thr = 0.25f * thr;
 
@@ -132,9 +142,10 @@ __global__ __launch_bounds__(kWalkBlock) void walk64_kernel(
   int sp = 0;
   Acc acc;
 
+  const ChildRec* root = blk + 4 * (int64_t)inst;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const ChildRec c = blk[k];
+    const ChildRec c = root[k];
     if (c.code == 0) continue;
     if (c.code < 0) {
       const float dx = c.x - px, dy = c.y - py;
@@ -210,11 +221,8 @@ __global__ __launch_bounds__(kWalkBlock) void walk64_kernel(
     A.ay[s] = acc.fy;
     A.inter[s] = acc.cnt;
   }
-  if (lane == 0) {
-    const int64_t members = (n - s0) < 64 ? (n - s0) : 64;
-    atomicAdd(&A.dev->interactions,
-              (unsigned long long)acc.cnt * (unsigned long long)members);
-  }
+  if (lane == 0)
+    atomicAdd(&A.dev->interactions, (unsigned long long)acc.cnt * (unsigned long long)cnt);
 }
 
 // ----------------------------------------------------------- G = 32, 16 --
@@ -227,21 +235,25 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
   __shared__ int32_t stk_node[kWaves * kGroupsPerWave][kWalkStack];
   __shared__ float stk_dep[kWaves * kGroupsPerWave][kWalkStack];
 
-  const int32_t n = A.n;
   const float eps2 = A.eps2;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int sub = lane / G;             // group within the wavefront
   const int glane = lane & (G - 1);     // lane within the group
   const int slot = wave * kGroupsPerWave + sub;
-  const int64_t s0 = ((int64_t)blockIdx.x * kWaves + wave) * 64 + sub * G;
-  if (((int64_t)blockIdx.x * kWaves + wave) * 64 >= n) return;
+  const int64_t g0 = ((int64_t)blockIdx.x * kWaves + wave) * kGroupsPerWave;
+  if (g0 >= A.ngroups) return;  // the whole wavefront
   if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
-  const int64_t s = s0 + glane;
-  const bool valid = s < n;
+  const int64_t g = g0 + sub;
+  const bool gvalid = g < A.ngroups;
+  const int32_t s0 = gvalid ? A.gtab[g] : 0;
+  const int32_t cnt = gvalid ? A.gtab[g + 1] - s0 : 0;
+  const int32_t s = s0 + glane;
+  const bool valid = glane < cnt;
   const float px = valid ? A.x[s] : 0.f;
   const float py = valid ? A.y[s] : 0.f;
-  const float depth0 = root_depth(A);
+  const int32_t inst = gvalid ? A.inst_of[s0] : 0;
+  const float depth0 = root_depth(A, inst);
   const uint64_t gmask = ((G == 64) ? ~0ull : ((1ull << G) - 1ull)) << (sub * G);
 
   Acc acc;
@@ -249,9 +261,11 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
   bool overflow = false;
 
   // root block: level-1 cells are always opened (kernels.cu:370-388)
+  const ChildRec* root = blk + 4 * (int64_t)inst;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const ChildRec c = blk[k];
+    if (!gvalid) break;
+    const ChildRec c = root[k];
     if (c.code == 0) continue;
     if (c.code < 0) {
       const float dx = c.x - px, dy = c.y - py;
@@ -316,11 +330,8 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
     A.ay[s] = acc.fy;
     A.inter[s] = acc.cnt;
   }
-  if (glane == 0) {
-    const int64_t members = (n - s0) < G ? (n - s0) : G;
-    atomicAdd(&A.dev->interactions,
-              (unsigned long long)acc.cnt * (unsigned long long)members);
-  }
+  if (glane == 0 && gvalid)
+    atomicAdd(&A.dev->interactions, (unsigned long long)acc.cnt * (unsigned long long)cnt);
 }
 
 template <int G>
@@ -345,11 +356,16 @@ void launch_forces(stellar_ctx* c, bool zero_counter) {
   const int32_t n = c->n;
   if (n == 0) return;
   Buffers& B = c->b;
-  const int64_t waves = (n + 63) / 64;
+  const int gi = c->p.group == 32 ? 1 : c->p.group == 16 ? 2 : 0;
+  const int64_t ngroups = c->ngroups[gi];
+  const int64_t per_wave = 64 / (c->p.group == 32 ? 32 : c->p.group == 16 ? 16 : 64);
+  const int64_t waves = (ngroups + per_wave - 1) / per_wave;
   const unsigned grid = (unsigned)((waves + (kWalkBlock / 64) - 1) / (kWalkBlock / 64));
   const float theta = c->p.theta;
   const float tdiv = (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? theta * theta : theta;
-  WalkArgs A{B.x, B.y, B.blk, B.qblk, n, c->p.eps2, tdiv, B.dev, B.ax, B.ay, B.inter};
+  WalkArgs A{B.x,       B.y,       B.blk,       B.qblk,   n,
+             c->p.eps2, tdiv,      B.dev,       B.ax,     B.ay,
+             B.inter,   B.gtab[gi], ngroups,    B.inst_of, B.ibbox};
   if (zero_counter) launch_zero_counter(c);
   switch (c->p.group) {
     case 32:

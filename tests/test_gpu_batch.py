@@ -1,0 +1,156 @@
+"""GPU ensembles: many independent simulations advanced by one context.
+
+Each instance of an ensemble must evolve exactly as it would alone: its
+tree equals the oracle's tree of its own bodies (bit-exact), and its forces,
+interaction counts and updated state equal those of a single-instance
+context on the same bodies bit for bit (same tree, same walk groups, same
+traversal).  Instances of 1 and 2 bodies and sizes that are not multiples of
+the walk group are included.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import first_tree_difference, ic, tree_equal
+
+pytestmark = pytest.mark.gpu
+
+COUNTS = [1000, 64, 5000, 1, 2, 333, 4096, 777, 65, 20000]
+EPS2 = float(np.float32(1e-4))
+
+
+@pytest.fixture(scope="module")
+def S():
+    from tests.conftest import gpu_available
+
+    if not gpu_available():
+        pytest.skip("no GPU")
+    import stellarsolver_amd as S
+
+    S.load_library()
+    return S
+
+
+def members(S, counts, base_seed=7):
+    """Per-instance initial conditions: models cycle disk, Plummer, colliding
+    disks; seed base_seed + i."""
+    out = []
+    for i, n in enumerate(counts):
+        p = S.ic_params([0, 1, 2][i % 3], seed=base_seed + i)
+        out.append(S.generate_ic(p, n))
+    return out
+
+
+def concat(sts):
+    return {k: np.concatenate([s[k] for s in sts]) for k in ("mass", "x", "y", "vx", "vy")}
+
+
+def batch_engine(S, sts, **kw):
+    p = S.default_params(eps2=EPS2, **kw)
+    e = S.Engine(params=p, counts=[s["x"].size for s in sts])
+    st = concat(sts)
+    e.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
+    return e
+
+
+def single_engine(S, st, **kw):
+    p = S.default_params(eps2=EPS2, **kw)
+    e = S.Engine(st["x"].size, p)
+    e.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
+    return e
+
+
+def test_batch_layout(S):
+    sts = members(S, COUNTS)
+    with batch_engine(S, sts) as e:
+        assert e.ninst == len(COUNTS)
+        assert e.n == sum(COUNTS)
+        off = np.zeros(len(COUNTS) + 1, np.int32)
+        assert S.load_library().stellar_instance_offsets(e._h, S._ip(off)) == 0
+        assert np.array_equal(off, np.concatenate([[0], np.cumsum(COUNTS)]))
+
+
+def test_batch_trees_bit_exact(S, oracle):
+    sts = members(S, COUNTS)
+    with batch_engine(S, sts) as e:
+        e.bbox()
+        e.build()
+        e.com()
+        for i, st in enumerate(sts):
+            print(f"instance {i} n={st['x'].size}", flush=True)
+            bb = oracle.bbox(st["x"], st["y"])
+            assert np.array_equal(e.get_bbox(i), np.asarray(bb, np.float32))
+            kt = oracle.build_keys(st["x"], st["y"], bb)
+            oracle.com_exact(kt, st["x"], st["y"], st["mass"])
+            t = e.export_tree(instance=i)
+            assert tree_equal(t, kt), first_tree_difference(t, kt)
+            for k in ("com_x", "com_y", "com_m"):
+                assert np.array_equal(t[k], kt[k]), k
+
+
+@pytest.mark.parametrize("group", [64, 32, 16])
+def test_batch_steps_match_single_contexts(S, group):
+    sts = members(S, COUNTS, base_seed=11)
+    steps = 3
+    with batch_engine(S, sts, group=group) as e:
+        e.run(steps)
+        got = e.download()
+        e.bbox()
+        e.build()
+        e.com()
+        e.sort()
+        e.forces()
+        fb = e.export_forces()
+        total = e.last_interactions()
+    off = np.concatenate([[0], np.cumsum(COUNTS)])
+    tot = 0
+    for i, st in enumerate(sts):
+        print(f"instance {i} n={st['x'].size}", flush=True)
+        with single_engine(S, st, group=group) as s:
+            s.run(steps)
+            want = s.download()
+            s.bbox()
+            s.build()
+            s.com()
+            s.sort()
+            s.forces()
+            fs = s.export_forces()
+            tot += s.last_interactions()
+        sl = slice(off[i], off[i + 1])
+        for k in want:
+            assert np.array_equal(got[k][sl], want[k]), (i, k)
+        for k in fs:
+            assert np.array_equal(fb[k][sl], fs[k]), (i, k)
+    assert total == tot
+
+
+def test_batch_fp64_diagnostics(S):
+    sts = members(S, COUNTS[:6], base_seed=3)
+    with batch_engine(S, sts) as e:
+        e.run(2)
+        ens = e.energy_fp64_instances()
+        tot = e.energy_fp64()
+        ids = np.array([0, 1000, 1063, 1064, 6064, 6065, 6066], np.int32)
+        bx, by = e.direct_fp64(ids)
+        pos = e.download()
+    off = np.concatenate([[0], np.cumsum(COUNTS[:6])])
+    for i, st in enumerate(sts):
+        with single_engine(S, st) as s:
+            s.run(2)
+            en = s.energy_fp64()
+            for k in en:
+                assert ens[k][i] == pytest.approx(en[k], rel=1e-12, abs=1e-12), (i, k)
+            loc = ids[(ids >= off[i]) & (ids < off[i + 1])]
+            if loc.size:
+                sx, sy = s.direct_fp64(loc - off[i])
+                sel = np.isin(ids, loc)
+                np.testing.assert_allclose(bx[sel], sx, rtol=1e-12, atol=0)
+                np.testing.assert_allclose(by[sel], sy, rtol=1e-12, atol=0)
+    for k in ("kinetic", "potential", "px", "py"):
+        assert tot[k] == pytest.approx(float(np.sum(ens[k])), rel=1e-12, abs=1e-12)
+    assert np.all(np.isfinite(pos["x"]))
+
+
+def test_batch_rejects_empty_instance(S):
+    p = S.default_params()
+    with pytest.raises(S.StellarError):
+        S.Engine(params=p, counts=[10, 0, 5])
