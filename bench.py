@@ -55,6 +55,14 @@ def parse():
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3, help="timed oracle steps after 1 warm-up (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--ensemble", action="store_true",
+                    help="batched ensemble workload: 256 instances, N_i log-uniform in "
+                         "[1024, 65536] (multiples of 64), models cycling disk / Plummer / "
+                         "colliding disks, theta 0.5, eps 0.01, dt 1e-3, graph-captured steps, "
+                         "autotuned walk shape; compared with the same instances run one "
+                         "context at a time")
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--graph", action="store_true", help="replay captured two-step graphs")
     return ap.parse_args()
 
 
@@ -92,6 +100,116 @@ def cpu_baseline(st, args, eps2):
     }
 
 
+def ensemble_members(S, ninst, seed=3):
+    """N_i log-uniform in [1024, 65536] rounded to multiples of 64; models
+    cycle disk (workload 0), Plummer (1), colliding disks (2); instance i uses
+    seed 1000*seed + i."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(1024), np.log(65536), ninst))
+    counts = (np.round(counts / 64) * 64).astype(np.int64).clip(1024, 65536)
+    sts = []
+    for i, n in enumerate(counts):
+        p = S.ic_params(i % 3, seed=1000 * seed + i)
+        sts.append(S.generate_ic(p, int(n)))
+    return [int(c) for c in counts], sts
+
+
+def run_ensemble(args, S):
+    eps2 = float(np.float32(args.eps * args.eps))
+    prec = S.PRECISION_MIXED if args.precision == "mixed" else S.PRECISION_FP32
+    counts, sts = ensemble_members(S, args.instances)
+    ntot = int(sum(counts))
+    cat = {k: np.concatenate([s[k] for s in sts]) for k in ("mass", "x", "y", "vx", "vy")}
+    base_flags = (S.FLAG_QUADRUPOLE if args.quad else 0) | (S.FLAG_TEXTBOOK_OPENING if args.textbook else 0)
+
+    # batched form: one context, graph-captured steps, autotuned walk shape
+    p = S.default_params(theta=args.theta, eps2=eps2, timestep=args.dt, group=args.group,
+                         precision=prec, flags=base_flags | S.FLAG_STAGE_TIMING)
+    eng = S.Engine(params=p, counts=counts)
+    eng.upload(cat["mass"], cat["x"], cat["y"], cat["vx"], cat["vy"])
+    tuning = eng.autotune(group_mask=0, reps=2)
+    eng.run(2)  # eager step timing for the stage split
+    stages = eng.stage_times()
+    inter = eng.last_interactions()
+    eng.set_params(flags=base_flags | S.FLAG_GRAPH)
+    eng.run(args.warmup + (args.warmup % 2))
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.run(args.steps)
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    batched = ntot * args.steps / wall
+    # accuracy on a sample of instances
+    eng.bbox(); eng.build(); eng.com(); eng.sort(); eng.forces()
+    f = eng.export_forces()
+    rng = np.random.default_rng(4)
+    off = np.concatenate([[0], np.cumsum(counts)])
+    tg = []
+    for i in rng.choice(len(counts), min(16, len(counts)), replace=False):
+        tg.append(off[i] + rng.choice(counts[i], min(512, counts[i]), replace=False))
+    tg = np.sort(np.concatenate(tg)).astype(np.int32)
+    dx, dy = eng.direct_fp64(tg)
+    err_rms, err_p99, err_max = rms_rel(f["ax"][tg], f["ay"][tg], dx, dy)
+    eng.close()
+
+    # plain form: the same instances, one context each, advanced one after another
+    plain_steps = max(2, min(args.steps, 10))
+    engines = []
+    for st in sts:
+        q = S.default_params(theta=args.theta, eps2=eps2, timestep=args.dt, group=args.group,
+                             precision=prec, flags=base_flags)
+        e = S.Engine(st["x"].size, q)
+        e.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
+        engines.append(e)
+    for e in engines:
+        e.run(1)
+    t0 = time.perf_counter()
+    for e in engines:
+        e.run(plain_steps)
+    t1 = time.perf_counter()
+    plain = ntot * plain_steps / (t1 - t0)
+    for e in engines:
+        e.close()
+
+    walk_ms = stages["forces"]
+    achieved = inter * OPS_PER_INTERACTION / (walk_ms * 1e-3) / 1e12
+    peak = VALU_PEAK_LANE_OPS / 1e12
+    out = {
+        "metric": "BH body-steps/s, batched ensemble (config 3 workload)",
+        "value": batched,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"ensemble of {len(counts)} instances, N_i log-uniform "
+                               "[1024, 65536] (x64), disk/Plummer/colliding disks",
+                   "n_bodies": ntot, "theta": args.theta, "eps": args.eps, "dt": args.dt,
+                   "precision": args.precision, "graph": True, "tuning": tuning,
+                   "multipoles": "monopole+quadrupole" if args.quad else "monopole"},
+        "plain_form_body_steps_per_s": plain,
+        "plain_form": f"{len(counts)} single-instance contexts run one after another, "
+                      f"{plain_steps} steps each, no graph",
+        "speedup_over_plain_form": batched / plain,
+        "rms_rel_force_err": err_rms,
+        "force_err_p99": err_p99,
+        "force_err_max": err_max,
+        "force_err_sample": int(tg.size),
+        "stage_ms": stages,
+        "interactions_per_step": inter,
+        "roofline": {"bound": "valu", "kernel": "walk64_kernel", "achieved": achieved,
+                     "peak": peak, "unit": "Tlane-op/s", "frac": achieved / peak,
+                     "traffic": None},
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -102,6 +220,8 @@ def main():
     import stellarsolver_amd as S
 
     S.load_library()
+    if args.ensemble:
+        return run_ensemble(args, S)
     n = args.n
     eps2 = float(np.float32(args.eps * args.eps))
     st = S.generate_ic(S.ic_params(args.workload), n)
@@ -109,17 +229,25 @@ def main():
     p = S.default_params(theta=args.theta, eps2=eps2, timestep=args.dt, dampening=1.0,
                          group=args.group, precision=prec,
                          flags=S.FLAG_STAGE_TIMING | (S.FLAG_QUADRUPOLE if args.quad else 0)
-                         | (S.FLAG_TEXTBOOK_OPENING if args.textbook else 0))
+                         | (S.FLAG_TEXTBOOK_OPENING if args.textbook else 0)
+                         | (S.FLAG_GRAPH if args.graph else 0))
     eng = S.Engine(n, p)
     eng.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
     e0 = None if args.no_energy else eng.energy_fp64()
 
+    if args.graph:
+        # stage times come from direct steps (graph replays record none)
+        eng.set_params(flags=p.flags & ~S.FLAG_GRAPH)
+        eng.run(2)
+        stages = eng.stage_times()
+        eng.set_params(flags=p.flags | S.FLAG_GRAPH)
     eng.run(args.warmup)
     eng.sync()
     t0 = time.perf_counter()
     eng.run(args.steps)  # synchronises once at the end
     t1 = time.perf_counter()
-    stages = eng.stage_times()  # mean device ms per step, HIP events on the engine stream
+    if not args.graph:
+        stages = eng.stage_times()  # mean device ms per step, HIP events on the engine stream
     inter = eng.last_interactions()
     ncells = eng.num_cells()
     wall = t1 - t0
@@ -169,6 +297,7 @@ def main():
         "config": {"workload": "colliding disks" if args.workload == 2 else f"workload {args.workload}",
                    "n_bodies": n, "theta": args.theta, "eps": args.eps, "dt": args.dt,
                    "group": args.group, "precision": args.precision, "seed": 2,
+                   "graph": bool(args.graph),
                    "opening": ("textbook (w/d <= theta" if args.textbook else
                                "reference (w^2/theta <= d^2+eps^2") + ", group vote)",
                    "multipoles": "monopole+quadrupole" if args.quad else "monopole"},

@@ -78,6 +78,10 @@ typedef struct stellar_params {
 /* Opening test w^2/theta^2 <= d^2+eps2, i.e. the textbook w/d <= theta of
  * the reference's older, commented-out kernel (kernels.cu:488-489). */
 #define STELLAR_FLAG_TEXTBOOK_OPENING 4
+/* stellar_run replays a captured HIP graph of two whole steps (captured on
+ * first use for each arrangement of the context's double buffers; an odd
+ * last step runs directly).  Stage times are not recorded in graph runs. */
+#define STELLAR_FLAG_GRAPH 8
 
 typedef struct stellar_ctx stellar_ctx;
 
@@ -175,6 +179,25 @@ int stellar_last_interactions(stellar_ctx* ctx, int64_t* total);
  * STELLAR_FLAG_STAGE_TIMING is set: [bbox, build, com, sort, forces,
  * update, output]. */
 int stellar_stage_times(stellar_ctx* ctx, float* ms7);
+
+/* ---- launch-shape autotuner (no counterpart in the reference) ---------- */
+typedef struct stellar_tuning {
+  int32_t group;         /* walk-group size: 64, 32 or 16                   */
+  int32_t nodes_per_pop; /* group 64: tree nodes fetched per round trip      */
+  int32_t block_threads; /* group 64: threads per walk workgroup             */
+  float walk_ms;         /* mean walk time of the choice when measured       */
+} stellar_tuning;
+/* Time the tree walk for every candidate launch shape on the current state
+ * (reps timed walks each, after one untimed) and keep the fastest for this
+ * context.  group_mask selects the walk-group sizes tried: 1 = 64, 2 = 32,
+ * 4 = 16; 0 means 64 only, which keeps the set of interactions unchanged
+ * (the shapes then differ only in speed and in summation order).  The
+ * choice is remembered per (log2 bodies per instance, ensemble size bucket,
+ * flags, group_mask) for the rest of the process, and a context in a known
+ * bucket takes it without measuring.  Leaves the state sorted (stage
+ * stellar_sort done).  out may be NULL. */
+int stellar_autotune(stellar_ctx* ctx, int32_t group_mask, int32_t reps,
+                     stellar_tuning* out);
 
 /* Message for the last non-OK status on this context. */
 const char* stellar_last_error(const stellar_ctx* ctx);

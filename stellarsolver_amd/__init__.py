@@ -36,6 +36,7 @@ PRECISION_MIXED = 1
 FLAG_STAGE_TIMING = 1
 FLAG_QUADRUPOLE = 2
 FLAG_TEXTBOOK_OPENING = 4
+FLAG_GRAPH = 8
 
 IC_DISK = 0
 IC_PLUMMER = 1
@@ -75,6 +76,15 @@ class StellarICParams(C.Structure):
         ("offset_x", C.c_double),
         ("bulk_vx", C.c_double),
         ("bulk_vy", C.c_double),
+    ]
+
+
+class StellarTuning(C.Structure):
+    _fields_ = [
+        ("group", C.c_int32),
+        ("nodes_per_pop", C.c_int32),
+        ("block_threads", C.c_int32),
+        ("walk_ms", C.c_float),
     ]
 
 
@@ -119,6 +129,7 @@ SIGNATURES = {
     "stellar_direct_fp64": (C.c_int, [_P, C.c_int32, _I, C.c_double, _D, _D]),
     "stellar_energy_fp64": (C.c_int, [_P, C.c_double, _D, _D, _D, _D]),
     "stellar_energy_fp64_instances": (C.c_int, [_P, C.c_double, _D, _D, _D, _D]),
+    "stellar_autotune": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(StellarTuning)]),
     "stellar_ic_defaults": (C.c_int, [C.c_int32, C.POINTER(StellarICParams)]),
     "stellar_generate_ic": (C.c_int, [C.POINTER(StellarICParams), C.c_int32, _F, _F, _F, _F, _F]),
 }
@@ -385,6 +396,13 @@ class Engine:
         self._check(self._lib.stellar_energy_fp64(self._h, e2, *(C.byref(q) for q in v)), "energy_fp64")
         return {"kinetic": v[0].value, "potential": v[1].value, "px": v[2].value, "py": v[3].value}
 
+    def autotune(self, group_mask: int = 0, reps: int = 3) -> dict:
+        t = StellarTuning()
+        self._check(self._lib.stellar_autotune(self._h, int(group_mask), int(reps), C.byref(t)), "autotune")
+        self.params.group = t.group
+        return {"group": t.group, "nodes_per_pop": t.nodes_per_pop,
+                "block_threads": t.block_threads, "walk_ms": t.walk_ms}
+
     def energy_fp64_instances(self, eps2: float | None = None) -> dict[str, np.ndarray]:
         e2 = float(self.params.eps2) if eps2 is None else float(eps2)
         v = {k: np.zeros(self.ninst, np.float64) for k in ("kinetic", "potential", "px", "py")}
@@ -394,6 +412,6 @@ class Engine:
 
 
 __all__ = [
-    "Engine", "StellarParams", "StellarICParams", "StellarError", "load_library",
+    "Engine", "StellarParams", "StellarICParams", "StellarTuning", "StellarError", "load_library",
     "default_params", "ic_params", "generate_ic", "LIB_PATH", "SIGNATURES", "STAGES",
 ]

@@ -30,6 +30,13 @@ namespace {
 
 constexpr int kBboxThreads = 256;
 
+// Sort configuration for captured steps: rocPRIM's merge-sort path at every
+// size (its onesweep path, chosen above 2^20 items, clears its counters
+// with memset nodes, which captured graphs must not contain).
+using MergeSortConfig = rocprim::radix_sort_config<rocprim::default_config,
+                                                   rocprim::default_config,
+                                                   rocprim::default_config, (size_t)1 << 40>;
+
 // min/max that never replace +0 by -0 and ignore NaN (the reference's
 // fminf/fmaxf over a box seeded with zeros, kernels.cu:51-58,62-114).
 __device__ __forceinline__ float mn(float a, float v) { return v < a ? v : a; }
@@ -533,9 +540,12 @@ size_t build_scratch_bytes(int32_t n) {
   int32_t* q = nullptr;
   rocprim::double_buffer<uint64_t> kb(k, k);
   rocprim::double_buffer<uint32_t> vb(v, v);
+  size_t a2 = 0;
   (void)rocprim::radix_sort_pairs(nullptr, a, kb, vb, (size_t)n, 0, 64);
+  (void)rocprim::radix_sort_pairs<MergeSortConfig>(nullptr, a2, kb, vb, (size_t)n, 0, 64);
   (void)rocprim::exclusive_scan(nullptr, b, q, q, 0, (size_t)n,
                                 rocprim::plus<int32_t>());
+  if (a2 > a) a = a2;
   return (a > b ? a : b) + 256;
 }
 
@@ -571,8 +581,11 @@ void launch_build(stellar_ctx* c) {
   rocprim::double_buffer<uint64_t> kb(B.key, B.key_alt);
   rocprim::double_buffer<uint32_t> vb(B.perm, B.perm_alt);
   size_t bytes = B.tmp_bytes;
-  (void)rocprim::radix_sort_pairs(B.tmp, bytes, kb, vb, (size_t)n, 0, 64,
-                                  c->stream);
+  if (c->capturing)
+    (void)rocprim::radix_sort_pairs<MergeSortConfig>(B.tmp, bytes, kb, vb, (size_t)n, 0, 64,
+                                                     c->stream);
+  else
+    (void)rocprim::radix_sort_pairs(B.tmp, bytes, kb, vb, (size_t)n, 0, 64, c->stream);
   // keep the sorted data in the primary buffers
   if (kb.current() != B.key) std::swap(B.key, B.key_alt);
   if (vb.current() != B.perm) std::swap(B.perm, B.perm_alt);

@@ -4,7 +4,10 @@
 // management of BarnesHutParticleSystem.cpp:28-84, 249-292.
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <vector>
 
 #include "engine.h"
@@ -110,11 +113,11 @@ hipEvent_t* step_events(stellar_ctx* c, int32_t step) {
 }
 
 void mark(stellar_ctx* c, hipEvent_t* ev, int k) {
-  if (timing(c)) (void)hipEventRecord(ev[k], c->stream);
+  if (ev) (void)hipEventRecord(ev[k], c->stream);
 }
 
 int run_step(stellar_ctx* c, int32_t step) {
-  hipEvent_t* ev = timing(c) ? step_events(c, step) : nullptr;
+  hipEvent_t* ev = timing(c) && !c->capturing ? step_events(c, step) : nullptr;
   mark(c, ev, 0);
   launch_bbox(c);
   mark(c, ev, 1);
@@ -134,6 +137,56 @@ int run_step(stellar_ctx* c, int32_t step) {
   if (e != hipSuccess) return hip_fail(c, e, "kernel launch");
   c->stage = kUpdateDone;
   c->sorted_storage = true;
+  return STELLAR_OK;
+}
+
+void drop_graphs(stellar_ctx* c) {
+  for (auto& g : c->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = stellar_ctx::Graph{};
+  }
+}
+
+// The two-step graph for the current buffer pointers, captured on first use.
+int two_step_graph(stellar_ctx* c, hipGraphExec_t* out) {
+  const Buffers& B = c->b;
+  const void* key[3] = {B.x, B.key, B.perm};
+  stellar_ctx::Graph* slot = &c->graphs[0];
+  for (auto& g : c->graphs) {
+    if (g.exec && g.key[0] == key[0] && g.key[1] == key[1] && g.key[2] == key[2]) {
+      g.used = ++c->graph_clock;
+      *out = g.exec;
+      return STELLAR_OK;
+    }
+    if (!g.exec || g.used < slot->used) slot = &g;
+  }
+  if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+  *slot = stellar_ctx::Graph{};
+  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  int rc = run_step(c, -1);
+  if (rc == STELLAR_OK) rc = run_step(c, -1);
+  c->capturing = false;
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+  // two steps swapped every double buffer twice: the pointers are back
+  if (B.x != key[0] || B.key != key[1] || B.perm != key[2]) {
+    (void)hipGraphDestroy(graph);
+    return fail(c, STELLAR_ERR_DEVICE, "buffers did not return after two captured steps");
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return hip_fail(c, ei, "hipGraphInstantiate");
+  for (int k = 0; k < 3; ++k) slot->key[k] = key[k];
+  slot->exec = exec;
+  slot->used = ++c->graph_clock;
+  *out = exec;
   return STELLAR_OK;
 }
 
@@ -373,6 +426,7 @@ int stellar_instance_offsets(const stellar_ctx* c, int32_t* offsets) {
 int stellar_destroy(stellar_ctx* c) {
   if (!c) return STELLAR_ERR_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   free_all(c);
   for (int k = 0; k < 8; ++k)
     if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
@@ -386,6 +440,7 @@ int stellar_set_params(stellar_ctx* c, const stellar_params* p) {
   if (!c) return STELLAR_ERR_ARG;
   if (!params_ok(p)) return fail(c, STELLAR_ERR_ARG, "parameters");
   c->p = *p;
+  drop_graphs(c);  // captured kernels carry the old parameters
   return STELLAR_OK;
 }
 
@@ -531,6 +586,23 @@ int stellar_step(stellar_ctx* c) {
 int stellar_run(stellar_ctx* c, int32_t steps) {
   if (!c || steps < 0) return STELLAR_ERR_ARG;
   if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
+  if (c->p.flags & STELLAR_FLAG_GRAPH) {
+    int32_t k = 0;
+    while (k + 2 <= steps) {
+      hipGraphExec_t exec = nullptr;
+      int rc = two_step_graph(c, &exec);
+      if (rc) return rc;
+      HIP_TRY(c, hipGraphLaunch(exec, c->stream));
+      c->stage = kUpdateDone;
+      c->sorted_storage = true;
+      k += 2;
+    }
+    for (; k < steps; ++k) {
+      int rc = run_step(c, -1);
+      if (rc) return rc;
+    }
+    return check_device(c);
+  }
   if (timing(c)) {
     int rc = reserve_run_events(c, steps);
     if (rc) return rc;
@@ -824,6 +896,88 @@ int stellar_energy_fp64(stellar_ctx* c, double eps2, double* kinetic,
   if (potential) *potential = tu;
   if (px) *px = tx;
   if (py) *py = ty;
+  return STELLAR_OK;
+}
+
+int stellar_autotune(stellar_ctx* c, int32_t group_mask, int32_t reps, stellar_tuning* out) {
+  if (!c || reps < 1 || group_mask < 0 || group_mask > 7) return STELLAR_ERR_ARG;
+  if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int, int>, stellar_tuning> cache;
+  int lg = 0;
+  for (int64_t per = c->n / c->ninst; per > 1; per >>= 1) ++lg;
+  int li = 0;
+  for (int32_t k = c->ninst; k > 1; k >>= 2) ++li;
+  const int fl = c->p.flags & (STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_TEXTBOOK_OPENING);
+  const auto key = std::make_tuple(lg, li, fl, group_mask, (int)c->p.precision);
+  auto apply = [&](const stellar_tuning& t) {
+    c->p.group = t.group;
+    c->walk_kpop = t.nodes_per_pop;
+    c->walk_waves = t.block_threads / 64;
+    drop_graphs(c);
+    if (out) *out = t;
+  };
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      apply(it->second);
+      return STELLAR_OK;
+    }
+  }
+  // a tree to walk
+  launch_bbox(c);
+  launch_build(c);
+  launch_com(c);
+  launch_sort(c);
+  HIP_TRY(c, hipGetLastError());
+  c->stage = kSortDone;
+  c->sorted_storage = true;
+  int rc = check_device(c);
+  if (rc) return rc;
+  const bool quad = (c->p.flags & STELLAR_FLAG_QUADRUPOLE) != 0;
+  std::vector<stellar_tuning> cand;
+  const int mask = group_mask == 0 ? 1 : group_mask;
+  if (mask & 1)
+    for (int kp : {1, 2, 4})
+      for (int w : {1, 2, 4, 8})
+        if (!(quad && kp == 4)) cand.push_back(stellar_tuning{64, kp, 64 * w, 0.f});
+  if (mask & 2) cand.push_back(stellar_tuning{32, 1, 256, 0.f});
+  if (mask & 4) cand.push_back(stellar_tuning{16, 1, 256, 0.f});
+  const stellar_params saved = c->p;
+  const int skp = c->walk_kpop, sw = c->walk_waves;
+  stellar_tuning best{};
+  best.walk_ms = 3.4e38f;
+  for (const stellar_tuning& t : cand) {
+    c->p.group = t.group;
+    c->walk_kpop = t.nodes_per_pop;
+    c->walk_waves = t.block_threads / 64;
+    launch_forces(c, true);  // untimed
+    HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+    for (int32_t r = 0; r < reps; ++r) launch_forces(c, true);
+    HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+    HIP_TRY(c, hipGetLastError());
+    rc = check_device(c);
+    if (rc) {
+      c->p = saved;
+      c->walk_kpop = skp;
+      c->walk_waves = sw;
+      return rc;
+    }
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    ms /= (float)reps;
+    if (ms < best.walk_ms) {
+      best = t;
+      best.walk_ms = ms;
+    }
+  }
+  c->stage = kForcesDone;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = best;
+  }
+  apply(best);
   return STELLAR_OK;
 }
 

@@ -134,6 +134,20 @@ struct stellar_ctx {
   bool sorted_storage = false;  // storage order == tree order of the build
   bool uploaded = false;
   hipEvent_t ev[8] = {};
+  // captured two-step graphs, keyed by the buffer pointers at capture time
+  // (every step swaps the body, key and permutation buffers once, so two
+  // steps return them to where they were)
+  struct Graph {
+    const void* key[3] = {};
+    hipGraphExec_t exec = nullptr;
+    uint64_t used = 0;
+  } graphs[4];
+  uint64_t graph_clock = 0;
+  bool capturing = false;
+  // walk launch shape (group 64): nodes popped per memory round trip and
+  // wavefronts per workgroup (stellar_autotune)
+  int walk_kpop = 0;   // 0: default (4 monopole, 2 quadrupole)
+  int walk_waves = 4;
   // per-step stage events of the current stellar_run (8 per step)
   std::vector<hipEvent_t> evrun;
   int32_t evrun_steps = 0;

@@ -109,11 +109,11 @@ constexpr int kStack64 = 1024;
 
 // The tree arrays are separate const __restrict__ arguments: that is what
 // lets the compiler read them through the scalar data cache.
-template <bool QUAD, int KPOP>
-__global__ __launch_bounds__(kWalkBlock) void walk64_kernel(
+template <bool QUAD, int KPOP, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
     const ChildRec* __restrict__ blk, const float4* __restrict__ qblk,
     WalkArgs A) {
-  constexpr int kWaves = kWalkBlock / 64;
+  constexpr int kWaves = WAVES;
   __shared__ uint32_t stk[kWaves][kStack64];
 
   const float eps2 = A.eps2;
@@ -334,16 +334,49 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
     atomicAdd(&A.dev->interactions, (unsigned long long)acc.cnt * (unsigned long long)cnt);
 }
 
+template <bool QUAD, int KPOP>
+void launch64(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
+  const int w = c->walk_waves;
+  const unsigned grid = (unsigned)((ngroups + w - 1) / w);
+  switch (w) {
+    case 1:
+      walk64_kernel<QUAD, KPOP, 1><<<grid, 64, 0, c->stream>>>(A.blk, A.qblk, A);
+      break;
+    case 2:
+      walk64_kernel<QUAD, KPOP, 2><<<grid, 128, 0, c->stream>>>(A.blk, A.qblk, A);
+      break;
+    case 8:
+      walk64_kernel<QUAD, KPOP, 8><<<grid, 512, 0, c->stream>>>(A.blk, A.qblk, A);
+      break;
+    default:
+      walk64_kernel<QUAD, KPOP, 4><<<grid, 256, 0, c->stream>>>(A.blk, A.qblk, A);
+      break;
+  }
+}
+
 template <int G>
-void launch_g(stellar_ctx* c, unsigned grid, const WalkArgs& A) {
+void launch_g(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
   const bool quad = (c->p.flags & STELLAR_FLAG_QUADRUPOLE) != 0;
   if (G == 64) {
-    if (quad)
-      walk64_kernel<true, 2><<<grid, kWalkBlock, 0, c->stream>>>(A.blk, A.qblk, A);
-    else
-      walk64_kernel<false, 4><<<grid, kWalkBlock, 0, c->stream>>>(A.blk, A.qblk, A);
+    const int kp = c->walk_kpop;
+    if (quad) {
+      if (kp == 1)
+        launch64<true, 1>(c, ngroups, A);
+      else
+        launch64<true, 2>(c, ngroups, A);
+    } else {
+      if (kp == 1)
+        launch64<false, 1>(c, ngroups, A);
+      else if (kp == 2)
+        launch64<false, 2>(c, ngroups, A);
+      else
+        launch64<false, 4>(c, ngroups, A);
+    }
     return;
   }
+  constexpr int64_t per_wave = 64 / G;
+  const int64_t waves = (ngroups + per_wave - 1) / per_wave;
+  const unsigned grid = (unsigned)((waves + (kWalkBlock / 64) - 1) / (kWalkBlock / 64));
   if (quad)
     walk_kernel<G, true><<<grid, kWalkBlock, 0, c->stream>>>(A.blk, A.qblk, A);
   else
@@ -358,9 +391,6 @@ void launch_forces(stellar_ctx* c, bool zero_counter) {
   Buffers& B = c->b;
   const int gi = c->p.group == 32 ? 1 : c->p.group == 16 ? 2 : 0;
   const int64_t ngroups = c->ngroups[gi];
-  const int64_t per_wave = 64 / (c->p.group == 32 ? 32 : c->p.group == 16 ? 16 : 64);
-  const int64_t waves = (ngroups + per_wave - 1) / per_wave;
-  const unsigned grid = (unsigned)((waves + (kWalkBlock / 64) - 1) / (kWalkBlock / 64));
   const float theta = c->p.theta;
   const float tdiv = (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? theta * theta : theta;
   WalkArgs A{B.x,       B.y,       B.blk,       B.qblk,   n,
@@ -369,13 +399,13 @@ void launch_forces(stellar_ctx* c, bool zero_counter) {
   if (zero_counter) launch_zero_counter(c);
   switch (c->p.group) {
     case 32:
-      launch_g<32>(c, grid, A);
+      launch_g<32>(c, ngroups, A);
       break;
     case 16:
-      launch_g<16>(c, grid, A);
+      launch_g<16>(c, ngroups, A);
       break;
     default:
-      launch_g<64>(c, grid, A);
+      launch_g<64>(c, ngroups, A);
       break;
   }
 }

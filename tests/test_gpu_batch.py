@@ -154,3 +154,71 @@ def test_batch_rejects_empty_instance(S):
     p = S.default_params()
     with pytest.raises(S.StellarError):
         S.Engine(params=p, counts=[10, 0, 5])
+
+
+def run_states(S, make, steps_plan, graph):
+    """Run the step counts of steps_plan one stellar_run call each (graph or
+    direct), downloading the state after each call."""
+    out = []
+    with make(graph) as e:
+        for k in steps_plan:
+            e.run(k)
+            out.append(e.download())
+    return out
+
+
+@pytest.mark.parametrize("kind", ["single", "ensemble"])
+def test_graph_replays_match_direct_steps(S, kind):
+    """Captured two-step graphs, replayed several times with direct steps and
+    host work in between, give the direct steps' results bit for bit."""
+    if kind == "single":
+        st = ic(2, 20000)
+
+        def make(graph):
+            flags = S.FLAG_GRAPH if graph else 0
+            p = S.default_params(eps2=EPS2, flags=flags)
+            e = S.Engine(st["x"].size, p)
+            e.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
+            return e
+    else:
+        sts = members(S, COUNTS, base_seed=5)
+
+        def make(graph):
+            flags = S.FLAG_GRAPH if graph else 0
+            return batch_engine(S, sts, flags=flags)
+    plan = [4, 3, 2, 5]
+    want = run_states(S, make, plan, graph=False)
+    got = run_states(S, make, plan, graph=True)
+    for i, (a, b) in enumerate(zip(want, got)):
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (i, k)
+
+
+def test_autotune_keeps_interactions(S):
+    st = ic(2, 65536)
+    with single_engine(S, st) as ref:
+        ref.bbox()
+        ref.build()
+        ref.com()
+        ref.sort()
+        ref.forces()
+        f0 = ref.export_forces()
+    with single_engine(S, st) as e:
+        t = e.autotune(group_mask=0, reps=2)
+        print("tuning", t, flush=True)
+        assert t["group"] == 64 and t["nodes_per_pop"] in (1, 2, 4)
+        assert t["block_threads"] in (64, 128, 256, 512) and t["walk_ms"] > 0
+        e.bbox()
+        e.build()
+        e.com()
+        e.sort()
+        e.forces()
+        f1 = e.export_forces()
+        # a second context of the same size class takes the cached choice
+        with single_engine(S, st) as e2:
+            t2 = e2.autotune(group_mask=0, reps=2)
+            assert t2 == t
+    assert np.array_equal(f0["inter"], f1["inter"])
+    num = (f1["ax"].astype(np.float64) - f0["ax"]) ** 2 + (f1["ay"].astype(np.float64) - f0["ay"]) ** 2
+    den = f0["ax"].astype(np.float64) ** 2 + f0["ay"].astype(np.float64) ** 2
+    assert np.sqrt(np.mean(num / den)) < 1e-6
