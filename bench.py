@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--quad", action="store_true", help="add cell quadrupole terms")
     ap.add_argument("--textbook", action="store_true",
                     help="opening test w/d <= theta instead of the reference's w^2/theta <= d^2")
+    ap.add_argument("--bmax", action="store_true",
+                    help="opening test bmax/d <= theta with a per-cell bound bmax")
     ap.add_argument("--error-sample", type=int, default=8192)
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3, help="timed oracle steps after 1 warm-up (0: skip)")
@@ -120,7 +122,8 @@ def run_ensemble(args, S):
     counts, sts = ensemble_members(S, args.instances)
     ntot = int(sum(counts))
     cat = {k: np.concatenate([s[k] for s in sts]) for k in ("mass", "x", "y", "vx", "vy")}
-    base_flags = (S.FLAG_QUADRUPOLE if args.quad else 0) | (S.FLAG_TEXTBOOK_OPENING if args.textbook else 0)
+    base_flags = ((S.FLAG_QUADRUPOLE if args.quad else 0) | (S.FLAG_TEXTBOOK_OPENING if args.textbook else 0)
+                  | (S.FLAG_BMAX_OPENING if args.bmax else 0))
 
     # batched form: one context, graph-captured steps, autotuned walk shape
     p = S.default_params(theta=args.theta, eps2=eps2, timestep=args.dt, group=args.group,
@@ -230,6 +233,7 @@ def main():
                          group=args.group, precision=prec,
                          flags=S.FLAG_STAGE_TIMING | (S.FLAG_QUADRUPOLE if args.quad else 0)
                          | (S.FLAG_TEXTBOOK_OPENING if args.textbook else 0)
+                         | (S.FLAG_BMAX_OPENING if args.bmax else 0)
                          | (S.FLAG_GRAPH if args.graph else 0))
     eng = S.Engine(n, p)
     eng.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
@@ -298,7 +302,8 @@ def main():
                    "n_bodies": n, "theta": args.theta, "eps": args.eps, "dt": args.dt,
                    "group": args.group, "precision": args.precision, "seed": 2,
                    "graph": bool(args.graph),
-                   "opening": ("textbook (w/d <= theta" if args.textbook else
+                   "opening": ("bmax (bmax/d <= theta" if args.bmax else
+                               "textbook (w/d <= theta" if args.textbook else
                                "reference (w^2/theta <= d^2+eps^2") + ", group vote)",
                    "multipoles": "monopole+quadrupole" if args.quad else "monopole"},
         "rms_rel_force_err": err_rms,

@@ -75,9 +75,19 @@ typedef struct stellar_params {
  * a += -Q.d/r^5 + (5/2)(d.Q.d) d/r^7 with r^2 = |d|^2 + eps2, moments
  * accumulated bottom-up in FP64. */
 #define STELLAR_FLAG_QUADRUPOLE 2
-/* Opening test w^2/theta^2 <= d^2+eps2, i.e. the textbook w/d <= theta of
- * the reference's older, commented-out kernel (kernels.cu:488-489). */
+/* Opening test w/d <= theta on the unsoftened distance d (the textbook test
+ * of the reference's older, commented-out kernel, kernels.cu:488-489),
+ * evaluated as w^2/theta^2 + eps2 <= d^2 + eps2 in FP32.  The reference's
+ * own test puts eps2 on one side only, so every cell narrower than about
+ * theta*eps passes, including the cell that holds the body itself. */
 #define STELLAR_FLAG_TEXTBOOK_OPENING 4
+/* Opening test with a per-cell radius instead of the cell width: a cell is
+ * accepted when every body of the walk group has bmax/theta <= d (unsoftened,
+ * as bmax^2/theta^2 + eps2 <= d^2 + eps2 in FP32),
+ * where bmax bounds the distance from the cell's centre of mass to its
+ * bodies (bottom-up in FP64: bmax = max over children of |c_k - C| +
+ * bmax_k).  Not in the reference; with it theta is a true angle bound. */
+#define STELLAR_FLAG_BMAX_OPENING 16
 /* stellar_run replays a captured HIP graph of two whole steps (captured on
  * first use for each arrangement of the context's double buffers; an odd
  * last step runs directly).  Stage times are not recorded in graph runs. */

@@ -434,8 +434,13 @@ void or_com_exact(int32_t n, int64_t ncells, const int32_t* child,
  * pushes it and then walks its (empty) child slots, so it drops out.
  *
  * Extensions (not in the reference, off by default):
- *   opening = 1: depth = radius^2/theta^2, i.e. the textbook w/d <= theta of
- *     the reference's older, commented-out kernel (kernels.cu:488-489).
+ *   opening = 1: depth = radius^2/theta^2 and the test is dp + eps2 <= r,
+ *     i.e. the textbook w/d <= theta of the reference's older, commented-out
+ *     kernel (kernels.cu:488-489) on the unsoftened distance d.  (With eps2
+ *     in both sides' r only, as the reference has it, every cell narrower
+ *     than about theta*eps passes -- including the cell that holds the body
+ *     itself, whose centre of mass then stands in for the body's neighbours.)
+ *   cthr != NULL: a cell's own threshold (or_bmax_exact) replaces dp.
  *   cq != NULL: accepted cells add their quadrupole term
  *     a += -Q.d/r^5 + (5/2)(d.Q.d) d/r^7, r^2 = |d|^2 + eps2. */
 #define OR_STACK 256
@@ -546,11 +551,12 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
             cmv = ccm[ch - n];
             if (cq) q = cq + 3 * (int64_t)(ch - n);
           }
+          const float th = (p->cthr && ch >= n) ? p->cthr[ch - n] : (p->opening ? dp + eps2 : dp);
           int all = 1;
           for (int32_t j = 0; j < gs; ++j) {
             const float dx = cxv - px[j], dy = cyv - py[j];
             rr[j] = dist2(dx, dy, eps2);
-            if (!(dp <= rr[j])) all = 0;
+            if (!(th <= rr[j])) all = 0;
           }
           if (ch < n || all) {
             for (int32_t j = 0; j < gs; ++j) {
@@ -584,8 +590,8 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
 
 /* Same acceptance rule stated recursively, accumulated in FP64. */
 typedef struct recctx {
-  int32_t n, gs;
-  const float *x, *y, *mass, *ccx, *ccy, *ccm, *cq;
+  int32_t n, gs, opening;
+  const float *x, *y, *mass, *ccx, *ccy, *ccm, *cq, *cthr;
   const int32_t* child;
   float eps2;
   const float* px;
@@ -626,11 +632,12 @@ static void rec_open(recctx* R, int32_t node, float depth) {
       continue;
     }
     const int32_t c = ch - R->n;
+    const float th = R->cthr ? R->cthr[c] : (R->opening ? dp + R->eps2 : dp);
     int all = 1;
     for (int32_t j = 0; j < R->gs && all; ++j) {
       const float dx = R->ccx[c] - R->px[j], dy = R->ccy[c] - R->py[j];
       const float r = dist2(dx, dy, R->eps2);
-      if (!(dp <= r)) all = 0;
+      if (!(th <= r)) all = 0;
     }
     if (all)
       rec_acc(R, R->ccx[c], R->ccy[c], R->ccm[c], R->cq ? R->cq + 3 * (int64_t)c : NULL);
@@ -664,7 +671,7 @@ int or_walk_recursive(int32_t n, const float* x, const float* y,
         px[G + j] = y[sorted[s0 + j]];
         f[j] = f[G + j] = 0.0;
       }
-      recctx R = {n, gs, x, y, mass, ccx, ccy, ccm, p->cq, child, p->eps2,
+      recctx R = {n, gs, p->opening, x, y, mass, ccx, ccy, ccm, p->cq, p->cthr, child, p->eps2,
                   px, px + G, f, f + G, 0};
       for (int k = 0; k < 4; ++k) {
         const int32_t ch = root[k];
@@ -748,6 +755,54 @@ void or_quad_exact(int32_t n, int64_t ncells, const int32_t* child,
     cq[3 * c + 0] = (float)qxx;
     cq[3 * c + 1] = (float)qxy;
     cq[3 * c + 2] = (float)qyy;
+  }
+  free(S);
+}
+
+/* Opening thresholds from the bmax bound (not in the reference). */
+void or_bmax_exact(int32_t n, int64_t ncells, const int32_t* child,
+                   const float* x, const float* y, const float* mass,
+                   float theta, float eps2, float* cthr) {
+  double* S = (double*)malloc(sizeof(double) * 4 * (size_t)(ncells + 1));
+  const double t2 = (double)theta * (double)theta;
+  for (int64_t c = ncells - 1; c >= 0; --c) {
+    double cx[4], cy[4], cb[4];
+    double sm = 0.0, sx = 0.0, sy = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      const int32_t ch = child[4 * c + k];
+      cx[k] = cy[k] = cb[k] = 0.0;
+      if (ch < 0) continue;
+      if (ch < n) {
+        const double m = (double)mass[ch];
+        cx[k] = (double)x[ch];
+        cy[k] = (double)y[ch];
+        sm += m;
+        sx += m * cx[k];
+        sy += m * cy[k];
+      } else {
+        const double* q = S + 4 * (size_t)(ch - n);
+        cx[k] = q[0] != 0.0 ? q[1] / q[0] : 0.0;
+        cy[k] = q[0] != 0.0 ? q[2] / q[0] : 0.0;
+        cb[k] = q[3];
+        sm += q[0];
+        sx += q[1];
+        sy += q[2];
+      }
+    }
+    const double ox = sm != 0.0 ? sx / sm : 0.0, oy = sm != 0.0 ? sy / sm : 0.0;
+    double b = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      if (child[4 * c + k] < 0) continue;
+      const double dx = cx[k] - ox, dy = cy[k] - oy;
+      const double d = sqrt(fma(dx, dx, dy * dy)) + cb[k];
+      b = d > b ? d : b;
+    }
+    double* o = S + 4 * (size_t)c;
+    o[0] = sm;
+    o[1] = sx;
+    o[2] = sy;
+    o[3] = b;
+    cthr[c] = (float)((b * b) / t2 + (double)eps2);
   }
   free(S);
 }
@@ -895,7 +950,7 @@ int or_step(int32_t n, float* x, float* y, float* vx, float* vy,
   t[4] = now_s();
   {
     /* the walk reads the pool layout: cells at n.. with CoM in cx/cy/cm */
-    or_walk_params wp = {p->theta, p->eps2, p->group, p->threads, 0, 0, NULL};
+    or_walk_params wp = {p->theta, p->eps2, p->group, p->threads, 0, 0, NULL, NULL};
     rc = or_walk(n, x, y, mass, bbox, sorted, child, child + 4 * (size_t)n,
                  ncells, cx + n, cy + n, cm + n, &wp, ax, ay, NULL);
     if (rc) goto out;

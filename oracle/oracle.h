@@ -98,9 +98,12 @@ typedef struct or_walk_params {
   float eps2;
   int32_t group;     /* walk-group size (32 = reference warp) */
   int32_t threads;   /* OpenMP threads, <=0 = all */
-  int32_t opening;   /* 0: w^2/theta <= r (reference); 1: w^2/theta^2 <= r */
+  int32_t opening;   /* 0: w^2/theta <= r (reference);
+                        1: w^2/theta^2 + eps2 <= r (textbook, unsoftened d) */
   int32_t pad;
   const float* cq;   /* NULL: monopole; else 3 quadrupole terms per cell */
+  const float* cthr; /* NULL: per-level thresholds; else one per cell
+                        (or_bmax_exact), compared with r like dp */
 } or_walk_params;
 
 int or_walk(int32_t n, const float* x, const float* y, const float* mass,
@@ -108,6 +111,16 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
             const int32_t* child, int64_t ncells, const float* ccx,
             const float* ccy, const float* ccm, const or_walk_params* p,
             float* ax, float* ay, int32_t* inter);
+
+/* Opening thresholds from the bottom-up bound bmax (the largest distance
+ * from a cell's FP64 centre of mass to any of its bodies, bounded through
+ * the children: bmax = max_k |c_k - C| + bmax_k, bodies 0):
+ * cthr[c] = (float)(bmax^2 / theta^2 + eps2), |c_k - C| = sqrt(fma(dx,dx,dy*dy)).
+ * A cell is accepted when every body of the group has cthr <= r, i.e.
+ * bmax/theta <= d on the unsoftened distance. */
+void or_bmax_exact(int32_t n, int64_t ncells, const int32_t* child,
+                   const float* x, const float* y, const float* mass,
+                   float theta, float eps2, float* cthr);
 
 /* Recursive (stack-free) statement of the same acceptance rule; FP64
  * accumulation.  Used to pin or_walk's traversal. */

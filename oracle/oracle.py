@@ -34,7 +34,7 @@ _I64 = C.POINTER(C.c_int64)
 
 class WalkParams(C.Structure):
     _fields_ = [("theta", C.c_float), ("eps2", C.c_float), ("group", C.c_int32), ("threads", C.c_int32),
-                ("opening", C.c_int32), ("pad", C.c_int32), ("cq", _F)]
+                ("opening", C.c_int32), ("pad", C.c_int32), ("cq", _F), ("cthr", _F)]
 
 
 class StepParams(C.Structure):
@@ -51,6 +51,7 @@ _SIG = {
     "or_canonicalize": (C.c_int, [C.c_int32, C.c_int32, _I, _I, _I, _F, _F, _F, _I, _I, _I, _I, _I, _F, _F, _F]),
     "or_build_keys": (C.c_int, [C.c_int32, _F, _F, _F, C.c_int64, _I, _I, _I, _I, _I, _I, _I64]),
     "or_com_exact": (None, [C.c_int32, C.c_int64, _I, _F, _F, _F, _F, _F, _F]),
+    "or_bmax_exact": (None, [C.c_int32, C.c_int64, _I, _F, _F, _F, C.c_float, C.c_float, _F]),
     "or_quad_exact": (None, [C.c_int32, C.c_int64, _I, _F, _F, _F, _F]),
     "or_walk": (C.c_int, [C.c_int32, _F, _F, _F, _F, _I, _I, _I, C.c_int64, _F, _F, _F,
                           C.POINTER(WalkParams), _F, _F, _I]),
@@ -213,20 +214,34 @@ def quad_exact(tree: dict, x, y, mass) -> None:
     tree["quad"] = q[: 3 * c]
 
 
-def _wp(tree, theta, eps2, group, threads, opening, quad):
+def _wp(tree, theta, eps2, group, threads, opening, quad, bmax=False):
     cq = None
     if quad:
         cq = tree["quad"] if tree["quad"].size else np.zeros(3, np.float32)
-    p = WalkParams(theta, eps2, group, threads, opening, 0, _f(cq))
-    return p, cq
+    th = None
+    if bmax:
+        th = tree["thr"] if tree["thr"].size else np.zeros(1, np.float32)
+    p = WalkParams(theta, eps2, group, threads, opening, 0, _f(cq), _f(th))
+    return p, (cq, th)
+
+
+def bmax_exact(tree: dict, x, y, mass, theta: float, eps2: float) -> None:
+    """Per-cell opening thresholds bmax^2/theta^2 + eps2 into tree["thr"]."""
+    x, y, mass = f32(x), f32(y), f32(mass)
+    c = tree["ncells"]
+    t = np.zeros(max(c, 1), np.float32)
+    lib().or_bmax_exact(x.size, c, _i(tree["child"]), _f(x), _f(y), _f(mass), float(theta),
+                        float(eps2), _f(t))
+    tree["thr"] = t[:c]
 
 
 def walk(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0,
-         opening: int = 0, quad: bool = False):
-    """Group-vote walk (kernels.cu:340-435) over a canonical tree; by body id."""
+         opening: int = 0, quad: bool = False, bmax: bool = False):
+    """Group-vote walk (kernels.cu:340-435) over a canonical tree; by body id.
+    bmax: per-cell thresholds tree["thr"] (bmax_exact) instead of levels."""
     x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
     n = x.size
-    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad)
+    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad, bmax)
     ax = np.zeros(n, np.float32)
     ay = np.zeros(n, np.float32)
     it = np.zeros(n, np.int32)
@@ -243,10 +258,10 @@ def walk(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, thre
 
 
 def walk_recursive(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0,
-                   opening: int = 0, quad: bool = False):
+                   opening: int = 0, quad: bool = False, bmax: bool = False):
     x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
     n = x.size
-    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad)
+    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad, bmax)
     ax = np.zeros(n, np.float64)
     ay = np.zeros(n, np.float64)
     it = np.zeros(n, np.int32)

@@ -37,6 +37,7 @@ FLAG_STAGE_TIMING = 1
 FLAG_QUADRUPOLE = 2
 FLAG_TEXTBOOK_OPENING = 4
 FLAG_GRAPH = 8
+FLAG_BMAX_OPENING = 16
 
 IC_DISK = 0
 IC_PLUMMER = 1

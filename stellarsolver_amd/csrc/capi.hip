@@ -941,7 +941,8 @@ int stellar_autotune(stellar_ctx* c, int32_t group_mask, int32_t reps, stellar_t
   if (mask & 1)
     for (int kp : {1, 2, 4})
       for (int w : {1, 2, 4, 8})
-        if (!(quad && kp == 4)) cand.push_back(stellar_tuning{64, kp, 64 * w, 0.f});
+        if (!((quad || (c->p.flags & STELLAR_FLAG_BMAX_OPENING)) && kp == 4))
+          cand.push_back(stellar_tuning{64, kp, 64 * w, 0.f});
   if (mask & 2) cand.push_back(stellar_tuning{32, 1, 256, 0.f});
   if (mask & 4) cand.push_back(stellar_tuning{16, 1, 256, 0.f});
   const stellar_params saved = c->p;

@@ -45,6 +45,9 @@ struct ComArgs {
   DevScalars* dev;
   int quad;
   int32_t nroot;  // root blocks (instances); cell c is block nroot + c
+  int bmax;       // per-cell opening thresholds bmax^2/t2 + eps2 in qblk .w
+  double t2;      // theta^2
+  double eps2;
 };
 
 template <bool EXT>
@@ -177,11 +180,41 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
       qxy += axy;
       qyy += ayy;
     }
-    A.qblk[4 * (int64_t)p + s] = make_float4((float)qxx, (float)qxy, (float)qyy, 0.f);
   }
+  float thr = 0.f;
+  double bm = 0.0;
+  if (A.bmax) {
+    // bmax bound: the largest distance from the FP64 centre of mass to a
+    // body, through the children (bodies contribute their distance)
+    const double ox = sm != 0.0 ? sx / sm : 0.0, oy = sm != 0.0 ? sy / sm : 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int32_t code = L.rec[i][k].code;
+      if (code == 0) continue;
+      double q[kMom];
+      child_mom<kMom, EXT>(L, i, k, code, q);
+      double cx, cy, cb;
+      if (code < 0) {
+        cx = q[1];
+        cy = q[2];
+        cb = 0.0;
+      } else {
+        cx = q[0] != 0.0 ? q[1] / q[0] : 0.0;
+        cy = q[0] != 0.0 ? q[2] / q[0] : 0.0;
+        cb = q[6];
+      }
+      const double dx = cx - ox, dy = cy - oy;
+      const double d = sqrt(__builtin_fma(dx, dx, dy * dy)) + cb;
+      bm = d > bm ? d : bm;
+    }
+    thr = (float)((bm * bm) / A.t2 + A.eps2);
+  }
+  if (A.quad || A.bmax)
+    A.qblk[4 * (int64_t)p + s] = make_float4((float)qxx, (float)qxy, (float)qyy, thr);
   out[3] = qxx;
   out[4] = qxy;
   out[5] = qyy;
+  out[6] = bm;
   double* g = A.csum + kSumStride * (int64_t)b;
 #pragma unroll
   for (int t = 0; t < kMom; ++t) {
@@ -388,7 +421,9 @@ void launch_com(stellar_ctx* c) {
   if (c->n == 0 || c->cap == 0) return;
   Buffers& B = c->b;
   ComArgs A{B.blk,  B.qblk, B.cparent, B.cslot, B.clevel, B.csum, B.ccom, B.dev,
-            (c->p.flags & STELLAR_FLAG_QUADRUPOLE) ? 1 : 0, c->ninst};
+            (c->p.flags & STELLAR_FLAG_QUADRUPOLE) ? 1 : 0, c->ninst,
+            (c->p.flags & STELLAR_FLAG_BMAX_OPENING) ? 1 : 0,
+            (double)c->p.theta * (double)c->p.theta, (double)c->p.eps2};
   // pass 1 over all cells; each later list holds at most kSlots cells per
   // chunk of the pass before, so grids are sized from that bound and chunks
   // beyond the actual length exit at once

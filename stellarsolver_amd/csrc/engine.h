@@ -27,8 +27,8 @@ constexpr int64_t kMaxBlocks = int64_t(1) << 26;  // root + cells
 constexpr int kMaxTieRun = 256;
 constexpr int kWalkStack = 256;  // >= 4 + 3*(kMaxLevel-1)
 constexpr int kWalkBlock = 256;  // threads per walk workgroup (4 wavefronts)
-// FP64 moments per cell: sum m, sum m x, sum m y, Qxx, Qxy, Qyy
-constexpr int kSumStride = 6;
+// FP64 moments per cell: sum m, sum m x, sum m y, Qxx, Qxy, Qyy, bmax
+constexpr int kSumStride = 7;
 
 // Child record of a tree block: the walk's only view of the tree.  A block is
 // the four child slots of one node (block 0 = root, block 1+c = cell c),

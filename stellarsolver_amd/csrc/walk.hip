@@ -54,6 +54,7 @@ struct WalkArgs {
   int64_t ngroups;
   const int32_t* inst_of;
   const float4* ibbox;
+  int unsoft;  // level thresholds compare with the unsoftened distance
 };
 
 // r = dx*dx + dy*dy + eps2 as dx*dx + (dy*dy + eps2), two fused
@@ -109,7 +110,7 @@ constexpr int kStack64 = 1024;
 
 // The tree arrays are separate const __restrict__ arguments: that is what
 // lets the compiler read them through the scalar data cache.
-template <bool QUAD, int KPOP, int WAVES>
+template <bool QUAD, bool THR, int KPOP, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
     const ChildRec* __restrict__ blk, const float4* __restrict__ qblk,
     WalkArgs A) {
@@ -137,6 +138,8 @@ __global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
   const float depth0 = root_depth(A, inst);
   float thr = depth0;
   for (int i = 0; i <= lane; ++i) thr = 0.25f * thr;
+  // textbook test on the unsoftened distance: dp + eps2 <= d^2 + eps2
+  if (A.unsoft) thr = thr + eps2;
 
   uint32_t* st = stk[wave];
   int sp = 0;
@@ -172,16 +175,17 @@ __global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
       dp[j] = __int_as_float(
           __builtin_amdgcn_readlane(__float_as_int(thr), (int)lvl[j]));
     }
+    constexpr bool kQ = QUAD || THR;  // the node's second block is needed
     int4 q[KPOP][4];
-    float4 qm[QUAD ? KPOP : 1][4];
+    float4 qm[kQ ? KPOP : 1][4];
 #pragma unroll
     for (int j = 0; j < KPOP; ++j) {
       const int4* Bq = reinterpret_cast<const int4*>(blk + 4 * (int64_t)node[j]);
 #pragma unroll
       for (int k = 0; k < 4; ++k) q[j][k] = Bq[k];
-      if (QUAD) {
+      if (kQ) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) qm[QUAD ? j : 0][k] = qblk[4 * (int64_t)node[j] + k];
+        for (int k = 0; k < 4; ++k) qm[kQ ? j : 0][k] = qblk[4 * (int64_t)node[j] + k];
       }
     }
 #pragma unroll
@@ -198,11 +202,12 @@ __global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
         const float r = dist2(dx, dy, eps2);
         // one vote for bodies and cells: a body always passes (threshold -1
         // against r >= eps2 > 0), so acceptance is a single scalar test
-        const int32_t ti = code < 0 ? __float_as_int(-1.0f) : __float_as_int(dp[jj]);
+        const float cthr = THR ? qm[kQ ? jj : 0][k].w : dp[jj];
+        const int32_t ti = code < 0 ? __float_as_int(-1.0f) : __float_as_int(cthr);
         const float t = __int_as_float(__builtin_amdgcn_readfirstlane(ti));
         if (__builtin_amdgcn_ballot_w64(!(t <= r)) == 0) {
           if (QUAD && code > 0)
-            acc.quad(cmv, r, dx, dy, qm[QUAD ? jj : 0][k]);
+            acc.quad(cmv, r, dx, dy, qm[kQ ? jj : 0][k]);
           else
             acc.mono(cmv, r, dx, dy);
         } else {
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
 }
 
 // ----------------------------------------------------------- G = 32, 16 --
-template <int G, bool QUAD>
+template <int G, bool QUAD, bool THR>
 __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
     const ChildRec* __restrict__ blk, const float4* __restrict__ qblk,
     WalkArgs A) {
@@ -288,7 +293,7 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
 #pragma unroll
     for (int k = 0; k < 4; ++k) q[k] = B[k];
     float4 qm[4];
-    if (QUAD) {
+    if (QUAD || THR) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) qm[k] = qblk[4 * (int64_t)node + k];
     }
@@ -304,7 +309,8 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
         acc.mono(cmv, r, dx, dy);
         continue;
       }
-      const uint64_t bad = __ballot(valid && !(dp <= r)) & gmask;
+      const float th = THR ? qm[k].w : (A.unsoft ? dp + eps2 : dp);
+      const uint64_t bad = __ballot(valid && !(th <= r)) & gmask;
       if (bad == 0) {
         if (QUAD)
           acc.quad(cmv, r, dx, dy, qm[k]);
@@ -334,53 +340,64 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
     atomicAdd(&A.dev->interactions, (unsigned long long)acc.cnt * (unsigned long long)cnt);
 }
 
-template <bool QUAD, int KPOP>
+template <bool QUAD, bool THR, int KPOP>
 void launch64(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
   const int w = c->walk_waves;
   const unsigned grid = (unsigned)((ngroups + w - 1) / w);
   switch (w) {
     case 1:
-      walk64_kernel<QUAD, KPOP, 1><<<grid, 64, 0, c->stream>>>(A.blk, A.qblk, A);
+      walk64_kernel<QUAD, THR, KPOP, 1><<<grid, 64, 0, c->stream>>>(A.blk, A.qblk, A);
       break;
     case 2:
-      walk64_kernel<QUAD, KPOP, 2><<<grid, 128, 0, c->stream>>>(A.blk, A.qblk, A);
+      walk64_kernel<QUAD, THR, KPOP, 2><<<grid, 128, 0, c->stream>>>(A.blk, A.qblk, A);
       break;
     case 8:
-      walk64_kernel<QUAD, KPOP, 8><<<grid, 512, 0, c->stream>>>(A.blk, A.qblk, A);
+      walk64_kernel<QUAD, THR, KPOP, 8><<<grid, 512, 0, c->stream>>>(A.blk, A.qblk, A);
       break;
     default:
-      walk64_kernel<QUAD, KPOP, 4><<<grid, 256, 0, c->stream>>>(A.blk, A.qblk, A);
+      walk64_kernel<QUAD, THR, KPOP, 4><<<grid, 256, 0, c->stream>>>(A.blk, A.qblk, A);
       break;
   }
 }
 
-template <int G>
-void launch_g(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
-  const bool quad = (c->p.flags & STELLAR_FLAG_QUADRUPOLE) != 0;
+// Nodes per pop: 4 when a node is one 64-byte block, at most 2 when it also
+// brings its second block (quadrupoles or per-cell thresholds), to bound
+// scalar register use.
+template <bool QUAD, bool THR>
+void launch64_kpop(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
+  const int kp = c->walk_kpop;
+  if (kp == 1)
+    launch64<QUAD, THR, 1>(c, ngroups, A);
+  else if (kp == 2 || QUAD || THR)
+    launch64<QUAD, THR, 2>(c, ngroups, A);
+  else
+    launch64<QUAD, THR, 4>(c, ngroups, A);
+}
+
+template <int G, bool QUAD, bool THR>
+void launch_shape(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
   if (G == 64) {
-    const int kp = c->walk_kpop;
-    if (quad) {
-      if (kp == 1)
-        launch64<true, 1>(c, ngroups, A);
-      else
-        launch64<true, 2>(c, ngroups, A);
-    } else {
-      if (kp == 1)
-        launch64<false, 1>(c, ngroups, A);
-      else if (kp == 2)
-        launch64<false, 2>(c, ngroups, A);
-      else
-        launch64<false, 4>(c, ngroups, A);
-    }
+    launch64_kpop<QUAD, THR>(c, ngroups, A);
     return;
   }
   constexpr int64_t per_wave = 64 / G;
   const int64_t waves = (ngroups + per_wave - 1) / per_wave;
   const unsigned grid = (unsigned)((waves + (kWalkBlock / 64) - 1) / (kWalkBlock / 64));
-  if (quad)
-    walk_kernel<G, true><<<grid, kWalkBlock, 0, c->stream>>>(A.blk, A.qblk, A);
+  walk_kernel<G, QUAD, THR><<<grid, kWalkBlock, 0, c->stream>>>(A.blk, A.qblk, A);
+}
+
+template <int G>
+void launch_g(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
+  const bool quad = (c->p.flags & STELLAR_FLAG_QUADRUPOLE) != 0;
+  const bool thr = (c->p.flags & STELLAR_FLAG_BMAX_OPENING) != 0;
+  if (quad && thr)
+    launch_shape<G, true, true>(c, ngroups, A);
+  else if (quad)
+    launch_shape<G, true, false>(c, ngroups, A);
+  else if (thr)
+    launch_shape<G, false, true>(c, ngroups, A);
   else
-    walk_kernel<G, false><<<grid, kWalkBlock, 0, c->stream>>>(A.blk, A.qblk, A);
+    launch_shape<G, false, false>(c, ngroups, A);
 }
 
 }  // namespace
@@ -395,7 +412,8 @@ void launch_forces(stellar_ctx* c, bool zero_counter) {
   const float tdiv = (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? theta * theta : theta;
   WalkArgs A{B.x,       B.y,       B.blk,       B.qblk,   n,
              c->p.eps2, tdiv,      B.dev,       B.ax,     B.ay,
-             B.inter,   B.gtab[gi], ngroups,    B.inst_of, B.ibbox};
+             B.inter,   B.gtab[gi], ngroups,    B.inst_of, B.ibbox,
+             (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? 1 : 0};
   if (zero_counter) launch_zero_counter(c);
   switch (c->p.group) {
     case 32:

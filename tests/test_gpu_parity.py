@@ -287,3 +287,30 @@ def test_quadrupole_headline_error(S):
         print(f"quadrupole {name} opening: rms relative error {res[name]:.3e}, "
               f"interactions/body {f['inter'].mean():.1f}", flush=True)
     assert res["textbook"] < 1e-3
+
+
+@pytest.mark.parametrize("quad", [False, True])
+@pytest.mark.parametrize("group", [64, 32, 16])
+@pytest.mark.parametrize("workload,n", [(2, 65536), (1, 65536)])
+def test_bmax_opening_parity(S, oracle, workload, n, group, quad):
+    """Per-cell bmax thresholds: the interaction set equals the oracle's (so
+    the FP64 bound and its FP32 threshold agree bit for bit); forces within
+    the rsqrt tolerance."""
+    print(f"bmax parity workload={workload} n={n} group={group} quad={quad}", flush=True)
+    st = ic(workload, n)
+    eps2 = float(np.float32(EPS[workload] ** 2))
+    theta = 0.6
+    flags = S.FLAG_BMAX_OPENING | (S.FLAG_QUADRUPOLE if quad else 0)
+    with make_engine(S, st, eps2, theta=theta, group=group, flags=flags) as e:
+        gpu_tree(e)
+        e.sort()
+        e.forces()
+        f = e.export_forces()
+    obb, ot = oracle_tree(oracle, st)
+    x, y, m = st["x"], st["y"], st["mass"]
+    oracle.bmax_exact(ot, x, y, m, theta, eps2)
+    if quad:
+        oracle.quad_exact(ot, x, y, m)
+    ax, ay, it = oracle.walk(ot, x, y, m, obb, theta, eps2, group, quad=quad, bmax=True)
+    assert np.array_equal(f["inter"], it)
+    assert rms_rel(f["ax"], f["ay"], ax, ay) < FORCE_RMS_TOL

@@ -393,10 +393,122 @@ def test_quad_walk_matches_recursive_statement(oracle, opening):
 
 
 def test_textbook_opening_is_theta_squared(oracle):
-    """w/d <= theta (textbook) equals the reference test at theta^2."""
+    """w/d <= theta (textbook, unsoftened d) equals the reference test at
+    theta^2 when the softening vanishes (eps2 = 2^-126, below every
+    threshold's rounding)."""
     st = ic(2, 20000)
     bb, kt = _tree(oracle, st)
     x, y, m = st["x"], st["y"], st["mass"]
-    _, _, a = oracle.walk(kt, x, y, m, bb, 0.5, 1e-4, 64, opening=1)
-    _, _, b = oracle.walk(kt, x, y, m, bb, 0.25, 1e-4, 64, opening=0)
+    tiny = float(np.finfo(np.float32).tiny)
+    _, _, a = oracle.walk(kt, x, y, m, bb, 0.5, tiny, 64, opening=1)
+    _, _, b = oracle.walk(kt, x, y, m, bb, 0.25, tiny, 64, opening=0)
     assert np.array_equal(a, b)
+
+
+def test_textbook_opening_never_accepts_own_cell(oracle):
+    """A tight cluster of three bodies (well inside the softening length) in
+    a field of 300: the reference's softened test accepts every cell
+    narrower than about theta*eps, including the one that holds the target
+    itself, so its error stays ~1e-3 however small theta gets; the textbook
+    test on the unsoftened distance opens it and converges."""
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(-50, 50, 300), [10.0, 10.0001, 10.0]]).astype(np.float32)
+    y = np.concatenate([rng.uniform(-50, 50, 300), [10.0, 10.0, 10.00013]]).astype(np.float32)
+    m = np.ones(x.size, np.float32)
+    m[-1] = 5.0
+    st = {"x": x, "y": y, "mass": m}
+    bb, kt = _tree(oracle, st)
+    eps2 = 1e-2
+    tg = np.array([300, 301, 302], np.int32)
+    dx, dy = oracle.direct_fp64(x, y, m, eps2, targets=tg)
+    angle = 0.01
+    ax, ay, _ = oracle.walk(kt, x, y, m, bb, angle * angle, eps2, 1, opening=0)
+    ref = np.hypot(ax[tg] - dx, ay[tg] - dy) / np.hypot(dx, dy)
+    ax, ay, _ = oracle.walk(kt, x, y, m, bb, angle, eps2, 1, opening=1)
+    tb = np.hypot(ax[tg] - dx, ay[tg] - dy) / np.hypot(dx, dy)
+    assert ref.max() > 1e-3
+    assert tb.max() < 1e-5
+
+
+# ---- bmax opening (per-cell thresholds) ------------------------------------
+EPS2_BMAX = 1e-4
+def test_bmax_two_bodies_closed_form(oracle):
+    """One cell of two bodies: bmax is the larger body distance from the
+    centre of mass, threshold bmax^2/theta^2."""
+    x = np.array([1.0, 1.5], np.float32)
+    y = np.array([1.0, 1.25], np.float32)
+    m = np.array([3.0, 1.0], np.float32)
+    st = {"x": x, "y": y, "mass": m}
+    bb, kt = _tree(oracle, st)
+    oracle.bmax_exact(kt, x, y, m, 0.5, EPS2_BMAX)
+    ox = (3.0 * 1.0 + 1.0 * 1.5) / 4.0
+    oy = (3.0 * 1.0 + 1.0 * 1.25) / 4.0
+    bm = max(np.hypot(1.0 - ox, 1.0 - oy), np.hypot(1.5 - ox, 1.25 - oy))
+    # every cell on the chain holds both bodies: same bound everywhere
+    assert kt["ncells"] >= 1
+    np.testing.assert_allclose(kt["thr"], np.float32(bm * bm / 0.25 + EPS2_BMAX), rtol=1e-6)
+
+
+def test_bmax_bounds_the_body_distances(oracle):
+    """bmax of each cell is at least the largest distance from its centre of
+    mass to any of its bodies, exactly that when every child is a body, and
+    at most the cell diagonal plus the largest child bound."""
+    st = ic(1, 5000)
+    x, y, m = st["x"], st["y"], st["mass"]
+    theta = 0.7
+    bb, kt = _tree(oracle, st)
+    tiny = float(np.finfo(np.float32).tiny)  # keeps bmax recoverable from the threshold
+    oracle.bmax_exact(kt, x, y, m, theta, tiny)
+    s = kt["sorted"]
+    X, Y, M = x[s].astype(np.float64), y[s].astype(np.float64), m[s].astype(np.float64)
+    n = x.size
+    bmax = np.sqrt(kt["thr"].astype(np.float64)) * theta
+    for c in range(0, kt["ncells"], 5):
+        a, k = kt["start"][c], kt["count"][c]
+        mm, xx, yy = M[a:a + k], X[a:a + k], Y[a:a + k]
+        ox, oy = (mm * xx).sum() / mm.sum(), (mm * yy).sum() / mm.sum()
+        true = np.max(np.hypot(xx - ox, yy - oy))
+        assert bmax[c] >= true * (1 - 1e-5) - 1e-9, c
+        ch = kt["child"][4 * c:4 * c + 4]
+        if np.all(ch < n):
+            assert bmax[c] == pytest.approx(true, rel=1e-5, abs=1e-9), c
+
+
+def test_bmax_walk_theta_to_zero_is_direct_sum(oracle):
+    st = ic(0, 2000)
+    x, y, m = st["x"], st["y"], st["mass"]
+    bb, kt = _tree(oracle, st)
+    oracle.bmax_exact(kt, x, y, m, 1e-6, EPS2_BMAX)
+    ax, ay, it = oracle.walk(kt, x, y, m, bb, 1e-6, 1e-4, 32, bmax=True)
+    dx, dy = oracle.direct_fp64(x, y, m, 1e-4)
+    assert np.all(it == x.size)
+    assert np.sqrt(np.mean(((ax - dx) ** 2 + (ay - dy) ** 2) / (dx ** 2 + dy ** 2))) < 1e-5
+
+
+@pytest.mark.parametrize("quad", [False, True])
+def test_bmax_walk_matches_recursive_statement(oracle, quad):
+    st = ic(2, 20000)
+    bb, kt = _tree(oracle, st)
+    x, y, m = st["x"], st["y"], st["mass"]
+    oracle.bmax_exact(kt, x, y, m, 0.5, EPS2_BMAX)
+    if quad:
+        oracle.quad_exact(kt, x, y, m)
+    args = (kt, x, y, m, bb, 0.5, 1e-4, 64)
+    ax, ay, it = oracle.walk(*args, quad=quad, bmax=True)
+    rx, ry, rit = oracle.walk_recursive(*args, quad=quad, bmax=True)
+    assert np.array_equal(it, rit)
+    assert np.sqrt(np.mean(((ax - rx) ** 2 + (ay - ry) ** 2) / (rx ** 2 + ry ** 2))) < 5e-6
+
+
+def test_bmax_error_falls_with_theta(oracle):
+    st = ic(2, 20000)
+    x, y, m = st["x"], st["y"], st["mass"]
+    bb, kt = _tree(oracle, st)
+    tg = np.arange(0, x.size, 37, dtype=np.int32)
+    dx, dy = oracle.direct_fp64(x, y, m, 1e-4, targets=tg)
+    errs = []
+    for th in (0.8, 0.5, 0.3):
+        oracle.bmax_exact(kt, x, y, m, th, EPS2_BMAX)
+        ax, ay, _ = oracle.walk(kt, x, y, m, bb, th, 1e-4, 64, bmax=True)
+        errs.append(np.sqrt(np.mean(((ax[tg] - dx) ** 2 + (ay[tg] - dy) ** 2) / (dx ** 2 + dy ** 2))))
+    assert errs[0] > errs[1] > errs[2]
