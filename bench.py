@@ -4,6 +4,10 @@
 Workload (default): colliding disks, N = 2^20 (two disks of 524,288 bodies,
 central masses 1e4, r ~ U[2,200], centres (+-300, 0), bulk velocities
 (-+5, +-1)), theta = 0.5, eps = 0.01, dt = 1e-3, seed 2, FP32 state.
+Force model (default): cell quadrupoles with the textbook opening test
+w/d <= theta, the configuration that holds the rms force error under 1e-3;
+the reference's own criterion (monopoles, w^2/theta <= d^2 + eps^2) is run in
+the same invocation and reported beside it ("reference_opening").
 
 One "step" = the six stages (bbox, build, centre of mass, tree-order sort,
 tree walk, update) plus the output buffer, over bodies already resident in
@@ -33,6 +37,9 @@ VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 # algorithmic VALU issue slots of one body-node interaction: 2 sub, 2 FMA for
 # d^2+eps^2, 1 rsqrt at quarter rate (4 slots), 3 mul, 2 FMA accumulate
 OPS_PER_INTERACTION = 13
+# a cell's quadrupole term adds 14: rs^2, rs^5 (2), rs^7, Q.d (4), d.Q.d (2),
+# the radial factor (2), and one more FMA per component (2)
+OPS_PER_QUAD_TERM = 14
 HBM_PEAK_GBS = 8000.0
 
 
@@ -48,11 +55,13 @@ def parse():
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--group", type=int, default=64)
     ap.add_argument("--precision", choices=("fp32", "mixed"), default="fp32")
-    ap.add_argument("--quad", action="store_true", help="add cell quadrupole terms")
-    ap.add_argument("--textbook", action="store_true",
-                    help="opening test w/d <= theta instead of the reference's w^2/theta <= d^2")
-    ap.add_argument("--bmax", action="store_true",
-                    help="opening test bmax/d <= theta with a per-cell bound bmax")
+    ap.add_argument("--multipole", choices=("quad", "mono"), default="quad",
+                    help="cell terms: monopole+quadrupole (default) or monopole only")
+    ap.add_argument("--opening", choices=("textbook", "reference", "bmax"), default="textbook",
+                    help="opening test: textbook w/d <= theta (default), the reference's "
+                         "w^2/theta <= d^2+eps^2, or bmax/d <= theta with a per-cell bound")
+    ap.add_argument("--no-reference-opening", action="store_true",
+                    help="skip the side run with the reference's criterion")
     ap.add_argument("--error-sample", type=int, default=8192)
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3, help="timed oracle steps after 1 warm-up (0: skip)")
@@ -78,13 +87,16 @@ def rms_rel(ax, ay, rx, ry):
 def cpu_baseline(st, args, eps2):
     """The reference algorithm (oracle restatement: sequential insertion build,
     FP32 centre of mass, DFS sort, group-of-32 vote walk, kick-drift) on the
-    host cores, same workload; median per-step time after one warm-up."""
+    host cores, same workload; median per-step time after one warm-up.  Each
+    timed step starts from the initial state: within a few steps two bodies of
+    this workload land on the same FP32 position, which the reference's
+    insertion build cannot separate (it splits until its pool overflows)."""
     from oracle import oracle as O
 
-    state = {k: st[k].copy() for k in ("x", "y", "vx", "vy", "mass")}
     threads = args.cpu_threads if args.cpu_threads > 0 else min(16, os.cpu_count() or 1)
     times = []
     for k in range(args.cpu_steps + 1):
+        state = {q: st[q].copy() for q in ("x", "y", "vx", "vy", "mass")}
         t0 = time.perf_counter()
         O.step(state, args.theta, eps2, args.dt, 1.0, group=32, threads=threads)
         t1 = time.perf_counter()
@@ -97,9 +109,67 @@ def cpu_baseline(st, args, eps2):
         "unit": "body-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"N={n} colliding disks, {args.cpu_steps} steps after 1 warm-up, median "
-                  f"{med * 1e3:.1f} ms/step, group 32, OpenMP walk, sequential build",
+        "sample": f"N={n} colliding disks, {args.cpu_steps} steps from the initial state after "
+                  f"1 warm-up, median {med * 1e3:.1f} ms/step, reference criterion, monopoles, "
+                  "group 32, OpenMP walk, sequential insertion build",
     }
+
+
+def model_flags(S, args):
+    f = S.FLAG_QUADRUPOLE if args.multipole == "quad" else 0
+    if args.opening == "textbook":
+        f |= S.FLAG_TEXTBOOK_OPENING
+    elif args.opening == "bmax":
+        f |= S.FLAG_BMAX_OPENING
+    return f
+
+
+def multipoles(args):
+    return "monopole+quadrupole" if args.multipole == "quad" else "monopole"
+
+
+def opening_text(args):
+    return {"textbook": "textbook (w/d <= theta, unsoftened d",
+            "reference": "reference (w^2/theta <= d^2+eps^2",
+            "bmax": "bmax (bmax/d <= theta"}[args.opening] + ", group vote)"
+
+
+def force_error(eng, n, sample):
+    """One untimed force pass on the current state against the FP64 direct
+    sum on a seeded subsample (rms, p99, max relative error)."""
+    eng.bbox()
+    eng.build()
+    eng.com()
+    eng.sort()
+    eng.forces()
+    f = eng.export_forces()
+    rng = np.random.default_rng(2)
+    tg = np.sort(rng.choice(n, min(n, sample), replace=False)).astype(np.int32)
+    dx, dy = eng.direct_fp64(tg)
+    return rms_rel(f["ax"][tg], f["ay"][tg], dx, dy), int(tg.size)
+
+
+def reference_opening_run(S, args, st, eps2):
+    """The reference's force model (monopoles, w^2/theta <= d^2+eps^2, walk
+    group 64) on the same workload: speed and error."""
+    n = st["x"].size
+    p = S.default_params(theta=args.theta, eps2=eps2, timestep=args.dt, group=args.group,
+                         flags=S.FLAG_STAGE_TIMING)
+    with S.Engine(n, p) as e:
+        e.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
+        e.run(args.warmup)
+        e.sync()
+        k = max(2, min(args.steps, 20))
+        t0 = time.perf_counter()
+        e.run(k)
+        t1 = time.perf_counter()
+        stages = e.stage_times()
+        (err, p99, emax), ns = force_error(e, n, args.error_sample)
+    return {"value": n * k / (t1 - t0), "unit": "body-steps/s", "steps": k,
+            "ms_per_step": (t1 - t0) / k * 1e3, "rms_rel_force_err": err,
+            "force_err_p99": p99, "force_err_max": emax, "stage_ms": stages,
+            "opening": "reference (w^2/theta <= d^2+eps^2, group vote)",
+            "multipoles": "monopole"}
 
 
 def ensemble_members(S, ninst, seed=3):
@@ -122,8 +192,7 @@ def run_ensemble(args, S):
     counts, sts = ensemble_members(S, args.instances)
     ntot = int(sum(counts))
     cat = {k: np.concatenate([s[k] for s in sts]) for k in ("mass", "x", "y", "vx", "vy")}
-    base_flags = ((S.FLAG_QUADRUPOLE if args.quad else 0) | (S.FLAG_TEXTBOOK_OPENING if args.textbook else 0)
-                  | (S.FLAG_BMAX_OPENING if args.bmax else 0))
+    base_flags = model_flags(S, args)
 
     # batched form: one context, graph-captured steps, autotuned walk shape
     p = S.default_params(theta=args.theta, eps2=eps2, timestep=args.dt, group=args.group,
@@ -134,6 +203,7 @@ def run_ensemble(args, S):
     eng.run(2)  # eager step timing for the stage split
     stages = eng.stage_times()
     inter = eng.last_interactions()
+    qinter = eng.last_quad_interactions()
     eng.set_params(flags=base_flags | S.FLAG_GRAPH)
     eng.run(args.warmup + (args.warmup % 2))
     eng.sync()
@@ -175,7 +245,7 @@ def run_ensemble(args, S):
         e.close()
 
     walk_ms = stages["forces"]
-    achieved = inter * OPS_PER_INTERACTION / (walk_ms * 1e-3) / 1e12
+    achieved = (inter * OPS_PER_INTERACTION + qinter * OPS_PER_QUAD_TERM) / (walk_ms * 1e-3) / 1e12
     peak = VALU_PEAK_LANE_OPS / 1e12
     out = {
         "metric": "BH body-steps/s, batched ensemble (config 3 workload)",
@@ -194,7 +264,7 @@ def run_ensemble(args, S):
                                "[1024, 65536] (x64), disk/Plummer/colliding disks",
                    "n_bodies": ntot, "theta": args.theta, "eps": args.eps, "dt": args.dt,
                    "precision": args.precision, "graph": True, "tuning": tuning,
-                   "multipoles": "monopole+quadrupole" if args.quad else "monopole"},
+                   "multipoles": multipoles(args), "opening": opening_text(args)},
         "plain_form_body_steps_per_s": plain,
         "plain_form": f"{len(counts)} single-instance contexts run one after another, "
                       f"{plain_steps} steps each, no graph",
@@ -231,9 +301,7 @@ def main():
     prec = S.PRECISION_MIXED if args.precision == "mixed" else S.PRECISION_FP32
     p = S.default_params(theta=args.theta, eps2=eps2, timestep=args.dt, dampening=1.0,
                          group=args.group, precision=prec,
-                         flags=S.FLAG_STAGE_TIMING | (S.FLAG_QUADRUPOLE if args.quad else 0)
-                         | (S.FLAG_TEXTBOOK_OPENING if args.textbook else 0)
-                         | (S.FLAG_BMAX_OPENING if args.bmax else 0)
+                         flags=S.FLAG_STAGE_TIMING | model_flags(S, args)
                          | (S.FLAG_GRAPH if args.graph else 0))
     eng = S.Engine(n, p)
     eng.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
@@ -253,21 +321,13 @@ def main():
     if not args.graph:
         stages = eng.stage_times()  # mean device ms per step, HIP events on the engine stream
     inter = eng.last_interactions()
+    qinter = eng.last_quad_interactions()
     ncells = eng.num_cells()
     wall = t1 - t0
     ms_per_step = wall / args.steps * 1e3
 
     # accuracy of the walk on the final state: one more untimed force pass
-    eng.bbox()
-    eng.build()
-    eng.com()
-    eng.sort()
-    eng.forces()
-    f = eng.export_forces()
-    rng = np.random.default_rng(2)
-    tg = np.sort(rng.choice(n, min(n, args.error_sample), replace=False)).astype(np.int32)
-    dx, dy = eng.direct_fp64(tg)
-    err_rms, err_p99, err_max = rms_rel(f["ax"][tg], f["ay"][tg], dx, dy)
+    (err_rms, err_p99, err_max), nsample = force_error(eng, n, args.error_sample)
     drift = None
     if e0 is not None:
         # v is a half step ahead of x (kick-drift); the drift figure uses the
@@ -283,7 +343,7 @@ def main():
     eng.close()
 
     walk_ms = stages["forces"]
-    achieved = inter * OPS_PER_INTERACTION / (walk_ms * 1e-3) / 1e12
+    achieved = (inter * OPS_PER_INTERACTION + qinter * OPS_PER_QUAD_TERM) / (walk_ms * 1e-3) / 1e12
     peak = VALU_PEAK_LANE_OPS / 1e12
     out = {
         "metric": "BH body-steps/s at N=1M, theta=0.5 + rms rel force err vs FP64 direct sum",
@@ -302,21 +362,18 @@ def main():
                    "n_bodies": n, "theta": args.theta, "eps": args.eps, "dt": args.dt,
                    "group": args.group, "precision": args.precision, "seed": 2,
                    "graph": bool(args.graph),
-                   "opening": ("bmax (bmax/d <= theta" if args.bmax else
-                               "textbook (w/d <= theta" if args.textbook else
-                               "reference (w^2/theta <= d^2+eps^2") + ", group vote)",
-                   "multipoles": "monopole+quadrupole" if args.quad else "monopole"},
+                   "opening": opening_text(args), "multipoles": multipoles(args)},
         "rms_rel_force_err": err_rms,
         "force_err_p99": err_p99,
         "force_err_max": err_max,
-        "force_err_sample": int(tg.size),
+        "force_err_sample": nsample,
         "stage_ms": stages,
         "cells": ncells,
         "interactions_per_step": inter,
         "interactions_per_body": inter / n,
         "roofline": {
             "bound": "valu",
-            "kernel": "walk_kernel<64>",
+            "kernel": "walk64_kernel",
             "achieved": achieved,
             "peak": peak,
             "unit": "Tlane-op/s",
@@ -324,9 +381,13 @@ def main():
             "traffic": None,
             "interactions_per_s": inter / (walk_ms * 1e-3),
             "ops_per_interaction": OPS_PER_INTERACTION,
+            "ops_per_quadrupole_term": OPS_PER_QUAD_TERM,
+            "quadrupole_interactions": qinter,
         },
         "drift": drift,
     }
+    if not args.no_reference_opening and args.opening != "reference":
+        out["reference_opening"] = reference_opening_run(S, args, st, eps2)
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(st, args, eps2)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

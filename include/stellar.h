@@ -182,8 +182,10 @@ int stellar_export_tree_instance(stellar_ctx* ctx, int32_t inst, int32_t* root4,
 int stellar_export_forces(stellar_ctx* ctx, float* ax, float* ay,
                           int32_t* interactions);
 
-/* Sum of interactions of the last walk (exact, 64-bit). */
+/* Sum of interactions of the last walk (exact, 64-bit), and how many of
+ * them added a cell's quadrupole term. */
 int stellar_last_interactions(stellar_ctx* ctx, int64_t* total);
+int stellar_last_quad_interactions(stellar_ctx* ctx, int64_t* total);
 
 /* Per-stage device milliseconds of the last step when
  * STELLAR_FLAG_STAGE_TIMING is set: [bbox, build, com, sort, forces,

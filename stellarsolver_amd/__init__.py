@@ -124,6 +124,7 @@ SIGNATURES = {
     "stellar_export_tree_instance": (C.c_int, [_P, C.c_int32, _I, _I, _I, _I, _I, _I, _F, _F, _F, C.c_int64, _I64]),
     "stellar_export_forces": (C.c_int, [_P, _F, _F, _I]),
     "stellar_last_interactions": (C.c_int, [_P, _I64]),
+    "stellar_last_quad_interactions": (C.c_int, [_P, _I64]),
     "stellar_stage_times": (C.c_int, [_P, _F]),
     "stellar_last_error": (C.c_char_p, [_P]),
     "stellar_status_string": (C.c_char_p, [C.c_int]),
@@ -370,6 +371,11 @@ class Engine:
     def last_interactions(self) -> int:
         v = C.c_int64()
         self._check(self._lib.stellar_last_interactions(self._h, C.byref(v)), "last_interactions")
+        return v.value
+
+    def last_quad_interactions(self) -> int:
+        v = C.c_int64()
+        self._check(self._lib.stellar_last_quad_interactions(self._h, C.byref(v)), "last_quad_interactions")
         return v.value
 
     def stage_times(self) -> dict[str, float]:

@@ -22,7 +22,10 @@ __global__ __launch_bounds__(256) void permute_kernel(
     float* __restrict__ m2, int32_t* __restrict__ id2,
     DevScalars* __restrict__ dev) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) dev->interactions = 0;  // for the walk that follows
+  if (s == 0) {  // for the walk that follows
+    dev->interactions = 0;
+    dev->quad_interactions = 0;
+  }
   if (s >= n) return;
   const uint32_t p = perm[s];
   x2[s] = x[p];
@@ -101,7 +104,10 @@ __global__ __launch_bounds__(256) void output_kernel(
   }
 }
 
-__global__ void zero_counter_kernel(DevScalars* dev) { dev->interactions = 0; }
+__global__ void zero_counter_kernel(DevScalars* dev) {
+  dev->interactions = 0;
+  dev->quad_interactions = 0;
+}
 
 // Positions scattered to body-id order (for diagnostics by id).
 __global__ __launch_bounds__(256) void by_id_kernel(

@@ -114,6 +114,7 @@ __global__ void init_scalars_kernel(DevScalars* dev) {
   dev->ncells = 0;
   dev->bbox[0] = dev->bbox[1] = dev->bbox[2] = dev->bbox[3] = 0.f;
   dev->interactions = 0;
+  dev->quad_interactions = 0;
 }
 
 // One level of box subdivision along one axis (kernels.cu:144-157): the

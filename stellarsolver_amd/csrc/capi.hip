@@ -646,6 +646,14 @@ int stellar_num_cells(stellar_ctx* c, int64_t* ncells) {
   return STELLAR_OK;
 }
 
+int stellar_last_quad_interactions(stellar_ctx* c, int64_t* total) {
+  if (!c || !total) return STELLAR_ERR_ARG;
+  int rc = check_device(c);
+  if (rc) return rc;
+  *total = (int64_t)c->b.host_dev->quad_interactions;
+  return STELLAR_OK;
+}
+
 int stellar_last_interactions(stellar_ctx* c, int64_t* total) {
   if (!c || !total) return STELLAR_ERR_ARG;
   int rc = check_device(c);

@@ -48,9 +48,11 @@ struct DevScalars {
   int32_t ncells;        // cells of the current tree (all instances)
   int32_t pad1;
   float bbox[4];         // unused (per-instance boxes live in Buffers::ibbox)
-  unsigned long long interactions;
-  uint32_t pad[46];
+  unsigned long long interactions;       // body-node interactions of the last walk
+  unsigned long long quad_interactions;  // of which with a cell's quadrupole term
+  uint32_t pad[52];
 };
+static_assert(sizeof(DevScalars) == 256, "one 256-byte line");
 
 // Positions per bounding-box tile (one workgroup; tiles never straddle two
 // instances).

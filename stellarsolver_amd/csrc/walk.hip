@@ -74,7 +74,7 @@ __device__ __forceinline__ float root_depth(const WalkArgs& A, int32_t inst) {
 
 struct Acc {
   float fx = 0.f, fy = 0.f;
-  int32_t cnt = 0;
+  int32_t cnt = 0, qcnt = 0;
   __device__ __forceinline__ void mono(float cm, float r, float dx, float dy) {
     const float rs = __builtin_amdgcn_rsqf(r);
     const float f = cm * rs * rs * rs;
@@ -96,6 +96,7 @@ struct Acc {
     fx = __builtin_fmaf(-rs5, qdx, __builtin_fmaf(g, dx, fx));
     fy = __builtin_fmaf(-rs5, qdy, __builtin_fmaf(g, dy, fy));
     ++cnt;
+    ++qcnt;
   }
 };
 
@@ -226,8 +227,12 @@ __global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
     A.ay[s] = acc.fy;
     A.inter[s] = acc.cnt;
   }
-  if (lane == 0)
+  if (lane == 0) {
     atomicAdd(&A.dev->interactions, (unsigned long long)acc.cnt * (unsigned long long)cnt);
+    if (QUAD)
+      atomicAdd(&A.dev->quad_interactions,
+                (unsigned long long)acc.qcnt * (unsigned long long)cnt);
+  }
 }
 
 // ----------------------------------------------------------- G = 32, 16 --
@@ -336,8 +341,12 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
     A.ay[s] = acc.fy;
     A.inter[s] = acc.cnt;
   }
-  if (glane == 0 && gvalid)
+  if (glane == 0 && gvalid) {
     atomicAdd(&A.dev->interactions, (unsigned long long)acc.cnt * (unsigned long long)cnt);
+    if (QUAD)
+      atomicAdd(&A.dev->quad_interactions,
+                (unsigned long long)acc.qcnt * (unsigned long long)cnt);
+  }
 }
 
 template <bool QUAD, bool THR, int KPOP>
