@@ -54,7 +54,6 @@ def parse():
     ap.add_argument("--eps", type=float, default=0.01)
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--group", type=int, default=64)
-    ap.add_argument("--precision", choices=("fp32", "mixed"), default="fp32")
     ap.add_argument("--multipole", choices=("quad", "mono"), default="quad",
                     help="cell terms: monopole+quadrupole (default) or monopole only")
     ap.add_argument("--opening", choices=("textbook", "reference", "bmax"), default="textbook",
@@ -188,7 +187,7 @@ def ensemble_members(S, ninst, seed=3):
 
 def run_ensemble(args, S):
     eps2 = float(np.float32(args.eps * args.eps))
-    prec = S.PRECISION_MIXED if args.precision == "mixed" else S.PRECISION_FP32
+    prec = S.PRECISION_FP32
     counts, sts = ensemble_members(S, args.instances)
     ntot = int(sum(counts))
     cat = {k: np.concatenate([s[k] for s in sts]) for k in ("mass", "x", "y", "vx", "vy")}
@@ -263,7 +262,7 @@ def run_ensemble(args, S):
         "config": {"workload": f"ensemble of {len(counts)} instances, N_i log-uniform "
                                "[1024, 65536] (x64), disk/Plummer/colliding disks",
                    "n_bodies": ntot, "theta": args.theta, "eps": args.eps, "dt": args.dt,
-                   "precision": args.precision, "graph": True, "tuning": tuning,
+                   "precision": "fp32", "graph": True, "tuning": tuning,
                    "multipoles": multipoles(args), "opening": opening_text(args)},
         "plain_form_body_steps_per_s": plain,
         "plain_form": f"{len(counts)} single-instance contexts run one after another, "
@@ -298,7 +297,7 @@ def main():
     n = args.n
     eps2 = float(np.float32(args.eps * args.eps))
     st = S.generate_ic(S.ic_params(args.workload), n)
-    prec = S.PRECISION_MIXED if args.precision == "mixed" else S.PRECISION_FP32
+    prec = S.PRECISION_FP32
     p = S.default_params(theta=args.theta, eps2=eps2, timestep=args.dt, dampening=1.0,
                          group=args.group, precision=prec,
                          flags=S.FLAG_STAGE_TIMING | model_flags(S, args)
@@ -360,7 +359,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "colliding disks" if args.workload == 2 else f"workload {args.workload}",
                    "n_bodies": n, "theta": args.theta, "eps": args.eps, "dt": args.dt,
-                   "group": args.group, "precision": args.precision, "seed": 2,
+                   "group": args.group, "precision": "fp32", "seed": 2,
                    "graph": bool(args.graph),
                    "opening": opening_text(args), "multipoles": multipoles(args)},
         "rms_rel_force_err": err_rms,

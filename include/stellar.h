@@ -44,9 +44,13 @@ extern "C" {
  * level-64 cell by rank instead of splitting forever. */
 #define STELLAR_MAX_LEVEL 64
 
-/* Force-path precision. */
-#define STELLAR_PRECISION_FP32 0  /* plain FP32 walk                         */
-#define STELLAR_PRECISION_MIXED 1 /* FP16-packed far field, FP32 near field  */
+/* Force-path precision.  Only FP32 is implemented: the walk issues one
+ * wavefront-wide instruction per (group, node) pair, so packed FP16 would
+ * need FP32 differences, conversions and FP32 accumulation around a
+ * non-packed rsqrt and saves no issue slots; MIXED is rejected with
+ * STELLAR_ERR_ARG.  Moments are summed in FP64 in every mode. */
+#define STELLAR_PRECISION_FP32 0  /* FP32 walk and update                    */
+#define STELLAR_PRECISION_MIXED 1 /* reserved: packed-FP16 far field         */
 
 typedef struct stellar_params {
   /* Opening test: a cell of x-width w is accepted when every body of the

@@ -56,9 +56,7 @@ bool params_ok(const stellar_params* p) {
   if (!p) return false;
   if (!(p->theta > 0.f) || !(p->eps2 > 0.f)) return false;
   if (p->group != 64 && p->group != 32 && p->group != 16) return false;
-  if (p->precision != STELLAR_PRECISION_FP32 &&
-      p->precision != STELLAR_PRECISION_MIXED)
-    return false;
+  if (p->precision != STELLAR_PRECISION_FP32) return false;  // MIXED: not implemented
   return true;
 }
 

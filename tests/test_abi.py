@@ -122,6 +122,14 @@ def test_bad_arguments_rejected_without_gpu(stellar):
     assert lib.stellar_create(C.byref(p), 10, 0, C.byref(h)) == stellar.STELLAR_ERR_ARG
     p.eps2 = 0.01
     assert lib.stellar_create(C.byref(p), -1, 0, C.byref(h)) == stellar.STELLAR_ERR_ARG
+    p.precision = stellar.PRECISION_MIXED  # reserved, not implemented
+    assert lib.stellar_create(C.byref(p), 10, 0, C.byref(h)) == stellar.STELLAR_ERR_ARG
+    p.precision = stellar.PRECISION_FP32
+    counts = (C.c_int32 * 3)(10, 0, 5)  # an empty instance
+    assert lib.stellar_batch_create(C.byref(p), 3, counts, 0, C.byref(h)) == stellar.STELLAR_ERR_ARG
+    assert lib.stellar_batch_create(C.byref(p), 0, counts, 0, C.byref(h)) == stellar.STELLAR_ERR_ARG
+    tune = stellar.StellarTuning()
+    assert lib.stellar_autotune(None, 0, 1, C.byref(tune)) == stellar.STELLAR_ERR_ARG
     bad = stellar.ic_params(None, model=77)
     with pytest.raises(stellar.StellarError):
         stellar.generate_ic(bad, 10)
