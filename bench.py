@@ -108,10 +108,14 @@ def cpu_baseline(st, args, eps2):
         "unit": "body-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"N={n} colliding disks, {args.cpu_steps} steps from the initial state after "
+        "sample": f"N={n} {WORKLOAD_NAMES.get(args.workload, args.workload)}, {args.cpu_steps} "
+                  f"steps from the initial state after "
                   f"1 warm-up, median {med * 1e3:.1f} ms/step, reference criterion, monopoles, "
                   "group 32, OpenMP walk, sequential insertion build",
     }
+
+
+WORKLOAD_NAMES = {0: "disk", 1: "Plummer a=1", 2: "colliding disks", 4: "deep Plummer a=0.01"}
 
 
 def model_flags(S, args):
@@ -357,9 +361,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "colliding disks" if args.workload == 2 else f"workload {args.workload}",
+        "config": {"workload": WORKLOAD_NAMES.get(args.workload, f"workload {args.workload}"),
                    "n_bodies": n, "theta": args.theta, "eps": args.eps, "dt": args.dt,
-                   "group": args.group, "precision": "fp32", "seed": 2,
+                   "group": args.group, "precision": "fp32", "seed": args.workload,
                    "graph": bool(args.graph),
                    "opening": opening_text(args), "multipoles": multipoles(args)},
         "rms_rel_force_err": err_rms,

@@ -57,7 +57,8 @@ def gpu_tree(e):
     return bb, t
 
 
-@pytest.mark.parametrize("workload,n", [(0, 4096), (1, 65536), (2, 65536), (1, 262144)])
+@pytest.mark.parametrize("workload,n", [(0, 4096), (1, 65536), (2, 65536), (1, 262144),
+                                        (4, 1 << 20)])
 def test_tree_bit_exact(S, oracle, workload, n):
     print(f"tree parity workload={workload} n={n}", flush=True)
     st = ic(workload, n)
@@ -314,3 +315,29 @@ def test_bmax_opening_parity(S, oracle, workload, n, group, quad):
     ax, ay, it = oracle.walk(ot, x, y, m, obb, theta, eps2, group, quad=quad, bmax=True)
     assert np.array_equal(f["inter"], it)
     assert rms_rel(f["ax"], f["ay"], ax, ay) < FORCE_RMS_TOL
+
+
+def test_deep_plummer_tree_and_forces(S, oracle):
+    """Clustered core (Plummer a = 0.01, eps = 1e-4, the deep-tree workload at
+    2^18 bodies, about 20 levels): the tree is bit-exact and the
+    quadrupole/textbook walk agrees with the oracle's interaction set."""
+    n = 1 << 18
+    st = ic(4, n)
+    eps2 = float(np.float32(1e-4 ** 2))
+    flags = S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING
+    with make_engine(S, st, eps2, flags=flags) as e:
+        gbb, gt = gpu_tree(e)
+        e.sort()
+        e.forces()
+        f = e.export_forces()
+    obb, ot = oracle_tree(oracle, st)
+    assert tree_equal(gt, ot), first_tree_difference(gt, ot)
+    print(f"deep plummer: max level {int(gt['level'].max())}, cells {gt['ncells']}", flush=True)
+    assert gt["level"].max() >= 20
+    oracle.quad_exact(ot, st["x"], st["y"], st["mass"])
+    ax, ay, it = oracle.walk(ot, st["x"], st["y"], st["mass"], obb, 0.5, eps2, 64,
+                             opening=1, quad=True)
+    assert np.array_equal(f["inter"], it)
+    # ~1700 FP32 terms per body whose pulls largely cancel in the dense core:
+    # summation order and rsqrt rounding leave a few parts in 1e6
+    assert rms_rel(f["ax"], f["ay"], ax, ay) < 1e-5
