@@ -214,6 +214,11 @@ typedef struct stellar_tuning {
  * stellar_sort done).  out may be NULL. */
 int stellar_autotune(stellar_ctx* ctx, int32_t group_mask, int32_t reps,
                      stellar_tuning* out);
+/* Set the group-64 walk launch shape by hand: nodes_per_pop 1, 2 or 4
+ * (4 is capped to 2 when a node's second block is read), block_threads 64,
+ * 128, 256 or 512. */
+int stellar_set_walk_shape(stellar_ctx* ctx, int32_t nodes_per_pop,
+                           int32_t block_threads);
 
 /* Message for the last non-OK status on this context. */
 const char* stellar_last_error(const stellar_ctx* ctx);

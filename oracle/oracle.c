@@ -461,25 +461,27 @@ static inline float dist2(float dx, float dy, float eps2) {
   return fmaf(dx, dx, fmaf(dy, dy, eps2));
 }
 
-/* FP32 accumulation of one accepted node for one body. */
+/* FP32 accumulation of one accepted node for one body.  With q (a cell's
+ * quadrupole) the three terms m d/r^3 - Q.d/r^5 + (5/2)(d.Q.d) d/r^7 are
+ * evaluated as rs^5 [ (m r + 2.5 (d.Q.d) rs^2) d - Q.d ], r = |d|^2 + eps2,
+ * rs = 1/sqrt(r): the form the device walk uses. */
 static inline void acc_node(float* fx, float* fy, float dx, float dy, float r,
                             float cmv, const float* q) {
   const float rs = rsqrt_ref(r);
-  const float f = cmv * rs * rs * rs;
   if (!q) {
+    const float f = cmv * rs * rs * rs;
     *fx = fmaf(f, dx, *fx);
     *fy = fmaf(f, dy, *fy);
     return;
   }
-  const float rs2 = rs * rs;
-  const float rs5 = rs2 * rs2 * rs;
-  const float rs7 = rs5 * rs2;
-  const float qdx = q[0] * dx + q[1] * dy;
-  const float qdy = q[1] * dx + q[2] * dy;
-  const float dqd = dx * qdx + dy * qdy;
-  const float g = f + 2.5f * dqd * rs7;
-  *fx = fmaf(-rs5, qdx, fmaf(g, dx, *fx));
-  *fy = fmaf(-rs5, qdy, fmaf(g, dy, *fy));
+  const float u = rs * rs;
+  const float rs5 = (u * u) * rs;
+  const float px = fmaf(q[0], dx, q[1] * dy);
+  const float py = fmaf(q[1], dx, q[2] * dy);
+  const float s = fmaf(dx, px, dy * py);
+  const float c = fmaf(2.5f * s, u, cmv * r);
+  *fx = fmaf(rs5, fmaf(c, dx, -px), *fx);
+  *fy = fmaf(rs5, fmaf(c, dy, -py), *fy);
 }
 
 int or_walk(int32_t n, const float* x, const float* y, const float* mass,

@@ -132,6 +132,7 @@ SIGNATURES = {
     "stellar_energy_fp64": (C.c_int, [_P, C.c_double, _D, _D, _D, _D]),
     "stellar_energy_fp64_instances": (C.c_int, [_P, C.c_double, _D, _D, _D, _D]),
     "stellar_autotune": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(StellarTuning)]),
+    "stellar_set_walk_shape": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "stellar_ic_defaults": (C.c_int, [C.c_int32, C.POINTER(StellarICParams)]),
     "stellar_generate_ic": (C.c_int, [C.POINTER(StellarICParams), C.c_int32, _F, _F, _F, _F, _F]),
 }
@@ -409,6 +410,10 @@ class Engine:
         self.params.group = t.group
         return {"group": t.group, "nodes_per_pop": t.nodes_per_pop,
                 "block_threads": t.block_threads, "walk_ms": t.walk_ms}
+
+    def set_walk_shape(self, nodes_per_pop: int, block_threads: int) -> None:
+        self._check(self._lib.stellar_set_walk_shape(self._h, int(nodes_per_pop), int(block_threads)),
+                    "set_walk_shape")
 
     def energy_fp64_instances(self, eps2: float | None = None) -> dict[str, np.ndarray]:
         e2 = float(self.params.eps2) if eps2 is None else float(eps2)

@@ -905,6 +905,18 @@ int stellar_energy_fp64(stellar_ctx* c, double eps2, double* kinetic,
   return STELLAR_OK;
 }
 
+int stellar_set_walk_shape(stellar_ctx* c, int32_t nodes_per_pop, int32_t block_threads) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (nodes_per_pop != 1 && nodes_per_pop != 2 && nodes_per_pop != 4)
+    return fail(c, STELLAR_ERR_ARG, "nodes_per_pop");
+  if (block_threads != 64 && block_threads != 128 && block_threads != 256 && block_threads != 512)
+    return fail(c, STELLAR_ERR_ARG, "block_threads");
+  c->walk_kpop = nodes_per_pop;
+  c->walk_waves = block_threads / 64;
+  drop_graphs(c);
+  return STELLAR_OK;
+}
+
 int stellar_autotune(stellar_ctx* c, int32_t group_mask, int32_t reps, stellar_tuning* out) {
   if (!c || reps < 1 || group_mask < 0 || group_mask > 7) return STELLAR_ERR_ARG;
   if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");

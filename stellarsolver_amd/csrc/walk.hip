@@ -82,19 +82,21 @@ struct Acc {
     fy = __builtin_fmaf(f, dy, fy);
     ++cnt;
   }
+  // Monopole plus quadrupole of an accepted cell:
+  //   a = m d/r^3 - Q.d/r^5 + (5/2)(d.Q.d) d/r^7
+  //     = rs^5 [ (m r^2 + 2.5 (d.Q.d) rs^2) d - Q.d ],  rs = 1/r, r^2 = |d|^2+eps2
+  // (20 issue slots with the rsqrt instead of 27 for the three terms apart).
   __device__ __forceinline__ void quad(float cm, float r, float dx, float dy,
                                        float4 q) {
     const float rs = __builtin_amdgcn_rsqf(r);
-    const float f = cm * rs * rs * rs;
-    const float rs2 = rs * rs;
-    const float rs5 = rs2 * rs2 * rs;
-    const float rs7 = rs5 * rs2;
-    const float qdx = q.x * dx + q.y * dy;
-    const float qdy = q.y * dx + q.z * dy;
-    const float dqd = dx * qdx + dy * qdy;
-    const float g = f + 2.5f * dqd * rs7;
-    fx = __builtin_fmaf(-rs5, qdx, __builtin_fmaf(g, dx, fx));
-    fy = __builtin_fmaf(-rs5, qdy, __builtin_fmaf(g, dy, fy));
+    const float u = rs * rs;
+    const float rs5 = (u * u) * rs;
+    const float px = __builtin_fmaf(q.x, dx, q.y * dy);  // Q.d
+    const float py = __builtin_fmaf(q.y, dx, q.z * dy);
+    const float s = __builtin_fmaf(dx, px, dy * py);      // d.Q.d
+    const float c = __builtin_fmaf(2.5f * s, u, cm * r);
+    fx = __builtin_fmaf(rs5, __builtin_fmaf(c, dx, -px), fx);
+    fy = __builtin_fmaf(rs5, __builtin_fmaf(c, dy, -py), fy);
     ++cnt;
     ++qcnt;
   }
@@ -188,6 +190,17 @@ __global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
 #pragma unroll
         for (int k = 0; k < 4; ++k) qm[kQ ? j : 0][k] = qblk[4 * (int64_t)node[j] + k];
       }
+    }
+    if (kQ) {
+      // keep the second blocks' loads here, batched with the first blocks,
+      // instead of letting them sink to each use (one wait per child)
+#pragma unroll
+      for (int j = 0; j < KPOP; ++j)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const float4 v = qm[kQ ? j : 0][k];
+          asm volatile("" ::"s"(v.x), "s"(v.y), "s"(v.z), "s"(v.w));
+        }
     }
 #pragma unroll
     for (int jj = KPOP - 1; jj >= 0; --jj) {
@@ -351,7 +364,9 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
 
 template <bool QUAD, bool THR, int KPOP>
 void launch64(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
-  const int w = c->walk_waves;
+  // default shape (measured on MI355X, N = 2^20): 8 wavefronts per workgroup
+  // for one-block nodes, 4 when the second block is read
+  const int w = c->walk_waves > 0 ? c->walk_waves : ((QUAD || THR) ? 4 : 8);
   const unsigned grid = (unsigned)((ngroups + w - 1) / w);
   switch (w) {
     case 1:
@@ -369,12 +384,13 @@ void launch64(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
   }
 }
 
-// Nodes per pop: 4 when a node is one 64-byte block, at most 2 when it also
-// brings its second block (quadrupoles or per-cell thresholds), to bound
-// scalar register use.
+// Nodes per pop: default 2 when a node is one 64-byte block, 1 when it also
+// brings its second block (quadrupoles or per-cell thresholds: two nodes'
+// four blocks exceed the scalar registers and the loads serialise); at most
+// 2 in that case.
 template <bool QUAD, bool THR>
 void launch64_kpop(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
-  const int kp = c->walk_kpop;
+  const int kp = c->walk_kpop > 0 ? c->walk_kpop : ((QUAD || THR) ? 1 : 2);
   if (kp == 1)
     launch64<QUAD, THR, 1>(c, ngroups, A);
   else if (kp == 2 || QUAD || THR)
