@@ -275,7 +275,10 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
     const int32_t b = live ? L.ids[i] : 0;
     bool left = false;
     if (live) {
-      L.pidx[i] = find_id(L.ids, 0, i, L.pblk[i]);
+      // a contiguous chunk (the first pass) maps block ids to indices directly
+      const int32_t base = p0 + A.nroot;
+      const int32_t pr = L.pblk[i] - base;
+      L.pidx[i] = list ? find_id(L.ids, 0, i, L.pblk[i]) : (pr >= 0 && pr < i ? pr : -1);
       int pend = 0;
       if (L.local[i]) {
 #pragma unroll
@@ -283,7 +286,8 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
           const int32_t code = L.rec[i][k].code;
           int ci = -1;
           if (code > 0) {
-            ci = find_id(L.ids, i + 1, ne, code);
+            const int32_t cr = code - base;
+            ci = list ? find_id(L.ids, i + 1, ne, code) : (cr > i && cr < ne ? cr : -1);
             if (ci >= 0) {
               ++pend;  // finished in this chunk
             } else if (EXT) {
