@@ -149,7 +149,7 @@ struct stellar_ctx {
   // walk launch shape (group 64): nodes popped per memory round trip and
   // wavefronts per workgroup (stellar_autotune)
   int walk_kpop = 0;   // 0: default (2 one-block nodes, 1 two-block nodes)
-  int walk_waves = 0;  // 0: default (8 one-block nodes, 4 two-block nodes)
+  int walk_waves = 0;  // 0: default (8)
   // per-step stage events of the current stellar_run (8 per step)
   std::vector<hipEvent_t> evrun;
   int32_t evrun_steps = 0;

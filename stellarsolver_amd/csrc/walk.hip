@@ -277,7 +277,7 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
   const float py = valid ? A.y[s] : 0.f;
   const int32_t inst = gvalid ? A.inst_of[s0] : 0;
   const float depth0 = root_depth(A, inst);
-  const uint64_t gmask = ((G == 64) ? ~0ull : ((1ull << G) - 1ull)) << (sub * G);
+  const uint64_t gmask = ((G >= 64) ? ~0ull : ((1ull << (G & 63)) - 1ull)) << (sub * G);
 
   Acc acc;
   int sp = 0;
@@ -365,8 +365,7 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
 template <bool QUAD, bool THR, int KPOP>
 void launch64(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
   // default shape (measured on MI355X, N = 2^20): 8 wavefronts per workgroup
-  // for one-block nodes, 4 when the second block is read
-  const int w = c->walk_waves > 0 ? c->walk_waves : ((QUAD || THR) ? 4 : 8);
+  const int w = c->walk_waves > 0 ? c->walk_waves : 8;
   const unsigned grid = (unsigned)((ngroups + w - 1) / w);
   switch (w) {
     case 1:
