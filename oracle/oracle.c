@@ -412,9 +412,11 @@ void or_com_exact(int32_t n, int64_t ncells, const int32_t* child,
     S[3 * c + 0] = sm;
     S[3 * c + 1] = sx;
     S[3 * c + 2] = sy;
+    /* centre of mass as the sums times the reciprocal mass */
+    const double inv = sm != 0.0 ? 1.0 / sm : 0.0;
     ccm[c] = (float)sm;
-    ccx[c] = sm != 0.0 ? (float)(sx / sm) : 0.f;
-    ccy[c] = sm != 0.0 ? (float)(sy / sm) : 0.f;
+    ccx[c] = (float)(sx * inv);
+    ccy[c] = (float)(sy * inv);
   }
   free(S);
 }
@@ -721,15 +723,17 @@ void or_quad_exact(int32_t n, int64_t ncells, const int32_t* child,
       } else {
         const double* q = S + 6 * (size_t)(ch - n);
         cm[k] = q[0];
-        cx[k] = q[0] != 0.0 ? q[1] / q[0] : 0.0;
-        cy[k] = q[0] != 0.0 ? q[2] / q[0] : 0.0;
+        const double ci = q[0] != 0.0 ? 1.0 / q[0] : 0.0;
+        cx[k] = q[1] * ci;
+        cy[k] = q[2] * ci;
         qq[k] = q + 3;
         sm += q[0];
         sx += q[1];
         sy += q[2];
       }
     }
-    const double ox = sm != 0.0 ? sx / sm : 0.0, oy = sm != 0.0 ? sy / sm : 0.0;
+    const double inv = sm != 0.0 ? 1.0 / sm : 0.0;
+    const double ox = sx * inv, oy = sy * inv;
     double qxx = 0.0, qxy = 0.0, qyy = 0.0;
     for (int k = 0; k < 4; ++k) {
       if (child[4 * c + k] < 0) continue;
@@ -783,15 +787,17 @@ void or_bmax_exact(int32_t n, int64_t ncells, const int32_t* child,
         sy += m * cy[k];
       } else {
         const double* q = S + 4 * (size_t)(ch - n);
-        cx[k] = q[0] != 0.0 ? q[1] / q[0] : 0.0;
-        cy[k] = q[0] != 0.0 ? q[2] / q[0] : 0.0;
+        const double ci = q[0] != 0.0 ? 1.0 / q[0] : 0.0;
+        cx[k] = q[1] * ci;
+        cy[k] = q[2] * ci;
         cb[k] = q[3];
         sm += q[0];
         sx += q[1];
         sy += q[2];
       }
     }
-    const double ox = sm != 0.0 ? sx / sm : 0.0, oy = sm != 0.0 ? sy / sm : 0.0;
+    const double inv = sm != 0.0 ? 1.0 / sm : 0.0;
+    const double ox = sx * inv, oy = sy * inv;
     double b = 0.0;
     for (int k = 0; k < 4; ++k) {
       if (child[4 * c + k] < 0) continue;

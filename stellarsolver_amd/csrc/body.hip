@@ -89,7 +89,8 @@ __global__ __launch_bounds__(256) void update_kernel(
 __global__ __launch_bounds__(256) void output_kernel(
     int32_t n, int64_t cap, int32_t nroot, const float* __restrict__ x,
     const float* __restrict__ y, const int32_t* __restrict__ id,
-    const float4* __restrict__ ccom, const DevScalars* __restrict__ dev,
+    const ChildRec* __restrict__ blk, const int32_t* __restrict__ cparent,
+    const uint8_t* __restrict__ cslot, const DevScalars* __restrict__ dev,
     float* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -99,7 +100,9 @@ __global__ __launch_bounds__(256) void output_kernel(
   }
   const int64_t c = i - n;
   if (c < dev->ncells && c < cap) {
-    const float4 q = ccom[c + nroot];
+    // a cell's centre of mass is its record in the parent's block
+    const int64_t b = c + nroot;
+    const ChildRec q = blk[4 * (int64_t)cparent[b] + (cslot[b] & 3)];
     reinterpret_cast<float2*>(out)[n + c] = make_float2(q.x, q.y);
   }
 }
@@ -279,7 +282,7 @@ void launch_output(stellar_ctx* c) {
   if (n == 0) return;
   Buffers& B = c->b;
   output_kernel<<<grid_for(n + c->cap, 256), 256, 0, c->stream>>>(
-      n, c->cap, c->ninst, B.x, B.y, B.id, B.ccom, B.dev, B.out);
+      n, c->cap, c->ninst, B.x, B.y, B.id, B.blk, B.cparent, B.cslot, B.dev, B.out);
 }
 
 void launch_zero_counter(stellar_ctx* c) {

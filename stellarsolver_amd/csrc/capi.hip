@@ -72,7 +72,7 @@ void free_all(stellar_ctx* c) {
                   B.ax,    B.ay,     B.inter,   B.key,    B.key_alt, B.perm,
                   B.perm_alt, B.key2, B.ncell,  B.coff,   B.tmp,    B.blk,
                   B.cstart, B.lam, B.cparent, B.clevel, B.cslot,
-                  B.csum,  B.ccom,    B.out,    B.dev,
+                  B.csum,  B.out,    B.dev,
                   B.bbox_part, B.scount, B.choff,
                   B.lmin,  B.qblk,  B.spine,   B.clist, B.clen,
                   B.key3,  B.ioff,  B.inst_of, B.ibbox, B.tstart, B.itile,
@@ -349,7 +349,6 @@ int create_ctx(const stellar_params* p, int32_t ninst, const int32_t* counts,
   chk(dmalloc(&B.choff, clcap / kMaxLevel + 64));
   chk(dmalloc(&B.clist, 2 * clcap));
   chk(dmalloc(&B.clen, 64));
-  chk(dmalloc(&B.ccom, NB));
   chk(dmalloc(&B.out, 2 * (N + C)));
   chk(dmalloc(&B.dev, 1));
   chk(dmalloc(&B.bbox_part, (size_t)c->ntiles));
@@ -758,13 +757,22 @@ int stellar_export_tree_instance(stellar_ctx* c, int32_t inst, int32_t* root4,
   }
   if (C > 0 && (com_x || com_y || com_m)) {
     if (c->stage < kComDone) return fail(c, STELLAR_ERR_STATE, "no centre of mass yet");
-    std::vector<float4> com((size_t)C);
-    HIP_TRY(c, hipMemcpy(com.data(), B.ccom + nr + c0, (size_t)C * sizeof(float4),
-                         hipMemcpyDeviceToHost));
+    // a cell's centre of mass and mass are its record in the parent's block
+    std::vector<int32_t> par((size_t)C);
+    std::vector<uint8_t> slt((size_t)C);
+    HIP_TRY(c, hipMemcpy(par.data(), B.cparent + nr + c0, (size_t)C * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(slt.data(), B.cslot + nr + c0, (size_t)C, hipMemcpyDeviceToHost));
     for (int64_t q = 0; q < C; ++q) {
-      if (com_x) com_x[q] = com[(size_t)q].x;
-      if (com_y) com_y[q] = com[(size_t)q].y;
-      if (com_m) com_m[q] = com[(size_t)q].z;
+      const int64_t pb = par[(size_t)q];
+      const int k = slt[(size_t)q] & 3;
+      ChildRec r{0.f, 0.f, 0.f, 0};
+      if (pb == inst)
+        r = rootb[(size_t)k];
+      else if (pb - nr - c0 >= 0 && pb - nr - c0 < q)
+        r = blk[(size_t)(4 * (pb - nr - c0) + k)];
+      if (com_x) com_x[q] = r.x;
+      if (com_y) com_y[q] = r.y;
+      if (com_m) com_m[q] = r.m;
     }
   }
   return STELLAR_OK;

@@ -41,7 +41,6 @@ struct ComArgs {
   const uint8_t* cslot;
   const uint8_t* clevel;
   double* csum;  // kSumStride per block
-  float4* ccom;
   DevScalars* dev;
   int quad;
   int32_t nroot;  // root blocks (instances); cell c is block nroot + c
@@ -137,10 +136,12 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
   out[0] = sm;
   out[1] = sx;
   out[2] = sy;
+  // centre of mass as sums times the reciprocal mass (one FP64 division)
+  const double inv = sm != 0.0 ? 1.0 / sm : 0.0;
+  const double ox = sx * inv, oy = sy * inv;
   const float fm = (float)sm;
-  const float fx = sm != 0.0 ? (float)(sx / sm) : 0.f;
-  const float fy = sm != 0.0 ? (float)(sy / sm) : 0.f;
-  A.ccom[b] = make_float4(fx, fy, fm, 0.f);
+  const float fx = (float)ox;
+  const float fy = (float)oy;
   const int32_t p = L.pblk[i];
   const int s = L.slot[i];
   A.blk[4 * (int64_t)p + s] = ChildRec{fx, fy, fm, b};
@@ -149,7 +150,6 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
     // traceless in-plane quadrupole about the FP64 centre of mass with the
     // parallel-axis shift of each child (operation order of the CPU
     // restatement, so the FP32 results agree bit for bit)
-    const double ox = sm != 0.0 ? sx / sm : 0.0, oy = sm != 0.0 ? sy / sm : 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int32_t code = L.rec[i][k].code;
@@ -163,8 +163,9 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
         cy = q[2];
       } else {
         cm = q[0];
-        cx = q[0] != 0.0 ? q[1] / q[0] : 0.0;
-        cy = q[0] != 0.0 ? q[2] / q[0] : 0.0;
+        const double ci = q[0] != 0.0 ? 1.0 / q[0] : 0.0;
+        cx = q[1] * ci;
+        cy = q[2] * ci;
       }
       const double dx = cx - ox, dy = cy - oy;
       const double t1 = dx * dx, t2 = dy * dy;
@@ -186,7 +187,6 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
   if (A.bmax) {
     // bmax bound: the largest distance from the FP64 centre of mass to a
     // body, through the children (bodies contribute their distance)
-    const double ox = sm != 0.0 ? sx / sm : 0.0, oy = sm != 0.0 ? sy / sm : 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int32_t code = L.rec[i][k].code;
@@ -199,8 +199,9 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
         cy = q[2];
         cb = 0.0;
       } else {
-        cx = q[0] != 0.0 ? q[1] / q[0] : 0.0;
-        cy = q[0] != 0.0 ? q[2] / q[0] : 0.0;
+        const double ci = q[0] != 0.0 ? 1.0 / q[0] : 0.0;
+        cx = q[1] * ci;
+        cy = q[2] * ci;
         cb = q[6];
       }
       const double dx = cx - ox, dy = cy - oy;
@@ -215,10 +216,14 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
   out[4] = qxy;
   out[5] = qyy;
   out[6] = bm;
+  // the global copy is read only by a parent that finishes in a later pass
+  // or chunk; a parent finished in this chunk takes the LDS copy
+  const int pi = L.pidx[i];
+  const bool keep = pi < 0 || !L.local[pi];
   double* g = A.csum + kSumStride * (int64_t)b;
 #pragma unroll
   for (int t = 0; t < kMom; ++t) {
-    g[t] = out[t];
+    if (keep) g[t] = out[t];
     L.mom[i][t] = out[t];
   }
 }
@@ -424,7 +429,7 @@ int64_t com_list_capacity(int64_t cap) {
 void launch_com(stellar_ctx* c) {
   if (c->n == 0 || c->cap == 0) return;
   Buffers& B = c->b;
-  ComArgs A{B.blk,  B.qblk, B.cparent, B.cslot, B.clevel, B.csum, B.ccom, B.dev,
+  ComArgs A{B.blk,  B.qblk, B.cparent, B.cslot, B.clevel, B.csum, B.dev,
             (c->p.flags & STELLAR_FLAG_QUADRUPOLE) ? 1 : 0, c->ninst,
             (c->p.flags & STELLAR_FLAG_BMAX_OPENING) ? 1 : 0,
             (double)c->p.theta * (double)c->p.theta, (double)c->p.eps2};

@@ -86,7 +86,6 @@ struct Buffers {
   int32_t *cstart = nullptr, *cparent = nullptr;
   uint8_t *clevel = nullptr, *cslot = nullptr;
   double* csum = nullptr;     // kSumStride FP64 moments per cell
-  float4* ccom = nullptr;     // per cell: com x, com y, mass, 0
   float4* qblk = nullptr;     // per child record: Qxx, Qxy, Qyy, 0
   int32_t* spine = nullptr;   // CoM pass leftovers, kMaxLevel slots per chunk
   int32_t* scount = nullptr;  // leftovers per chunk
