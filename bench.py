@@ -72,6 +72,9 @@ def parse():
                          "autotuned walk shape; compared with the same instances run one "
                          "context at a time")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device three times (a plain copy of known "
+                         "size that scales rocprofv3's FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--graph", action="store_true", help="replay captured two-step graphs")
     return ap.parse_args()
 
@@ -293,6 +296,16 @@ def main():
                           "produced (the walk would shard by contiguous ranges of the "
                           "tree order with one position all-gather per step)"}))
         return 2
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(256 << 20, device="cuda", dtype=torch.float32)  # 1 GiB
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b = a.clone()
+            torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": b.numel() * 4, "copies": 3}))
+        return 0
     import stellarsolver_amd as S
 
     S.load_library()
