@@ -14,7 +14,8 @@ tree walk, update) plus the output buffer, over bodies already resident in
 HBM.  Prints one JSON line: body-steps/s, per-stage times, the walk's
 roofline position (interaction rate against FP32 VALU issue peak), the rms
 relative force error against an FP64 direct sum on a seeded 8192-body
-subsample, FP64 energy/momentum drift, and the reference algorithm timed on
+subsample, FP64 energy, momentum and angular-momentum drift (velocities
+synchronised with the positions), and the reference algorithm timed on
 the host cores (the CPU oracle) on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -159,6 +160,17 @@ def force_error(eng, n, sample):
     tg = np.sort(rng.choice(n, min(n, sample), replace=False)).astype(np.int32)
     dx, dy = eng.direct_fp64(tg)
     return rms_rel(f["ax"][tg], f["ay"][tg], dx, dy), int(tg.size)
+
+
+def synced_diagnostics(eng, dt, forces):
+    """FP64 energy, momentum and angular momentum with velocities brought
+    level with the positions (v + dt/2 a), summed over instances; forces=True
+    first evaluates the accelerations at the current positions."""
+    if forces:
+        for stage in ("bbox", "build", "com", "sort", "forces"):
+            getattr(eng, stage)()
+    d = eng.diagnostics_fp64(half_dt=0.5 * dt)
+    return {k: float(np.sum(v)) for k, v in d.items()}
 
 
 def reference_opening_run(S, args, st, eps2):
@@ -327,7 +339,7 @@ def main():
                          | (S.FLAG_GRAPH if args.graph else 0))
     eng = S.Engine(n, p)
     eng.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
-    e0 = None if args.no_energy else eng.energy_fp64()
+    e0 = None if args.no_energy else synced_diagnostics(eng, args.dt, forces=True)
 
     if args.graph:
         # stage times come from direct steps (graph replays record none)
@@ -352,15 +364,17 @@ def main():
     (err_rms, err_p99, err_max), nsample = force_error(eng, n, args.error_sample)
     drift = None
     if e0 is not None:
-        # v is a half step ahead of x (kick-drift); the drift figure uses the
-        # state as stored, which bounds the true drift to O(dt) of E
-        e1 = eng.energy_fp64()
+        # the force pass of the error check left forces at the final positions
+        e1 = synced_diagnostics(eng, args.dt, forces=False)
         E0 = e0["kinetic"] + e0["potential"]
         E1 = e1["kinetic"] + e1["potential"]
         msum = float(st["mass"].astype(np.float64).sum())
         vsum = float((st["mass"].astype(np.float64) * np.hypot(st["vx"], st["vy"])).sum())
         drift = {"dE_over_E": (E1 - E0) / abs(E0),
                  "dP_over_sum_m_v": float(np.hypot(e1["px"] - e0["px"], e1["py"] - e0["py"]) / vsum),
+                 "dLz_over_Lz": (e1["lz"] - e0["lz"]) / abs(e0["lz"]),
+                 "velocities": "synchronised, v + dt/2 a(x) (the update stores v half a step behind x)",
+                 "steps": args.warmup + args.steps,
                  "total_mass": msum}
     eng.close()
 

@@ -216,7 +216,7 @@ int stellar_autotune(stellar_ctx* ctx, int32_t group_mask, int32_t reps,
                      stellar_tuning* out);
 /* Set the group-64 walk launch shape by hand: nodes_per_pop 1, 2 or 4
  * (4 is capped to 2 when a node's second block is read), block_threads 64,
- * 128, 256 or 512. */
+ * 128, 256, 512 or 1024. */
 int stellar_set_walk_shape(stellar_ctx* ctx, int32_t nodes_per_pop,
                            int32_t block_threads);
 
@@ -238,6 +238,24 @@ int stellar_energy_fp64(stellar_ctx* ctx, double eps2, double* kinetic,
 /* The same per instance: arrays of ninst values, any pointer may be NULL. */
 int stellar_energy_fp64_instances(stellar_ctx* ctx, double eps2, double* kinetic,
                                   double* potential, double* px, double* py);
+
+/* Conserved quantities of one instance, FP64.  No counterpart in the
+ * reference (it reports none); its update kernels.cu:522-537 fixes what
+ * "velocity" means here. */
+typedef struct stellar_diagnostics {
+  double kinetic;    /* sum m |v|^2 / 2                                    */
+  double potential;  /* -sum_{i<j} m_i m_j / sqrt(r_ij^2 + eps2)            */
+  double px, py;     /* sum m v                                             */
+  double lz;         /* sum m (x vy - y vx), about the origin               */
+} stellar_diagnostics;
+/* out: ninst records.  half_dt != 0 synchronises the velocities with the
+ * positions first, v + half_dt * a, using the accelerations of the last
+ * stellar_forces call: the kick-drift update stores v half a step behind x,
+ * so half_dt = dt/2 with forces evaluated at the stored positions gives the
+ * time-centred energy.  Then stellar_forces must be the last stage run
+ * (STELLAR_ERR_STATE otherwise). */
+int stellar_diagnostics_fp64(stellar_ctx* ctx, double eps2, double half_dt,
+                             stellar_diagnostics* out);
 
 /* ---- initial conditions (host) ------------------------------------------ */
 #define STELLAR_IC_DISK 0          /* central mass + uniform-radius disk     */

@@ -894,12 +894,31 @@ void or_energy(int32_t n, const float* x, const float* y, const float* vx,
                const float* vy, const float* mass, double eps2,
                int32_t threads, double* kinetic, double* potential,
                double* px, double* py) {
-  double k = 0.0, u = 0.0, mx = 0.0, my = 0.0;
+  double out[5];
+  or_diagnostics(n, x, y, vx, vy, mass, NULL, NULL, 0.0, eps2, threads, out);
+  *kinetic = out[0];
+  *potential = out[1];
+  *px = out[2];
+  *py = out[3];
+}
+
+/* Velocities are taken as v + half_dt * a (FP64) when ax is given: with the
+ * kick-drift update of kernels.cu:522-537 the stored v lags the positions by
+ * half a step, and a = the accelerations at the stored positions brings it
+ * level.  Angular momentum is about the origin: Lz = sum m (x vy - y vx). */
+void or_diagnostics(int32_t n, const float* x, const float* y, const float* vx,
+                    const float* vy, const float* mass, const float* ax,
+                    const float* ay, double half_dt, double eps2,
+                    int32_t threads, double* out) {
+  double k = 0.0, u = 0.0, mx = 0.0, my = 0.0, lz = 0.0;
   for (int32_t i = 0; i < n; ++i) {
-    const double m = mass[i], v2 = (double)vx[i] * vx[i] + (double)vy[i] * vy[i];
-    k += 0.5 * m * v2;
-    mx += m * vx[i];
-    my += m * vy[i];
+    const double m = mass[i];
+    const double vu = ax ? (double)vx[i] + half_dt * (double)ax[i] : (double)vx[i];
+    const double vv = ax ? (double)vy[i] + half_dt * (double)ay[i] : (double)vy[i];
+    k += 0.5 * m * (vu * vu + vv * vv);
+    mx += m * vu;
+    my += m * vv;
+    lz += m * ((double)x[i] * vv - (double)y[i] * vu);
   }
 #pragma omp parallel for schedule(dynamic, 16) reduction(+ : u) \
     num_threads(nthreads(threads))
@@ -911,10 +930,11 @@ void or_energy(int32_t n, const float* x, const float* y, const float* vx,
     }
     u -= (double)mass[i] * s;
   }
-  *kinetic = k;
-  *potential = u;
-  *px = mx;
-  *py = my;
+  out[0] = k;
+  out[1] = u;
+  out[2] = mx;
+  out[3] = my;
+  out[4] = lz;
 }
 
 /* ------------------------------------------------------------------------ */

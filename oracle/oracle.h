@@ -154,6 +154,12 @@ void or_energy(int32_t n, const float* x, const float* y, const float* vx,
                const float* vy, const float* mass, double eps2,
                int32_t threads, double* kinetic, double* potential,
                double* px, double* py);
+/* out[5] = kinetic, potential, px, py, Lz (about the origin); with ax, ay
+ * non-NULL the velocities are v + half_dt * a. */
+void or_diagnostics(int32_t n, const float* x, const float* y, const float* vx,
+                    const float* vy, const float* mass, const float* ax,
+                    const float* ay, double half_dt, double eps2,
+                    int32_t threads, double* out);
 
 /* One whole reference step on the CPU (bbox, insertion build, FP32 CoM,
  * sort, group-of-G walk, update), state by body id.  Work arrays are

@@ -61,6 +61,8 @@ _SIG = {
     "or_update": (None, [C.c_int32, _F, _F, _F, _F, _F, _F, C.c_float, C.c_float]),
     "or_invariants": (C.c_int32, [C.c_int32, C.c_int32, _I, _I, _I, _F]),
     "or_energy": (None, [C.c_int32, _F, _F, _F, _F, _F, C.c_double, C.c_int32, _D, _D, _D, _D]),
+    "or_diagnostics": (None, [C.c_int32, _F, _F, _F, _F, _F, _F, _F, C.c_double, C.c_double,
+                              C.c_int32, _D]),
     "or_step": (C.c_int, [C.c_int32, _F, _F, _F, _F, _F, _F, _F, C.POINTER(StepParams), _D]),
     "or_num_threads": (C.c_int, []),
 }
@@ -307,6 +309,19 @@ def energy(x, y, vx, vy, mass, eps2: float, threads: int = 0) -> dict:
     a = [f32(q) for q in (x, y, vx, vy, mass)]
     lib().or_energy(a[0].size, *(_f(q) for q in a), float(eps2), threads, *(C.byref(q) for q in v))
     return {"kinetic": v[0].value, "potential": v[1].value, "px": v[2].value, "py": v[3].value}
+
+
+def diagnostics(x, y, vx, vy, mass, eps2: float, ax=None, ay=None, half_dt: float = 0.0,
+                threads: int = 0) -> dict:
+    """Kinetic and potential energy, momentum and angular momentum about the
+    origin in FP64; with ax, ay given the velocities are v + half_dt * a."""
+    a = [f32(q) for q in (x, y, vx, vy, mass)]
+    acc = [f32(q) for q in (ax, ay)] if ax is not None else None
+    out = np.zeros(5, np.float64)
+    lib().or_diagnostics(a[0].size, *(_f(q) for q in a),
+                         _f(acc[0]) if acc else None, _f(acc[1]) if acc else None,
+                         float(half_dt), float(eps2), threads, _d(out))
+    return dict(zip(("kinetic", "potential", "px", "py", "lz"), (float(v) for v in out)))
 
 
 def step(state: dict, theta: float, eps2: float, dt: float, d: float, group: int = 32,

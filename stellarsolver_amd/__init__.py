@@ -89,6 +89,10 @@ class StellarTuning(C.Structure):
     ]
 
 
+class StellarDiagnostics(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("kinetic", "potential", "px", "py", "lz")]
+
+
 # exported symbol -> (restype, argtypes)
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
@@ -131,6 +135,7 @@ SIGNATURES = {
     "stellar_direct_fp64": (C.c_int, [_P, C.c_int32, _I, C.c_double, _D, _D]),
     "stellar_energy_fp64": (C.c_int, [_P, C.c_double, _D, _D, _D, _D]),
     "stellar_energy_fp64_instances": (C.c_int, [_P, C.c_double, _D, _D, _D, _D]),
+    "stellar_diagnostics_fp64": (C.c_int, [_P, C.c_double, C.c_double, C.POINTER(StellarDiagnostics)]),
     "stellar_autotune": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(StellarTuning)]),
     "stellar_set_walk_shape": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "stellar_ic_defaults": (C.c_int, [C.c_int32, C.POINTER(StellarICParams)]),
@@ -415,6 +420,16 @@ class Engine:
         self._check(self._lib.stellar_set_walk_shape(self._h, int(nodes_per_pop), int(block_threads)),
                     "set_walk_shape")
 
+    def diagnostics_fp64(self, eps2: float | None = None, half_dt: float = 0.0) -> dict[str, np.ndarray]:
+        """Per-instance FP64 kinetic and potential energy, momentum and angular
+        momentum about the origin; half_dt != 0 takes velocities as
+        v + half_dt * a with the accelerations of the last forces stage."""
+        e2 = float(self.params.eps2) if eps2 is None else float(eps2)
+        out = (StellarDiagnostics * self.ninst)()
+        self._check(self._lib.stellar_diagnostics_fp64(self._h, e2, float(half_dt), out), "diagnostics_fp64")
+        return {k: np.array([getattr(d, k) for d in out], np.float64)
+                for k, _ in StellarDiagnostics._fields_}
+
     def energy_fp64_instances(self, eps2: float | None = None) -> dict[str, np.ndarray]:
         e2 = float(self.params.eps2) if eps2 is None else float(eps2)
         v = {k: np.zeros(self.ninst, np.float64) for k in ("kinetic", "potential", "px", "py")}
@@ -424,6 +439,7 @@ class Engine:
 
 
 __all__ = [
-    "Engine", "StellarParams", "StellarICParams", "StellarTuning", "StellarError", "load_library",
+    "Engine", "StellarParams", "StellarICParams", "StellarTuning", "StellarDiagnostics", "StellarError",
+    "load_library",
     "default_params", "ic_params", "generate_ic", "LIB_PATH", "SIGNATURES", "STAGES",
 ]

@@ -185,27 +185,34 @@ __global__ __launch_bounds__(kTile) void direct_fp64_kernel(
   }
 }
 
-// Plummer-softened potential, kinetic energy and momentum of kTile targets
-// (slice blockIdx.y of tile blockIdx.x, positions of one instance) summed
-// over the instance's sources: part[4*(t*kSlices + slice) + {0..3}] = pot,
-// kin, px, py with pot = -1/2 sum_i m_i sum_{j != i} m_j / sqrt(r^2 + eps2).
+// Plummer-softened potential, kinetic energy, momentum and angular momentum
+// about the origin of kTile targets (slice blockIdx.y of tile blockIdx.x,
+// positions of one instance) summed over the instance's sources:
+// part[kEnergyVals*(t*kSlices + slice) + {0..4}] = pot, kin, px, py, lz with
+// pot = -1/2 sum_i m_i sum_{j != i} m_j / sqrt(r^2 + eps2).  With ax
+// non-null the velocities are v + half_dt * a (FP64): the kick-drift update
+// leaves v half a step behind the positions, and the accelerations of the
+// stored positions bring it level.
 constexpr int kSlices = kEnergySlices;
+constexpr int kVals = kEnergyVals;
 static_assert(kSlices * kTile == kBboxTile, "slices cover a tile");
 __global__ __launch_bounds__(kTile) void energy_fp64_kernel(
     const int32_t* __restrict__ tstart, const int32_t* __restrict__ inst_of,
     const int32_t* __restrict__ ioff, const float* __restrict__ x,
     const float* __restrict__ y, const float* __restrict__ vx,
-    const float* __restrict__ vy, const float* __restrict__ m, double eps2,
-    double* __restrict__ part) {
+    const float* __restrict__ vy, const float* __restrict__ m,
+    const float* __restrict__ ax, const float* __restrict__ ay, double half_dt,
+    double eps2, double* __restrict__ part) {
   __shared__ float sx[kTile], sy[kTile], sm[kTile];
-  __shared__ double red[4][kTile / 64];
+  __shared__ double red[kVals][kTile / 64];
   const int32_t t0 = tstart[blockIdx.x], t1 = tstart[blockIdx.x + 1];
   const int32_t inst = inst_of[t0];
   const int32_t lo = ioff[inst], hi = ioff[inst + 1];
-  double vals[4] = {0.0, 0.0, 0.0, 0.0};
+  double vals[kVals] = {0.0, 0.0, 0.0, 0.0, 0.0};
   const int32_t tb = t0 + (int32_t)blockIdx.y * kTile;
   if (tb >= t1) {  // a short tile: this slice has no targets
-    if (threadIdx.x < 4) part[4 * ((int64_t)blockIdx.x * kSlices + blockIdx.y) + threadIdx.x] = 0.0;
+    if (threadIdx.x < kVals)
+      part[kVals * ((int64_t)blockIdx.x * kSlices + blockIdx.y) + threadIdx.x] = 0.0;
     return;
   }
   {
@@ -229,24 +236,30 @@ __global__ __launch_bounds__(kTile) void energy_fp64_kernel(
       }
     }
     if (active) {
-      const double mi = m[i], u = vx[i], v = vy[i];
+      const double mi = m[i];
+      double u = vx[i], v = vy[i];
+      if (ax) {
+        u += half_dt * (double)ax[i];
+        v += half_dt * (double)ay[i];
+      }
       vals[0] += -0.5 * mi * pot;
       vals[1] += 0.5 * mi * (u * u + v * v);
       vals[2] += mi * u;
       vals[3] += mi * v;
+      vals[4] += mi * (xi * v - yi * u);
     }
   }
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < kVals; ++k)
     for (int off = 32; off > 0; off >>= 1) vals[k] += __shfl_xor(vals[k], off);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
-    for (int k = 0; k < 4; ++k) red[k][w] = vals[k];
+    for (int k = 0; k < kVals; ++k) red[k][w] = vals[k];
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kVals; ++k) {
       double s = 0.0;
       for (int q = 0; q < kTile / 64; ++q) s += red[k][q];
-      part[4 * ((int64_t)blockIdx.x * kSlices + blockIdx.y) + k] = s;
+      part[kVals * ((int64_t)blockIdx.x * kSlices + blockIdx.y) + k] = s;
     }
   }
 }
@@ -302,12 +315,12 @@ void launch_direct_fp64(stellar_ctx* c, int32_t nt, const int32_t* dtargets,
 
 int energy_parts(stellar_ctx* c) { return c->ntiles * kSlices; }
 
-void launch_energy_fp64(stellar_ctx* c, double eps2, double* dpart,
-                        int32_t nparts) {
-  (void)nparts;
+void launch_energy_fp64(stellar_ctx* c, double eps2, double half_dt, bool sync,
+                        double* dpart) {
   Buffers& B = c->b;
   energy_fp64_kernel<<<dim3((unsigned)c->ntiles, kSlices), kTile, 0, c->stream>>>(
-      B.tstart, B.inst_of, B.ioff, B.x, B.y, B.vx, B.vy, B.m, eps2, dpart);
+      B.tstart, B.inst_of, B.ioff, B.x, B.y, B.vx, B.vy, B.m, sync ? B.ax : nullptr,
+      sync ? B.ay : nullptr, half_dt, eps2, dpart);
 }
 
 }  // namespace stellar

@@ -385,6 +385,7 @@ int create_ctx(const stellar_params* p, int32_t ninst, const int32_t* counts,
   chk(hipMemsetAsync(B.ibbox, 0, (size_t)ninst * sizeof(float4), c->stream));
   chk(hipMemsetAsync(B.ax, 0, N * sizeof(float), c->stream));
   chk(hipMemsetAsync(B.ay, 0, N * sizeof(float), c->stream));
+  chk(hipMemsetAsync(B.inter, 0, N * sizeof(int32_t), c->stream));
   launch_init_scalars(c);
   chk(hipGetLastError());
   chk(hipStreamSynchronize(c->stream));
@@ -854,40 +855,62 @@ int stellar_direct_fp64(stellar_ctx* c, int32_t ntargets, const int32_t* targets
   return STELLAR_OK;
 }
 
+// Per-instance FP64 sums of the energy kernel's slices, in a fixed order
+// (the tiles of an instance in position order), so results do not depend
+// on scheduling.
+static int diagnostics(stellar_ctx* c, double eps2, double half_dt, bool sync,
+                       stellar_diagnostics* out) {
+  const int32_t ni = c->ninst;
+  for (int32_t i = 0; i < ni; ++i) out[i] = stellar_diagnostics{0.0, 0.0, 0.0, 0.0, 0.0};
+  if (c->n == 0) return STELLAR_OK;
+  const int parts = energy_parts(c);
+  double* dpart = nullptr;
+  HIP_TRY(c, hipMalloc(&dpart, (size_t)parts * kEnergyVals * sizeof(double)));
+  launch_energy_fp64(c, eps2, half_dt, sync, dpart);
+  std::vector<double> h((size_t)parts * kEnergyVals);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h.data(), dpart, h.size() * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dpart);
+  if (e != hipSuccess) return hip_fail(c, e, "FP64 diagnostics");
+  for (int32_t i = 0; i < ni; ++i) {
+    stellar_diagnostics& d = out[i];
+    for (int64_t q = (int64_t)c->itile[(size_t)i] * kEnergySlices;
+         q < (int64_t)c->itile[(size_t)i + 1] * kEnergySlices; ++q) {
+      const double* v = &h[(size_t)(kEnergyVals * q)];
+      d.potential += v[0];
+      d.kinetic += v[1];
+      d.px += v[2];
+      d.py += v[3];
+      d.lz += v[4];
+    }
+  }
+  return STELLAR_OK;
+}
+
+int stellar_diagnostics_fp64(stellar_ctx* c, double eps2, double half_dt,
+                             stellar_diagnostics* out) {
+  if (!c || !out) return STELLAR_ERR_ARG;
+  if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
+  const bool sync = half_dt != 0.0;
+  if (sync && c->stage != kForcesDone)
+    return fail(c, STELLAR_ERR_STATE, "synchronised velocities need forces at the current positions");
+  return diagnostics(c, eps2, half_dt, sync, out);
+}
+
 int stellar_energy_fp64_instances(stellar_ctx* c, double eps2, double* kinetic,
                                   double* potential, double* px, double* py) {
   if (!c) return STELLAR_ERR_ARG;
   if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
-  const int32_t ni = c->ninst;
-  std::vector<double> k((size_t)ni, 0.0), u((size_t)ni, 0.0), mx((size_t)ni, 0.0),
-      my((size_t)ni, 0.0);
-  if (c->n > 0) {
-    const int parts = energy_parts(c);
-    double* dpart = nullptr;
-    HIP_TRY(c, hipMalloc(&dpart, (size_t)parts * 4 * sizeof(double)));
-    launch_energy_fp64(c, eps2, dpart, parts);
-    std::vector<double> h((size_t)parts * 4);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(h.data(), dpart, h.size() * 8, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(dpart);
-    if (e != hipSuccess) return hip_fail(c, e, "stellar_energy_fp64");
-    // fixed summation order: tiles of an instance in position order
-    for (int32_t i = 0; i < ni; ++i)
-      for (int64_t q = (int64_t)c->itile[(size_t)i] * kEnergySlices;
-           q < (int64_t)c->itile[(size_t)i + 1] * kEnergySlices; ++q) {
-        u[(size_t)i] += h[(size_t)(4 * q + 0)];
-        k[(size_t)i] += h[(size_t)(4 * q + 1)];
-        mx[(size_t)i] += h[(size_t)(4 * q + 2)];
-        my[(size_t)i] += h[(size_t)(4 * q + 3)];
-      }
-  }
-  for (int32_t i = 0; i < ni; ++i) {
-    if (kinetic) kinetic[i] = k[(size_t)i];
-    if (potential) potential[i] = u[(size_t)i];
-    if (px) px[i] = mx[(size_t)i];
-    if (py) py[i] = my[(size_t)i];
+  std::vector<stellar_diagnostics> d((size_t)c->ninst);
+  const int rc = diagnostics(c, eps2, 0.0, false, d.data());
+  if (rc) return rc;
+  for (int32_t i = 0; i < c->ninst; ++i) {
+    if (kinetic) kinetic[i] = d[(size_t)i].kinetic;
+    if (potential) potential[i] = d[(size_t)i].potential;
+    if (px) px[i] = d[(size_t)i].px;
+    if (py) py[i] = d[(size_t)i].py;
   }
   return STELLAR_OK;
 }
@@ -917,7 +940,8 @@ int stellar_set_walk_shape(stellar_ctx* c, int32_t nodes_per_pop, int32_t block_
   if (!c) return STELLAR_ERR_ARG;
   if (nodes_per_pop != 1 && nodes_per_pop != 2 && nodes_per_pop != 4)
     return fail(c, STELLAR_ERR_ARG, "nodes_per_pop");
-  if (block_threads != 64 && block_threads != 128 && block_threads != 256 && block_threads != 512)
+  if (block_threads != 64 && block_threads != 128 && block_threads != 256 && block_threads != 512 &&
+      block_threads != 1024)
     return fail(c, STELLAR_ERR_ARG, "block_threads");
   c->walk_kpop = nodes_per_pop;
   c->walk_waves = block_threads / 64;
@@ -966,7 +990,7 @@ int stellar_autotune(stellar_ctx* c, int32_t group_mask, int32_t reps, stellar_t
   const int mask = group_mask == 0 ? 1 : group_mask;
   if (mask & 1)
     for (int kp : {1, 2, 4})
-      for (int w : {1, 2, 4, 8})
+      for (int w : {1, 2, 4, 8, 16})
         if (!((quad || (c->p.flags & STELLAR_FLAG_BMAX_OPENING)) && kp == 4))
           cand.push_back(stellar_tuning{64, kp, 64 * w, 0.f});
   if (mask & 2) cand.push_back(stellar_tuning{32, 1, 256, 0.f});

@@ -27,6 +27,20 @@ constexpr int64_t kMaxBlocks = int64_t(1) << 26;  // root + cells
 constexpr int kMaxTieRun = 256;
 constexpr int kWalkStack = 256;  // >= 4 + 3*(kMaxLevel-1)
 constexpr int kWalkBlock = 256;  // threads per walk workgroup (4 wavefronts)
+
+// MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs
+// (workgroup b to XCD b % 8), each XCD with its own L2.  xcd_block maps
+// hardware workgroup ids to logical ones so that each XCD gets one
+// contiguous range of logical workgroups: neighbouring walk groups (nearby
+// bodies, mostly the same tree nodes) then share one L2.  A bijection on
+// [0, nb) for any nb.
+constexpr int kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb / kXcds, r = nb % kXcds;
+  const uint32_t x = b % kXcds, i = b / kXcds;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // FP64 moments per cell: sum m, sum m x, sum m y, Qxx, Qxy, Qyy, bmax
 constexpr int kSumStride = 7;
 
@@ -59,6 +73,8 @@ static_assert(sizeof(DevScalars) == 256, "one 256-byte line");
 constexpr int kBboxTile = 4096;
 // FP64 energy: slices of 256 targets per bounding-box tile.
 constexpr int kEnergySlices = kBboxTile / 256;
+// FP64 sums per slice: potential, kinetic, px, py, Lz.
+constexpr int kEnergyVals = 5;
 
 struct Buffers {
   // body state in tree order of the last sort (double-buffered for the sort)
@@ -172,8 +188,8 @@ size_t build_scratch_bytes(int32_t n);
 size_t mintree_bytes(int32_t n);
 void launch_direct_fp64(stellar_ctx* c, int32_t nt, const int32_t* dtargets,
                         double eps2, double* dax, double* day);
-void launch_energy_fp64(stellar_ctx* c, double eps2, double* dpart,
-                        int32_t nparts);
+void launch_energy_fp64(stellar_ctx* c, double eps2, double half_dt, bool sync,
+                        double* dpart);
 int energy_parts(stellar_ctx* c);
 int64_t com_list_capacity(int64_t cap);
 

@@ -100,6 +100,33 @@ struct Acc {
     ++cnt;
     ++qcnt;
   }
+  // Either of the two above, chosen by a wavefront-uniform flag, with the
+  // accumulating multiply-adds shared after the branch: both sides produce
+  // (a, bx, by) and add a*bx, a*by, so the results stay in place instead of
+  // being copied out of one side (the same arithmetic as mono / quad).
+  __device__ __forceinline__ void term(bool with_quad, float cm, float r,
+                                       float dx, float dy, float4 q) {
+    const float rs = __builtin_amdgcn_rsqf(r);
+    float a, bx, by;
+    if (with_quad) {
+      const float u = rs * rs;
+      a = (u * u) * rs;
+      const float px = __builtin_fmaf(q.x, dx, q.y * dy);
+      const float py = __builtin_fmaf(q.y, dx, q.z * dy);
+      const float s = __builtin_fmaf(dx, px, dy * py);
+      const float c = __builtin_fmaf(2.5f * s, u, cm * r);
+      bx = __builtin_fmaf(c, dx, -px);
+      by = __builtin_fmaf(c, dy, -py);
+      ++qcnt;
+    } else {
+      a = cm * rs * rs * rs;
+      bx = dx;
+      by = dy;
+    }
+    fx = __builtin_fmaf(a, bx, fx);
+    fy = __builtin_fmaf(a, by, fy);
+    ++cnt;
+  }
 };
 
 // ---------------------------------------------------------------- G = 64 --
@@ -111,21 +138,17 @@ constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
 // end are dropped and the walk stops with kErrStack.
 constexpr int kStack64 = 1024;
 
-// The tree arrays are separate const __restrict__ arguments: that is what
-// lets the compiler read them through the scalar data cache.
-template <bool QUAD, bool THR, int KPOP, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
-    const ChildRec* __restrict__ blk, const float4* __restrict__ qblk,
-    WalkArgs A) {
-  constexpr int kWaves = WAVES;
-  __shared__ uint32_t stk[kWaves][kStack64];
-
+// One walk group on one wavefront: the traversal of group g, its forces and
+// interaction counts.  The tree arrays are separate const __restrict__
+// arguments: that is what lets the compiler read them through the scalar
+// data cache.
+template <bool QUAD, bool THR, int KPOP>
+__device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
+                                             const float4* __restrict__ qblk,
+                                             const WalkArgs& A, int64_t g,
+                                             uint32_t* st) {
   const float eps2 = A.eps2;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int64_t g = (int64_t)blockIdx.x * kWaves + wave;
-  if (g >= A.ngroups) return;
-  if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
   const int32_t s0 = __builtin_amdgcn_readfirstlane(A.gtab[g]);
   const int32_t cnt = __builtin_amdgcn_readfirstlane(A.gtab[g + 1]) - s0;
   const int32_t inst = __builtin_amdgcn_readfirstlane(A.inst_of[s0]);
@@ -144,7 +167,6 @@ __global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
   // textbook test on the unsoftened distance: dp + eps2 <= d^2 + eps2
   if (A.unsoft) thr = thr + eps2;
 
-  uint32_t* st = stk[wave];
   int sp = 0;
   Acc acc;
 
@@ -219,11 +241,8 @@ __global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
         const float cthr = THR ? qm[kQ ? jj : 0][k].w : dp[jj];
         const int32_t ti = code < 0 ? __float_as_int(-1.0f) : __float_as_int(cthr);
         const float t = __int_as_float(__builtin_amdgcn_readfirstlane(ti));
-        if (__builtin_amdgcn_ballot_w64(!(t <= r)) == 0) {
-          if (QUAD && code > 0)
-            acc.quad(cmv, r, dx, dy, qm[kQ ? jj : 0][k]);
-          else
-            acc.mono(cmv, r, dx, dy);
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(!(t <= r)) == 0, 1)) {
+          acc.term(QUAD && code > 0, cmv, r, dx, dy, qm[kQ ? jj : 0][k]);
         } else {
           if (lane == 0 && sp < kStack64) st[sp] = (uint32_t)code | child_lvl;
           ++sp;
@@ -248,6 +267,23 @@ __global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
   }
 }
 
+// One wavefront per walk group, the WAVES wavefronts of a workgroup on
+// consecutive groups: they start together and visit mostly the same nodes
+// at about the same time, which the CU's scalar cache serves once.
+// Workgroups are mapped so that each XCD walks one contiguous range of
+// groups (xcd_block), which its L2 serves.
+template <bool QUAD, bool THR, int KPOP, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
+    const ChildRec* __restrict__ blk, const float4* __restrict__ qblk,
+    WalkArgs A) {
+  __shared__ uint32_t stk[WAVES][kStack64];
+  const int wave = threadIdx.x >> 6;
+  const int64_t g = (int64_t)xcd_block(blockIdx.x, gridDim.x) * WAVES + wave;
+  if (g >= A.ngroups) return;
+  if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
+  walk64_group<QUAD, THR, KPOP>(blk, qblk, A, g, stk[wave]);
+}
+
 // ----------------------------------------------------------- G = 32, 16 --
 template <int G, bool QUAD, bool THR>
 __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
@@ -264,7 +300,8 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
   const int sub = lane / G;             // group within the wavefront
   const int glane = lane & (G - 1);     // lane within the group
   const int slot = wave * kGroupsPerWave + sub;
-  const int64_t g0 = ((int64_t)blockIdx.x * kWaves + wave) * kGroupsPerWave;
+  const int64_t g0 =
+      ((int64_t)xcd_block(blockIdx.x, gridDim.x) * kWaves + wave) * kGroupsPerWave;
   if (g0 >= A.ngroups) return;  // the whole wavefront
   if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
   const int64_t g = g0 + sub;
@@ -376,6 +413,9 @@ void launch64(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
       break;
     case 8:
       walk64_kernel<QUAD, THR, KPOP, 8><<<grid, 512, 0, c->stream>>>(A.blk, A.qblk, A);
+      break;
+    case 16:
+      walk64_kernel<QUAD, THR, KPOP, 16><<<grid, 1024, 0, c->stream>>>(A.blk, A.qblk, A);
       break;
     default:
       walk64_kernel<QUAD, THR, KPOP, 4><<<grid, 256, 0, c->stream>>>(A.blk, A.qblk, A);

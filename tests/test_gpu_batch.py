@@ -128,6 +128,7 @@ def test_batch_fp64_diagnostics(S):
     with batch_engine(S, sts) as e:
         e.run(2)
         ens = e.energy_fp64_instances()
+        dia = e.diagnostics_fp64()
         tot = e.energy_fp64()
         ids = np.array([0, 1000, 1063, 1064, 6064, 6065, 6066], np.int32)
         bx, by = e.direct_fp64(ids)
@@ -137,8 +138,11 @@ def test_batch_fp64_diagnostics(S):
         with single_engine(S, st) as s:
             s.run(2)
             en = s.energy_fp64()
+            ds = s.diagnostics_fp64()
             for k in en:
                 assert ens[k][i] == pytest.approx(en[k], rel=1e-12, abs=1e-12), (i, k)
+            for k in ds:
+                assert dia[k][i] == pytest.approx(ds[k][0], rel=1e-12, abs=1e-12), (i, k)
             loc = ids[(ids >= off[i]) & (ids < off[i + 1])]
             if loc.size:
                 sx, sy = s.direct_fp64(loc - off[i])
@@ -207,7 +211,7 @@ def test_autotune_keeps_interactions(S):
         t = e.autotune(group_mask=0, reps=2)
         print("tuning", t, flush=True)
         assert t["group"] == 64 and t["nodes_per_pop"] in (1, 2, 4)
-        assert t["block_threads"] in (64, 128, 256, 512) and t["walk_ms"] > 0
+        assert t["block_threads"] in (64, 128, 256, 512, 1024) and t["walk_ms"] > 0
         e.bbox()
         e.build()
         e.com()

@@ -219,6 +219,36 @@ def test_direct_and_energy_fp64(S, oracle):
         assert np.isclose(en[k], oe[k], rtol=1e-11, atol=1e-9), k
 
 
+def test_diagnostics_fp64_with_synchronised_velocities(S, oracle):
+    """Energy, momentum and angular momentum on the device equal the oracle's
+    FP64 sums, with raw velocities and with v + dt/2 a from the forces stage
+    (the accelerations exported by the engine); the synchronised form is
+    refused unless the forces stage was the last one run."""
+    st = ic(1, 6000)
+    eps2 = 1e-4
+    half = 0.5e-3
+    with make_engine(S, st, eps2) as e:
+        e.run(2)
+        raw = e.diagnostics_fp64(eps2=eps2)
+        s0 = e.download()
+        with pytest.raises(S.StellarError) as err:
+            e.diagnostics_fp64(eps2=eps2, half_dt=half)
+        assert err.value.status == S.STELLAR_ERR_STATE
+        for stage in ("bbox", "build", "com", "sort", "forces"):
+            getattr(e, stage)()
+        f = e.export_forces()
+        syn = e.diagnostics_fp64(eps2=eps2, half_dt=half)
+    o = oracle.diagnostics(s0["x"], s0["y"], s0["vx"], s0["vy"], s0["mass"], eps2)
+    os_ = oracle.diagnostics(s0["x"], s0["y"], s0["vx"], s0["vy"], s0["mass"], eps2,
+                             ax=f["ax"], ay=f["ay"], half_dt=half)
+    for k in o:
+        print(k, raw[k][0], o[k], syn[k][0], os_[k], flush=True)
+        assert raw[k][0] == pytest.approx(o[k], rel=1e-11, abs=1e-9), k
+        assert syn[k][0] == pytest.approx(os_[k], rel=1e-11, abs=1e-9), k
+    assert syn["kinetic"][0] != raw["kinetic"][0]
+    assert syn["potential"][0] == raw["potential"][0]
+
+
 def test_headline_size_tree_and_counts(S, oracle):
     """N = 2^20 colliding disks: bit-exact tree and interaction counts; force
     error against the FP64 direct sum on a seeded 8192-body subsample."""

@@ -336,6 +336,60 @@ def test_energy_two_body(oracle):
     assert np.isclose(e["px"], 2.0) and np.isclose(e["py"], 10.0)
 
 
+def test_diagnostics_two_body_closed_form(oracle):
+    x = np.array([0.0, 3.0], np.float32)
+    y = np.array([0.0, 4.0], np.float32)
+    m = np.array([2.0, 5.0], np.float32)
+    vx = np.array([1.0, 0.0], np.float32)
+    vy = np.array([0.0, 2.0], np.float32)
+    d = oracle.diagnostics(x, y, vx, vy, m, 0.0)
+    assert np.isclose(d["kinetic"], 11.0) and np.isclose(d["potential"], -2.0)
+    assert np.isclose(d["px"], 2.0) and np.isclose(d["py"], 10.0)
+    assert np.isclose(d["lz"], 5.0 * (3.0 * 2.0 - 4.0 * 0.0))
+    # velocities advanced by half_dt * a before the sums
+    ax = np.array([2.0, -1.0], np.float32)
+    ay = np.array([4.0, 0.5], np.float32)
+    s = oracle.diagnostics(x, y, vx, vy, m, 0.0, ax=ax, ay=ay, half_dt=0.5)
+    u = vx + 0.5 * ax
+    v = vy + 0.5 * ay
+    assert np.isclose(s["kinetic"], float(np.sum(0.5 * m * (u * u + v * v))))
+    assert np.isclose(s["px"], float(np.sum(m * u))) and np.isclose(s["py"], float(np.sum(m * v)))
+    assert np.isclose(s["lz"], float(np.sum(m * (x * v - y * u))))
+    assert s["potential"] == d["potential"]
+
+
+def test_synchronised_energy_of_kick_drift_orbit(oracle):
+    """Two equal masses on an eccentric orbit (half the circular speed, the
+    pair falls to 0.43 of its separation) stepped with the reference update
+    (kick then drift) under direct forces: energy taken with v + dt/2 a(x)
+    is conserved to O(dt^2), the raw half-step velocities, or v - dt/2 a,
+    only to O(dt), which pins the sign and size of the synchronisation;
+    angular momentum is conserved to rounding (central pair forces)."""
+    x = np.array([-0.5, 0.5], np.float32)
+    y = np.zeros(2, np.float32)
+    m = np.ones(2, np.float32)
+    vc = np.float32(0.5 * np.sqrt(2.0) / 2.0)
+    vx = np.zeros(2, np.float32)
+    vy = np.array([-vc, vc], np.float32)
+    dt, eps2 = 1e-3, 1e-10  # the direct sum's self term needs eps2 > 0
+    sync, raw, wrong, lz = [], [], [], []
+    for _ in range(800):
+        ax, ay = (a.astype(np.float32) for a in oracle.direct_fp64(x, y, m, eps2))
+        s = oracle.diagnostics(x, y, vx, vy, m, eps2, ax=ax, ay=ay, half_dt=dt / 2)
+        r = oracle.diagnostics(x, y, vx, vy, m, eps2)
+        w = oracle.diagnostics(x, y, vx, vy, m, eps2, ax=ax, ay=ay, half_dt=-dt / 2)
+        sync.append(s["kinetic"] + s["potential"])
+        raw.append(r["kinetic"] + r["potential"])
+        wrong.append(w["kinetic"] + w["potential"])
+        lz.append(s["lz"])
+        x, y, vx, vy = oracle.update(x, y, vx, vy, ax, ay, dt, 1.0)
+    sync, raw, wrong, lz = (np.array(q) for q in (sync, raw, wrong, lz))
+    e0 = abs(sync[0])
+    assert np.ptp(sync) / e0 < 5e-5
+    assert np.ptp(raw) / e0 > 1e-3 and np.ptp(wrong) / e0 > 1e-3
+    assert np.ptp(lz) / abs(lz[0]) < 1e-5
+
+
 # ------------------------------------------------------------ quadrupole ---
 def test_quadrupole_matches_body_range_moments(oracle):
     """Q of each cell = sum over its bodies of m (3 d d^T - |d|^2 I), d from
