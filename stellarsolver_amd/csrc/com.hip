@@ -15,9 +15,11 @@
 // and its children have larger ids.  A pass
 // takes a sorted list of unfinished cells in chunks of kChunk; a workgroup
 // stages its chunk's child records and child moments in LDS and finishes
-// every cell whose unfinished descendants all lie in the chunk, following
-// dependency chains with LDS arrival counters (no device-scope fences: on
-// gfx950 each one writes back the issuing XCD's L2).  The cells it cannot
+// every cell whose unfinished descendants all lie in the chunk, level by
+// level from the deepest (a cell's children are one level deeper), the
+// cells of one level spread densely over the workgroup's threads (no
+// device-scope fences: on gfx950 each one writes back the issuing XCD's
+// L2).  The cells it cannot
 // finish are ancestors of the first cell after the chunk -- at most one per
 // level -- and form the next pass's list.  Pass 1 runs over all cells; after
 // a few passes the list is short and one workgroup finishes it, walking its
@@ -32,7 +34,10 @@ namespace {
 constexpr int kChunk = 256;          // cells per workgroup and chunk
 constexpr int kComThreads = 256;
 constexpr int kSlots = kMaxLevel;    // leftover cells per chunk (one per level)
-constexpr int kMom = kSumStride;     // FP64 moments per child slot
+constexpr int kMom = kSumStride;     // FP64 moments kept in global memory
+// In LDS a finished cell also keeps its FP64 centre of mass (the quotient its
+// parent would otherwise recompute): m, mx, my, Qxx, Qxy, Qyy, bmax, cx, cy.
+constexpr int kMomL = kMom + 2;
 
 struct ComArgs {
   ChildRec* blk;
@@ -53,18 +58,21 @@ template <bool EXT>
 struct ChunkLds {
   int32_t ids[kChunk];          // block ids, ascending
   int32_t pidx[kChunk];         // chunk index of the parent, -1 if none
-  int32_t pend[kChunk];         // unfinished children inside the chunk
-  int32_t pblk[kChunk];         // parent block
-  uint8_t slot[kChunk];         // slot in the parent block
+  int16_t bylev[kChunk];        // local entries grouped by level, deepest first
+  uint8_t lev[kChunk];          // level of each entry
   uint8_t local[kChunk];        // every unfinished descendant is in the chunk
   int16_t cidx[kChunk][4];      // chunk index of each cell child, -1 if none
   ChildRec rec[kChunk][4];
-  double mom[kChunk][kMom];     // moments of the entry once finished
+  double mom[kChunk][kMomL];    // moments of the entry once finished
   // moments of cell children finished before this chunk (EXT only)
-  double ext[EXT ? kChunk : 1][4][kMom];
+  double ext[EXT ? kChunk : 1][4][kMomL];
   int32_t wcnt[kComThreads / 64];
   int32_t wmin[kComThreads / 64];
   int32_t nout;
+  int32_t lvcnt[kMaxLevel + 1];  // local entries per level (levels 1..kMaxLevel)
+  int32_t lvoff[kMaxLevel + 1];  // their offset in bylev
+  uint8_t lvlist[kMaxLevel + 1]; // levels present, deepest first
+  int32_t nlev;
 };
 static_assert(kChunk == kComThreads, "one chunk entry per thread");
 static_assert(kChunk <= 32767, "chunk indices are int16");
@@ -83,68 +91,49 @@ __device__ __forceinline__ int find_id(const int32_t* ids, int lo, int hi, int32
   return -1;
 }
 
-// Moments of child slot k of entry i into registers: a body's (m, x, y), or
-// a cell's sums, finished in this chunk or, staged in L.ext, before it (never
-// in the first pass, EXT = false, whose local cells have their whole subtree
-// in the chunk).  NM = 3 or kMom values.
-template <int NM, bool EXT>
-__device__ __forceinline__ void child_mom(const ChunkLds<EXT>& L, int i, int k,
-                                          int32_t code, double (&q)[kMom]) {
-  if (code < 0) {
-    const ChildRec& r = L.rec[i][k];
-    q[0] = (double)r.m;
-    q[1] = (double)r.x;
-    q[2] = (double)r.y;
-#pragma unroll
-    for (int t = 3; t < NM; ++t) q[t] = 0.0;
-    return;
-  }
-  const int ci = L.cidx[i][k];
-  if (!EXT || ci >= 0) {
-#pragma unroll
-    for (int t = 0; t < NM; ++t) q[t] = L.mom[ci < 0 ? 0 : ci][t];
-  } else {
-#pragma unroll
-    for (int t = 0; t < NM; ++t) q[t] = L.ext[EXT ? i : 0][k][t];
-  }
-}
-
-// Moments of chunk entry i from its child slots (slot order, FP64), kept in
-// L.mom[i] and written to global memory.
+// Moments of entry i from its child slots (slot order, FP64) into L.mom[i]
+// (publish_cell writes them out).  Every slot is read the same way,
+// without branches: a cell child's moments come from L.mom (finished in this
+// chunk) or L.ext (before it), a body's are (m, m x, m y, 0, 0, 0, 0, x, y)
+// from its record, and an empty slot, whose record is all zeros, gives
+// zeros, which leave every sum unchanged (no sum here is ever -0).
 template <bool EXT>
 __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
   const int32_t b = L.ids[i];
+  double cm[4], cmx[4], cmy[4], cqxx[4], cqxy[4], cqyy[4], cbm[4], ccx[4], ccy[4];
+  bool used[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const ChildRec r = L.rec[i][k];
+    const int ci = L.cidx[i][k];
+    const double* src = (!EXT || ci >= 0) ? L.mom[ci < 0 ? 0 : ci] : L.ext[EXT ? i : 0][k];
+    const bool cell = r.code > 0;
+    used[k] = r.code != 0;
+    const double m = (double)r.m, x = (double)r.x, y = (double)r.y;
+    cm[k] = cell ? src[0] : m;
+    cmx[k] = cell ? src[1] : m * x;
+    cmy[k] = cell ? src[2] : m * y;
+    cqxx[k] = cell ? src[3] : 0.0;
+    cqxy[k] = cell ? src[4] : 0.0;
+    cqyy[k] = cell ? src[5] : 0.0;
+    cbm[k] = cell ? src[6] : 0.0;
+    ccx[k] = cell ? src[7] : x;
+    ccy[k] = cell ? src[8] : y;
+  }
   double sm = 0.0, sx = 0.0, sy = 0.0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int32_t code = L.rec[i][k].code;
-    if (code == 0) continue;
-    double q[kMom];
-    child_mom<3, EXT>(L, i, k, code, q);
-    if (code < 0) {
-      const double m = q[0];
-      sm += m;
-      sx += m * q[1];
-      sy += m * q[2];
-    } else {
-      sm += q[0];
-      sx += q[1];
-      sy += q[2];
-    }
+    sm += cm[k];
+    sx += cmx[k];
+    sy += cmy[k];
   }
-  double out[kMom];
+  double out[kMomL];
   out[0] = sm;
   out[1] = sx;
   out[2] = sy;
   // centre of mass as sums times the reciprocal mass (one FP64 division)
   const double inv = sm != 0.0 ? 1.0 / sm : 0.0;
   const double ox = sx * inv, oy = sy * inv;
-  const float fm = (float)sm;
-  const float fx = (float)ox;
-  const float fy = (float)oy;
-  const int32_t p = L.pblk[i];
-  const int s = L.slot[i];
-  A.blk[4 * (int64_t)p + s] = ChildRec{fx, fy, fm, b};
   double qxx = 0.0, qxy = 0.0, qyy = 0.0;
   if (A.quad) {
     // traceless in-plane quadrupole about the FP64 centre of mass with the
@@ -152,79 +141,53 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
     // restatement, so the FP32 results agree bit for bit)
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int32_t code = L.rec[i][k].code;
-      if (code == 0) continue;
-      double q[kMom];
-      child_mom<kMom, EXT>(L, i, k, code, q);
-      double cm, cx, cy;
-      if (code < 0) {
-        cm = q[0];
-        cx = q[1];
-        cy = q[2];
-      } else {
-        cm = q[0];
-        const double ci = q[0] != 0.0 ? 1.0 / q[0] : 0.0;
-        cx = q[1] * ci;
-        cy = q[2] * ci;
-      }
-      const double dx = cx - ox, dy = cy - oy;
+      const double dx = ccx[k] - ox, dy = ccy[k] - oy;
       const double t1 = dx * dx, t2 = dy * dy;
-      double axx = cm * (2.0 * t1 - t2);
-      double axy = cm * (3.0 * (dx * dy));
-      double ayy = cm * (2.0 * t2 - t1);
-      if (code > 0) {
-        axx = q[3] + axx;
-        axy = q[4] + axy;
-        ayy = q[5] + ayy;
-      }
-      qxx += axx;
-      qxy += axy;
-      qyy += ayy;
+      qxx += cqxx[k] + cm[k] * (2.0 * t1 - t2);
+      qxy += cqxy[k] + cm[k] * (3.0 * (dx * dy));
+      qyy += cqyy[k] + cm[k] * (2.0 * t2 - t1);
     }
   }
-  float thr = 0.f;
   double bm = 0.0;
   if (A.bmax) {
     // bmax bound: the largest distance from the FP64 centre of mass to a
     // body, through the children (bodies contribute their distance)
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int32_t code = L.rec[i][k].code;
-      if (code == 0) continue;
-      double q[kMom];
-      child_mom<kMom, EXT>(L, i, k, code, q);
-      double cx, cy, cb;
-      if (code < 0) {
-        cx = q[1];
-        cy = q[2];
-        cb = 0.0;
-      } else {
-        const double ci = q[0] != 0.0 ? 1.0 / q[0] : 0.0;
-        cx = q[1] * ci;
-        cy = q[2] * ci;
-        cb = q[6];
-      }
-      const double dx = cx - ox, dy = cy - oy;
-      const double d = sqrt(__builtin_fma(dx, dx, dy * dy)) + cb;
-      bm = d > bm ? d : bm;
+      const double dx = ccx[k] - ox, dy = ccy[k] - oy;
+      const double d = sqrt(__builtin_fma(dx, dx, dy * dy)) + cbm[k];
+      bm = used[k] && d > bm ? d : bm;
     }
-    thr = (float)((bm * bm) / A.t2 + A.eps2);
   }
-  if (A.quad || A.bmax)
-    A.qblk[4 * (int64_t)p + s] = make_float4((float)qxx, (float)qxy, (float)qyy, thr);
   out[3] = qxx;
   out[4] = qxy;
   out[5] = qyy;
   out[6] = bm;
-  // the global copy is read only by a parent that finishes in a later pass
-  // or chunk; a parent finished in this chunk takes the LDS copy
-  const int pi = L.pidx[i];
-  const bool keep = pi < 0 || !L.local[pi];
-  double* g = A.csum + kSumStride * (int64_t)b;
+  out[7] = ox;
+  out[8] = oy;
 #pragma unroll
-  for (int t = 0; t < kMom; ++t) {
-    if (keep) g[t] = out[t];
-    L.mom[i][t] = out[t];
+  for (int t = 0; t < kMomL; ++t) L.mom[i][t] = out[t];
+}
+
+// Writes finished entry i out, by the thread that staged it (block b, child
+// slot s of parent block p): its child record {CoM, m, b} in the parent's
+// block, {Qxx, Qxy, Qyy, bmax threshold} beside it, and its FP64 moments
+// when a parent finishing in a later pass or chunk will read them (a parent
+// finished in this chunk took the LDS copy).
+template <bool EXT>
+__device__ void publish_cell(const ComArgs& A, const ChunkLds<EXT>& L, int i, int32_t b,
+                             int32_t p, int s) {
+  const double* o = L.mom[i];
+  A.blk[4 * (int64_t)p + s] = ChildRec{(float)o[7], (float)o[8], (float)o[0], b};
+  if (A.quad || A.bmax) {
+    const float thr = A.bmax ? (float)((o[6] * o[6]) / A.t2 + A.eps2) : 0.f;
+    A.qblk[4 * (int64_t)p + s] = make_float4((float)o[3], (float)o[4], (float)o[5], thr);
+  }
+  const int pi = L.pidx[i];
+  if (pi < 0 || !L.local[pi]) {
+    double* g = A.csum + kSumStride * (int64_t)b;
+#pragma unroll
+    for (int t = 0; t < kMom; ++t) g[t] = o[t];
   }
 }
 
@@ -238,8 +201,8 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
 // unfinished descendants of entry i all lie in the chunk exactly when some
 // later entry of the chunk, or `next`, has level <= level(i).
 //
-// Everything a dependency chain needs is staged in LDS first, so the chains
-// issue global stores only and never wait on global memory.
+// Everything the finishing steps need is staged in LDS first, so they issue
+// global stores only and never wait on global memory.
 template <bool EXT>
 __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t* list,
                               int32_t p0, int32_t p1, int32_t next,
@@ -247,14 +210,17 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
   const int ne = p1 - p0;
   const int i = threadIdx.x;  // one entry per thread (kChunk == blockDim)
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int32_t pb = 0;  // parent block and slot of entry i
+  int sl = 0;
   {
     int lev = 255;
     if (i < ne) {
       const int32_t b = list ? list[p0 + i] : p0 + i + A.nroot;
       L.ids[i] = b;
       lev = A.clevel[b];
-      L.pblk[i] = A.cparent[b];
-      L.slot[i] = A.cslot[b];
+      L.lev[i] = (uint8_t)lev;
+      pb = A.cparent[b];
+      sl = A.cslot[b];
       const ChildRec* r = A.blk + 4 * (int64_t)b;
 #pragma unroll
       for (int k = 0; k < 4; ++k) L.rec[i][k] = r[k];
@@ -282,9 +248,8 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
     if (live) {
       // a contiguous chunk (the first pass) maps block ids to indices directly
       const int32_t base = p0 + A.nroot;
-      const int32_t pr = L.pblk[i] - base;
-      L.pidx[i] = list ? find_id(L.ids, 0, i, L.pblk[i]) : (pr >= 0 && pr < i ? pr : -1);
-      int pend = 0;
+      const int32_t pr = pb - base;
+      L.pidx[i] = list ? find_id(L.ids, 0, i, pb) : (pr >= 0 && pr < i ? pr : -1);
       if (L.local[i]) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -293,13 +258,16 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
           if (code > 0) {
             const int32_t cr = code - base;
             ci = list ? find_id(L.ids, i + 1, ne, code) : (cr > i && cr < ne ? cr : -1);
-            if (ci >= 0) {
-              ++pend;  // finished in this chunk
-            } else if (EXT) {
+            if (ci < 0 && EXT) {
               // finished by an earlier pass or a later chunk
               const double* g = A.csum + kSumStride * (int64_t)code;
+              double* e = L.ext[EXT ? i : 0][k];
 #pragma unroll
-              for (int t = 0; t < kMom; ++t) L.ext[EXT ? i : 0][k][t] = g[t];
+              for (int t = 0; t < kMom; ++t) e[t] = g[t];
+              // its centre of mass, as it computed it: sums times 1/m
+              const double inv = g[0] != 0.0 ? 1.0 / g[0] : 0.0;
+              e[7] = g[1] * inv;
+              e[8] = g[2] * inv;
             }
           }
           L.cidx[i][k] = (int16_t)ci;
@@ -307,7 +275,6 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
       } else {
         left = true;
       }
-      L.pend[i] = pend;
     }
     // leftovers keep ascending id order: rank = leftovers before entry i
     const uint64_t bal = __builtin_amdgcn_ballot_w64(left);
@@ -321,22 +288,49 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
       for (int w = 0; w < kComThreads / 64; ++w) tot += L.wcnt[w];
       L.nout = tot;
     }
+    for (int l = threadIdx.x; l <= kMaxLevel; l += kComThreads) L.lvcnt[l] = 0;
   }
   __syncthreads();
-  if (i < ne && L.local[i] && L.pend[i] == 0) {
-    int cur = i;
-    while (true) {
-      finish_cell<EXT>(A, L, cur);
-      const int p = L.pidx[cur];
-      if (p < 0 || !L.local[p]) break;  // the parent belongs to a later pass
-      __threadfence_block();
-      if (atomicSub(&L.pend[p], 1) != 1) break;
-      __threadfence_block();
-      cur = p;
+  // group the local entries by level (counting sort in LDS)
+  int rk = 0;
+  const bool fin = i < ne && L.local[i];
+  if (fin) rk = atomicAdd(&L.lvcnt[L.lev[i]], 1);
+  __syncthreads();
+  static_assert(kMaxLevel == 64, "one lane per cell level 1..64");
+  if (threadIdx.x < 64) {
+    // lane j holds level 64 - j (cells have levels 1..64): offsets are an
+    // exclusive scan in lane order, the present levels a ballot
+    const int l = kMaxLevel - lane;
+    const int c = L.lvcnt[l];
+    int incl = c;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(incl, off);
+      if (lane >= off) incl += o;
     }
+    L.lvoff[l] = incl - c;
+    const uint64_t present = __builtin_amdgcn_ballot_w64(c > 0);
+    if (c > 0) L.lvlist[__popcll(present & ((1ull << lane) - 1ull))] = (uint8_t)l;
+    if (lane == 0) L.nlev = __popcll(present);
   }
   __syncthreads();
+  if (fin) L.bylev[L.lvoff[L.lev[i]] + rk] = (int16_t)i;
+  __syncthreads();
+  // deepest level first: every child of a cell is then finished
+  const int nl = L.nlev;
+  for (int k = 0; k < nl; ++k) {
+    const int l = L.lvlist[k];
+    const int cnt = L.lvcnt[l], off = L.lvoff[l];
+    for (int t = threadIdx.x; t < cnt; t += kComThreads) finish_cell<EXT>(A, L, L.bylev[off + t]);
+    __syncthreads();
+  }
+  if (fin) publish_cell<EXT>(A, L, i, L.ids[i], pb, sl);
   if (threadIdx.x == 0) *out_n = L.nout < kSlots ? L.nout : kSlots;
+}
+
+// A later pass (and its compaction) is skipped once an earlier list fits in
+// one chunk: the last pass then takes that list (com_final_kernel).
+__device__ __forceinline__ bool list_done(const int32_t* a, const int32_t* b) {
+  return (a && *a <= kChunk) || (b && *b <= kChunk);
 }
 
 // Pass over list positions: one chunk per workgroup.  list == nullptr: all
@@ -344,9 +338,11 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
 template <bool EXT>
 __global__ __launch_bounds__(kComThreads) void com_pass_kernel(
     ComArgs A, const int32_t* __restrict__ list, const int32_t* __restrict__ len,
-    int32_t* __restrict__ out_slots, int32_t* __restrict__ out_counts) {
+    int32_t* __restrict__ out_slots, int32_t* __restrict__ out_counts,
+    const int32_t* __restrict__ skip_a, const int32_t* __restrict__ skip_b) {
   __shared__ ChunkLds<EXT> L;
   if (A.dev->err & kBuildErrors) return;  // no complete tree to sum
+  if (list_done(skip_a, skip_b)) return;
   const int32_t m = list ? *len : A.dev->ncells;
   const int32_t p0 = blockIdx.x * kChunk;
   if (p0 >= m) return;
@@ -363,9 +359,11 @@ __global__ __launch_bounds__(kComThreads) void com_pass_kernel(
 __global__ __launch_bounds__(1024) void com_offsets_kernel(
     const ComArgs A, const int32_t* __restrict__ len_in, int32_t nchunks_max,
     const int32_t* __restrict__ counts, int32_t* __restrict__ offs,
-    int32_t* __restrict__ len_out) {
+    int32_t* __restrict__ len_out, const int32_t* __restrict__ skip_a,
+    const int32_t* __restrict__ skip_b) {
   __shared__ int32_t part[1024];
   if (A.dev->err & kBuildErrors) return;
+  if (list_done(skip_a, skip_b)) return;
   const int32_t m = len_in ? *len_in : A.dev->ncells;
   int32_t nchunks = (m + kChunk - 1) / kChunk;
   if (nchunks > nchunks_max) nchunks = nchunks_max;
@@ -393,8 +391,10 @@ __global__ __launch_bounds__(1024) void com_offsets_kernel(
 __global__ __launch_bounds__(64) void com_gather_kernel(
     const ComArgs A, const int32_t* __restrict__ len_in,
     const int32_t* __restrict__ slots, const int32_t* __restrict__ counts,
-    const int32_t* __restrict__ offs, int32_t* __restrict__ list) {
+    const int32_t* __restrict__ offs, int32_t* __restrict__ list,
+    const int32_t* __restrict__ skip_a, const int32_t* __restrict__ skip_b) {
   if (A.dev->err & kBuildErrors) return;
+  if (list_done(skip_a, skip_b)) return;
   const int32_t m = len_in ? *len_in : A.dev->ncells;
   const int32_t c = blockIdx.x;
   if (c * kChunk >= m) return;
@@ -404,12 +404,17 @@ __global__ __launch_bounds__(64) void com_gather_kernel(
 
 // Last pass: one workgroup finishes the whole list, chunks from last to
 // first (every descendant of a chunk's cells is then in the chunk or done).
+// The list is the first of the pass-1, pass-2 leftovers that fits in one
+// chunk (the passes after it were skipped), else the pass-3 leftovers.
 __global__ __launch_bounds__(kComThreads) void com_final_kernel(
-    ComArgs A, const int32_t* __restrict__ list, const int32_t* __restrict__ len,
-    int32_t* __restrict__ scratch_slots, int32_t* __restrict__ scratch_count) {
+    ComArgs A, const int32_t* __restrict__ l0, const int32_t* __restrict__ l1,
+    const int32_t* __restrict__ lens, int32_t* __restrict__ scratch_slots,
+    int32_t* __restrict__ scratch_count) {
   __shared__ ChunkLds<true> L;
   if (A.dev->err & kBuildErrors) return;
-  const int32_t m = *len;
+  const bool short0 = lens[0] <= kChunk, short1 = lens[1] <= kChunk;
+  const int32_t* list = short0 ? l0 : short1 ? l1 : l0;
+  const int32_t m = short0 ? lens[0] : short1 ? lens[1] : lens[2];
   const int32_t nch = (m + kChunk - 1) / kChunk;
   for (int32_t c = nch - 1; c >= 0; --c) {
     const int32_t p0 = c * kChunk;
@@ -442,17 +447,26 @@ void launch_com(stellar_ctx* c) {
   int32_t* l0 = B.clist;
   int32_t* l1 = B.clist + com_list_capacity(c->cap);
   hipStream_t s = c->stream;
-  auto compact = [&](const int32_t* len_in, int64_t ch, int32_t* list, int32_t* len_out) {
-    com_offsets_kernel<<<1, 1024, 0, s>>>(A, len_in, (int32_t)ch, B.scount, B.choff, len_out);
-    com_gather_kernel<<<(unsigned)ch, 64, 0, s>>>(A, len_in, B.spine, B.scount, B.choff, list);
+  // a pass runs only while every list before it is longer than one chunk
+  auto compact = [&](const int32_t* len_in, int64_t ch, int32_t* list, int32_t* len_out,
+                     const int32_t* ska, const int32_t* skb) {
+    com_offsets_kernel<<<1, 1024, 0, s>>>(A, len_in, (int32_t)ch, B.scount, B.choff, len_out,
+                                          ska, skb);
+    com_gather_kernel<<<(unsigned)ch, 64, 0, s>>>(A, len_in, B.spine, B.scount, B.choff, list,
+                                                  ska, skb);
   };
-  com_pass_kernel<false><<<(unsigned)ch1, kComThreads, 0, s>>>(A, nullptr, nullptr, B.spine, B.scount);
-  compact(nullptr, ch1, l0, B.clen + 0);
-  com_pass_kernel<true><<<(unsigned)ch2, kComThreads, 0, s>>>(A, l0, B.clen + 0, B.spine, B.scount);
-  compact(B.clen + 0, ch2, l1, B.clen + 1);
-  com_pass_kernel<true><<<(unsigned)ch3, kComThreads, 0, s>>>(A, l1, B.clen + 1, B.spine, B.scount);
-  compact(B.clen + 1, ch3, l0, B.clen + 2);
-  com_final_kernel<<<1, kComThreads, 0, s>>>(A, l0, B.clen + 2, B.spine, B.scount);
+  const int32_t* c0 = B.clen + 0;
+  const int32_t* c1 = B.clen + 1;
+  com_pass_kernel<false><<<(unsigned)ch1, kComThreads, 0, s>>>(A, nullptr, nullptr, B.spine,
+                                                               B.scount, nullptr, nullptr);
+  compact(nullptr, ch1, l0, B.clen + 0, nullptr, nullptr);
+  com_pass_kernel<true><<<(unsigned)ch2, kComThreads, 0, s>>>(A, l0, c0, B.spine, B.scount,
+                                                              c0, nullptr);
+  compact(c0, ch2, l1, B.clen + 1, c0, nullptr);
+  com_pass_kernel<true><<<(unsigned)ch3, kComThreads, 0, s>>>(A, l1, c1, B.spine, B.scount,
+                                                              c0, c1);
+  compact(c1, ch3, l0, B.clen + 2, c0, c1);
+  com_final_kernel<<<1, kComThreads, 0, s>>>(A, l0, l1, B.clen, B.spine, B.scount);
 }
 
 }  // namespace stellar
