@@ -38,6 +38,10 @@ HIPFLAGS = [
     "-fPIC",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # branches on wavefront-uniform conditions (the tree walk's per-child
+    # tests) stay plain scalar branches instead of being restructured with
+    # exec-mask bookkeeping
+    "-mllvm", "-structurizecfg-skip-uniform-regions=true",
     "-Wall",
     "-Wno-unused-result",
     f"-I{INCLUDE}",
