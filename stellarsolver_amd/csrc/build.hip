@@ -424,7 +424,21 @@ struct LinkArgs {
   const int32_t* ioff;
   int32_t nroot;  // number of instances = root blocks; cell c is block nroot + c
   MinTree mt;
+  // second block per node (quadrupoles / bmax): child slot thresholds, a
+  // cell's level threshold (the bmax pass replaces it), -1 for a body, which
+  // every body of a group accepts
+  float4* qblk;
+  int qthr;
+  const float4* ibbox;
+  float theta_div;
+  float eps2;
+  int unsoft;
 };
+
+__device__ __forceinline__ void set_cell_threshold(const LinkArgs& A, int64_t slot_index,
+                                                   float t) {
+  reinterpret_cast<float*>(A.qblk + slot_index)[3] = A.unsoft ? t + A.eps2 : t;
+}
 
 // One thread per adjacent pair s: creates the cells that pair opens, at
 // levels lam(s-1)+1 .. lam(s), with ids coff[s] + k, and links each cell but
@@ -451,6 +465,10 @@ __global__ __launch_bounds__(256) void cells_kernel(LinkArgs A) {
   if (nc == 0) return;
   const int lp = s > 0 ? A.lam[s - 1] : 0;
   const int64_t base = A.coff[s];
+  // level threshold (before eps2) of the cell at level lp + 1 + k, k = 1 ...
+  float lt = 0.f;
+  if (A.qthr && nc > 1) lt = walk_level_threshold(walk_root_depth(A.ibbox[inst], A.theta_div),
+                                                  lp + 1, 0.f, false);
   for (int k = 0; k < nc; ++k) {
     const int64_t c = base + k;
     if (c >= A.cap) break;
@@ -470,6 +488,10 @@ __global__ __launch_bounds__(256) void cells_kernel(LinkArgs A) {
     if (k > 0) {
       A.cparent[b] = b - 1;
       A.blk[4 * (int64_t)(b - 1) + dg] = ChildRec{0.f, 0.f, 0.f, b};
+      if (A.qthr) {
+        lt = 0.25f * lt;
+        set_cell_threshold(A, 4 * (int64_t)(b - 1) + dg, lt);
+      }
     }
   }
 }
@@ -501,6 +523,10 @@ __global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
       A.cparent[b] = pblk;
       const int dg = digit_at(A.K, s, lp + 1);
       A.blk[4 * (int64_t)pblk + dg] = ChildRec{0.f, 0.f, 0.f, b};
+      if (A.qthr)
+        set_cell_threshold(A, 4 * (int64_t)pblk + dg,
+                           walk_level_threshold(walk_root_depth(A.ibbox[A.inst_of[s]], A.theta_div),
+                                                lp + 1, 0.f, false));
     }
   }
   int64_t bid;  // block of body s's parent
@@ -526,6 +552,7 @@ __global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
   }
   const uint32_t p = A.perm[s];
   A.blk[4 * bid + slot] = ChildRec{A.x[p], A.y[p], A.m[p], -1 - s};
+  if (A.qthr) A.qblk[4 * bid + slot] = make_float4(0.f, 0.f, 0.f, -1.f);
 }
 
 inline unsigned grid_for(int64_t items, int threads) {
@@ -604,10 +631,15 @@ void launch_build(stellar_ctx* c) {
   bytes = B.tmp_bytes;
   (void)rocprim::exclusive_scan(B.tmp, bytes, B.ncell, B.coff, 0, (size_t)n,
                                 rocprim::plus<int32_t>(), c->stream);
+  const float theta = c->p.theta;
+  const bool textbook = (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) != 0;
   LinkArgs A{n,         c->cap,   K,       B.coff,    B.ncell,  B.lam,
              B.perm,    B.x,      B.y,     B.m,       B.blk,    B.cstart,
              B.cparent, B.clevel, B.cslot, B.dev,     B.inst_of, B.ioff,
-             c->ninst,  mt};
+             c->ninst,  mt,
+             B.qblk,
+             (c->p.flags & (STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_BMAX_OPENING)) ? 1 : 0,
+             B.ibbox,   textbook ? theta * theta : theta, c->p.eps2, textbook ? 1 : 0};
   cells_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(A);
   link_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(A);
 }

@@ -171,7 +171,8 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
 
 // Writes finished entry i out, by the thread that staged it (block b, child
 // slot s of parent block p): its child record {CoM, m, b} in the parent's
-// block, {Qxx, Qxy, Qyy, bmax threshold} beside it, and its FP64 moments
+// block, {Qxx, Qxy, Qyy} beside it (with the bmax threshold in bmax mode;
+// otherwise the level threshold the build put there stays), and its FP64 moments
 // when a parent finishing in a later pass or chunk will read them (a parent
 // finished in this chunk took the LDS copy).
 template <bool EXT>
@@ -179,9 +180,15 @@ __device__ void publish_cell(const ComArgs& A, const ChunkLds<EXT>& L, int i, in
                              int32_t p, int s) {
   const double* o = L.mom[i];
   A.blk[4 * (int64_t)p + s] = ChildRec{(float)o[7], (float)o[8], (float)o[0], b};
-  if (A.quad || A.bmax) {
-    const float thr = A.bmax ? (float)((o[6] * o[6]) / A.t2 + A.eps2) : 0.f;
+  if (A.bmax) {
+    const float thr = (float)((o[6] * o[6]) / A.t2 + A.eps2);
     A.qblk[4 * (int64_t)p + s] = make_float4((float)o[3], (float)o[4], (float)o[5], thr);
+  } else if (A.quad) {
+    // .w holds the cell's level threshold, written by the build
+    float* q = reinterpret_cast<float*>(A.qblk + 4 * (int64_t)p + s);
+    q[0] = (float)o[3];
+    q[1] = (float)o[4];
+    q[2] = (float)o[5];
   }
   const int pi = L.pidx[i];
   if (pi < 0 || !L.local[pi]) {

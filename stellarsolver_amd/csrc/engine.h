@@ -28,6 +28,26 @@ constexpr int kMaxTieRun = 256;
 constexpr int kWalkStack = 256;  // >= 4 + 3*(kMaxLevel-1)
 constexpr int kWalkBlock = 256;  // threads per walk workgroup (4 wavefronts)
 
+// Opening thresholds of the group walk (kernels.cu:349,376,394-395): the
+// root's children are pushed with depth0 = radius^2/theta_div, radius = half
+// the x extent of the instance's box, and each level down multiplies it by
+// 0.25, so a cell at level L >= 2 is accepted by a body when
+// depth0 * 0.25^(L-1) <= d^2 + eps2 (theta_div = theta; theta^2 for the
+// textbook test, which compares the unsoftened distance: threshold + eps2).
+// The build and the centre-of-mass pass store this value beside each child
+// record for the walks that read a second block per node; the arithmetic is
+// the walk's own, so the values are the same bits.
+__device__ __forceinline__ float walk_root_depth(float4 bb, float theta_div) {
+  const float radius = (float)(0.5 * (double)(bb.y - bb.x));
+  return (float)((double)(radius * radius) / (double)theta_div);
+}
+__device__ __forceinline__ float walk_level_threshold(float depth0, int level, float eps2,
+                                                      bool unsoft) {
+  float thr = depth0;
+  for (int i = 1; i < level; ++i) thr = 0.25f * thr;
+  return unsoft ? thr + eps2 : thr;
+}
+
 // MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs
 // (workgroup b to XCD b % 8), each XCD with its own L2.  xcd_block maps
 // hardware workgroup ids to logical ones so that each XCD gets one

@@ -63,15 +63,6 @@ __device__ __forceinline__ float dist2(float dx, float dy, float eps2) {
   return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, eps2));
 }
 
-// depth = radius^2/theta with radius = half the x extent of the instance's
-// box (kernels.cu:349,376)
-__device__ __forceinline__ float root_depth(const WalkArgs& A, int32_t inst) {
-  const float4 bb = A.ibbox[inst];
-  const float bl = bb.x, br = bb.y;
-  const float radius = (float)(0.5 * (double)(br - bl));
-  return (float)((double)(radius * radius) / (double)A.theta_div);
-}
-
 struct Acc {
   float fx = 0.f, fy = 0.f;
   int32_t cnt = 0, qcnt = 0;
@@ -158,14 +149,19 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
   const float px = A.x[sl];
   const float py = A.y[sl];
 
+  // A node's second block (quadrupoles, bmax) carries each child's threshold
+  // (a cell's level threshold or bmax threshold, -1 for a body); otherwise
   // thr(level) = threshold for the children of a node at `level`:
   // 0.25*depth(level), depth(1) = depth0, depth(l+1) = 0.25*depth(l)
   // (kernels.cu:394-395), one level per lane.
-  const float depth0 = root_depth(A, inst);
-  float thr = depth0;
-  for (int i = 0; i <= lane; ++i) thr = 0.25f * thr;
-  // textbook test on the unsoftened distance: dp + eps2 <= d^2 + eps2
-  if (A.unsoft) thr = thr + eps2;
+  constexpr bool kQ = QUAD || THR;  // the node's second block is needed
+  float thr = 0.f;
+  if (!kQ) {
+    thr = walk_root_depth(A.ibbox[inst], A.theta_div);
+    for (int i = 0; i <= lane; ++i) thr = 0.25f * thr;
+    // textbook test on the unsoftened distance: dp + eps2 <= d^2 + eps2
+    if (A.unsoft) thr = thr + eps2;
+  }
 
   int sp = 0;
   Acc acc;
@@ -197,10 +193,9 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
           __builtin_amdgcn_readfirstlane(st[sp + (j < take ? j : 0)]);
       node[j] = w & kIdxMask;
       lvl[j] = w >> kIdxBits;  // level - 1
-      dp[j] = __int_as_float(
-          __builtin_amdgcn_readlane(__float_as_int(thr), (int)lvl[j]));
+      dp[j] = kQ ? 0.f
+                 : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr), (int)lvl[j]));
     }
-    constexpr bool kQ = QUAD || THR;  // the node's second block is needed
     int4 q[KPOP][4];
     float4 qm[kQ ? KPOP : 1][4];
 #pragma unroll
@@ -238,8 +233,8 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
         const float r = dist2(dx, dy, eps2);
         // one vote for bodies and cells: a body always passes (threshold -1
         // against r >= eps2 > 0), so acceptance is a single scalar test
-        const float cthr = THR ? qm[kQ ? jj : 0][k].w : dp[jj];
-        const int32_t ti = code < 0 ? __float_as_int(-1.0f) : __float_as_int(cthr);
+        const int32_t ti = kQ ? __float_as_int(qm[kQ ? jj : 0][k].w)
+                              : (code < 0 ? __float_as_int(-1.0f) : __float_as_int(dp[jj]));
         const float t = __int_as_float(__builtin_amdgcn_readfirstlane(ti));
         if (__builtin_expect(__builtin_amdgcn_ballot_w64(!(t <= r)) == 0, 1)) {
           acc.term(QUAD && code > 0, cmv, r, dx, dy, qm[kQ ? jj : 0][k]);
@@ -313,7 +308,7 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
   const float px = valid ? A.x[s] : 0.f;
   const float py = valid ? A.y[s] : 0.f;
   const int32_t inst = gvalid ? A.inst_of[s0] : 0;
-  const float depth0 = root_depth(A, inst);
+  const float depth0 = walk_root_depth(A.ibbox[inst], A.theta_div);
   const uint64_t gmask = ((G >= 64) ? ~0ull : ((1ull << (G & 63)) - 1ull)) << (sub * G);
 
   Acc acc;
