@@ -162,6 +162,16 @@ def force_error(eng, n, sample):
     return rms_rel(f["ax"][tg], f["ay"][tg], dx, dy), int(tg.size)
 
 
+def stage_gbs(stages, n, cells):
+    """Algorithmic HBM bytes of each memory-bound stage per step (N bodies,
+    C cells: bbox 8N; build 8N + 20C; centre of mass 12N + 12C; sort 4N + 24C
+    + 24N for the bodies moved into tree order; update 40N) over its measured
+    time, in GB/s: what the stage moves at least, not what it fetches."""
+    bytes_ = {"bbox": 8 * n, "build": 8 * n + 20 * cells, "com": 12 * n + 12 * cells,
+              "sort": 4 * n + 24 * cells + 24 * n, "update": 40 * n}
+    return {k: b / (stages[k] * 1e-3) / 1e9 for k, b in bytes_.items() if stages.get(k)}
+
+
 def synced_diagnostics(eng, dt, forces):
     """FP64 energy, momentum and angular momentum with velocities brought
     level with the positions (v + dt/2 a), summed over instances; forces=True
@@ -404,6 +414,7 @@ def main():
         "force_err_max": err_max,
         "force_err_sample": nsample,
         "stage_ms": stages,
+        "stage_algorithmic_gbs": stage_gbs(stages, n, ncells),
         "cells": ncells,
         "interactions_per_step": inter,
         "interactions_per_body": inter / n,

@@ -298,6 +298,32 @@ def test_quadrupole_parity(S, oracle, workload, n, textbook):
     assert rms_rel(f["ax"], f["ay"], ax, ay) < FORCE_RMS_TOL
 
 
+@pytest.mark.parametrize("theta", [0.3, 0.8])
+@pytest.mark.parametrize("model", ["reference", "textbook+quad"])
+def test_theta_parity(S, oracle, theta, model):
+    """Opening angles 0.3 and 0.8 (the colliding-disks runs at theta in
+    {0.3, 0.5, 0.8}): exact interaction counts and forces within the rsqrt
+    tolerance, for the reference's monopole walk and the quadrupole walk with
+    the textbook test (whose thresholds the build stores per child)."""
+    print(f"theta parity theta={theta} model={model}", flush=True)
+    st = ic(2, 65536)
+    eps2 = float(np.float32(EPS[2] ** 2))
+    quad = model != "reference"
+    flags = (S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING) if quad else 0
+    with make_engine(S, st, eps2, theta=theta, flags=flags) as e:
+        gpu_tree(e)
+        e.sort()
+        e.forces()
+        f = e.export_forces()
+    obb, ot = oracle_tree(oracle, st)
+    if quad:
+        oracle.quad_exact(ot, st["x"], st["y"], st["mass"])
+    ax, ay, it = oracle.walk(ot, st["x"], st["y"], st["mass"], obb, theta, eps2, 64,
+                             opening=int(quad), quad=quad)
+    assert np.array_equal(f["inter"], it)
+    assert rms_rel(f["ax"], f["ay"], ax, ay) < FORCE_RMS_TOL
+
+
 def test_quadrupole_headline_error(S):
     """N = 2^20 colliding disks with quadrupoles: the force error against the
     FP64 direct sum on a seeded 8192-body subsample."""
