@@ -14,8 +14,11 @@
 //   * the vote is one 64-lane ballot; lanes past the last body repeat the
 //     last body, so their votes change nothing and need no mask;
 //   * the stack (in LDS, one region per wavefront, written by lane 0) holds
-//     node index and level in one word; the threshold dp of a level comes
-//     from a per-lane table with one v_readlane;
+//     node index and level in one word; with a second block per node
+//     (quadrupoles, bmax) each child's threshold comes with it, otherwise
+//     the threshold dp of a level comes from a per-lane table (v_readlane);
+//   * the eight wavefronts of a workgroup take consecutive groups and each
+//     XCD one contiguous range of workgroups (scalar-cache and L2 reuse);
 //   * bodies are stored in tree order: the group's positions and outputs are
 //     contiguous 256-byte rows.
 // G = 32 and 16 (walk_kernel) run 2 or 4 independent groups per wavefront
@@ -29,7 +32,7 @@
 // children (kernels.cu:373-387, 397-398), so that body exerts no force.
 //
 // QUAD adds each accepted cell's quadrupole term (a second 64-byte block of
-// Qxx, Qxy, Qyy per node); bodies stay monopoles.
+// {Qxx, Qxy, Qyy, threshold} per node); bodies stay monopoles.
 #include "engine.h"
 
 namespace stellar {
