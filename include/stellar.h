@@ -214,6 +214,11 @@ typedef struct stellar_tuning {
  * stellar_sort done).  out may be NULL. */
 int stellar_autotune(stellar_ctx* ctx, int32_t group_mask, int32_t reps,
                      stellar_tuning* out);
+/* Testing hook, no counterpart in the reference: caps the keys a bucket of
+ * the key sort sorts in one block pass (1 .. 2048, default 2048), so that tests
+ * can send every bucket through the multi-pass path that larger buckets take. */
+int stellar_debug_sort_cap(stellar_ctx* ctx, int32_t cap);
+
 /* Set the group-64 walk launch shape by hand: nodes_per_pop 1, 2 or 4
  * (4 is capped to 2 when a node's second block is read), block_threads 64,
  * 128, 256, 512 or 1024. */

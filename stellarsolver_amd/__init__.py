@@ -138,6 +138,7 @@ SIGNATURES = {
     "stellar_diagnostics_fp64": (C.c_int, [_P, C.c_double, C.c_double, C.POINTER(StellarDiagnostics)]),
     "stellar_autotune": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(StellarTuning)]),
     "stellar_set_walk_shape": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "stellar_debug_sort_cap": (C.c_int, [_P, C.c_int32]),
     "stellar_ic_defaults": (C.c_int, [C.c_int32, C.POINTER(StellarICParams)]),
     "stellar_generate_ic": (C.c_int, [C.POINTER(StellarICParams), C.c_int32, _F, _F, _F, _F, _F]),
 }

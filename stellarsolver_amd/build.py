@@ -48,7 +48,7 @@ HIPFLAGS = [
     f"-I{CSRC}",
 ]
 
-LIB_SOURCES = ["capi.hip", "build.hip", "com.hip", "walk.hip", "body.hip"]
+LIB_SOURCES = ["capi.hip", "build.hip", "sort.hip", "com.hip", "walk.hip", "body.hip"]
 HOST_LIB_SOURCES = ["ic.cpp"]
 APP_SOURCES = ["BarnesHutParticleSystem.cpp", "main.cpp"]
 

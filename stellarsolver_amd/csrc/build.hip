@@ -574,6 +574,8 @@ size_t build_scratch_bytes(int32_t n) {
   (void)rocprim::exclusive_scan(nullptr, b, q, q, 0, (size_t)n,
                                 rocprim::plus<int32_t>());
   if (a2 > a) a = a2;
+  const size_t c = sample_sort_scan_bytes(n);
+  if (c > a) a = c;
   return (a > b ? a : b) + 256;
 }
 
@@ -606,17 +608,21 @@ void launch_build(stellar_ctx* c) {
   const int l1 = 32 - ibits / 2;
   keys_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, n, B.inst_of, B.ibbox,
                                                         l1, ibits, B.key, B.perm);
-  rocprim::double_buffer<uint64_t> kb(B.key, B.key_alt);
-  rocprim::double_buffer<uint32_t> vb(B.perm, B.perm_alt);
   size_t bytes = B.tmp_bytes;
-  if (c->capturing)
-    (void)rocprim::radix_sort_pairs<MergeSortConfig>(B.tmp, bytes, kb, vb, (size_t)n, 0, 64,
-                                                     c->stream);
-  else
-    (void)rocprim::radix_sort_pairs(B.tmp, bytes, kb, vb, (size_t)n, 0, 64, c->stream);
-  // keep the sorted data in the primary buffers
-  if (kb.current() != B.key) std::swap(B.key, B.key_alt);
-  if (vb.current() != B.perm) std::swap(B.perm, B.perm_alt);
+  if (sample_sort_fits(n)) {
+    launch_sample_sort(c);  // sorted data back in (key, perm)
+  } else {
+    rocprim::double_buffer<uint64_t> kb(B.key, B.key_alt);
+    rocprim::double_buffer<uint32_t> vb(B.perm, B.perm_alt);
+    if (c->capturing)
+      (void)rocprim::radix_sort_pairs<MergeSortConfig>(B.tmp, bytes, kb, vb, (size_t)n, 0, 64,
+                                                       c->stream);
+    else
+      (void)rocprim::radix_sort_pairs(B.tmp, bytes, kb, vb, (size_t)n, 0, 64, c->stream);
+    // keep the sorted data in the primary buffers
+    if (kb.current() != B.key) std::swap(B.key, B.key_alt);
+    if (vb.current() != B.perm) std::swap(B.perm, B.perm_alt);
+  }
   ties_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, B.id, n, B.key, B.inst_of,
                                                         B.ibbox, l1, B.perm, B.key2, B.key3,
                                                         B.dev);

@@ -76,7 +76,7 @@ void free_all(stellar_ctx* c) {
                   B.bbox_part, B.scount, B.choff,
                   B.lmin,  B.qblk,  B.spine,   B.clist, B.clen,
                   B.key3,  B.ioff,  B.inst_of, B.ibbox, B.tstart, B.itile,
-                  B.gtab[0], B.gtab[1], B.gtab[2]};
+                  B.gtab[0], B.gtab[1], B.gtab[2], B.ss_spl, B.ss_hist, B.ss_off};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (B.host_dev) (void)hipHostFree(B.host_dev);
@@ -328,6 +328,12 @@ int create_ctx(const stellar_params* p, int32_t ninst, const int32_t* counts,
   chk(dmalloc(&B.key_alt, N));
   chk(dmalloc(&B.perm, N));
   chk(dmalloc(&B.perm_alt, N));
+  if (sample_sort_fits((int32_t)N)) {
+    const size_t hb = (size_t)sample_sort_buckets((int32_t)N) * (size_t)sample_sort_tiles((int32_t)N);
+    chk(dmalloc(&B.ss_spl, (size_t)kSsMaxBuckets));
+    chk(dmalloc(&B.ss_hist, hb));
+    chk(dmalloc(&B.ss_off, hb));
+  }
   chk(dmalloc(&B.key2, N));
   chk(dmalloc(&B.key3, N));
   chk(dmalloc(&B.ncell, N));
@@ -475,6 +481,7 @@ int stellar_upload(stellar_ctx* c, const float* mass, const float* x,
   c->uploaded = true;
   c->stage = kNone;
   c->sorted_storage = false;
+  c->ss_have_spl = false;  // the key sort samples the new state
   return STELLAR_OK;
 }
 
@@ -933,6 +940,14 @@ int stellar_energy_fp64(stellar_ctx* c, double eps2, double* kinetic,
   if (potential) *potential = tu;
   if (px) *px = tx;
   if (py) *py = ty;
+  return STELLAR_OK;
+}
+
+int stellar_debug_sort_cap(stellar_ctx* c, int32_t cap) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (cap < 1 || cap > kSsCapMax) return fail(c, STELLAR_ERR_ARG, "sort cap");
+  c->sort_cap = cap;
+  drop_graphs(c);
   return STELLAR_OK;
 }
 

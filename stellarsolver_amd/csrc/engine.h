@@ -25,6 +25,12 @@ constexpr int kMaxLevel = STELLAR_MAX_LEVEL;
 constexpr int kBboxMaxBlocks = 1024;
 constexpr int64_t kMaxBlocks = int64_t(1) << 26;  // root + cells
 constexpr int kMaxTieRun = 256;
+// Key sort (sort.hip): sample sort for up to kSsMaxN bodies, buckets of at
+// most kSsCapMax keys sorted in one LDS pass.
+constexpr int64_t kSsMaxN = (int64_t)1 << 22;
+constexpr int kSsBucket = 1536;  // target keys per bucket
+constexpr int kSsMaxBuckets = (int)((kSsMaxN + kSsBucket - 1) / kSsBucket);
+constexpr int kSsCapMax = 2048;
 constexpr int kWalkStack = 256;  // >= 4 + 3*(kMaxLevel-1)
 constexpr int kWalkBlock = 256;  // threads per walk workgroup (4 wavefronts)
 
@@ -109,6 +115,9 @@ struct Buffers {
   // build
   uint64_t *key = nullptr, *key_alt = nullptr;  // level 1..32 codes
   uint32_t *perm = nullptr, *perm_alt = nullptr;
+  uint64_t* ss_spl = nullptr;   // sample sort: splitters
+  uint32_t* ss_hist = nullptr;  // sample sort: per-workgroup bucket counts
+  uint32_t* ss_off = nullptr;   // their exclusive scan
   uint64_t* key2 = nullptr;   // 32 deeper levels (valid inside tie runs)
   uint32_t* key3 = nullptr;   // the deepest levels when the key holds instances
   int32_t* ncell = nullptr;   // cells opened by each adjacent pair
@@ -185,6 +194,8 @@ struct stellar_ctx {
   // wavefronts per workgroup (stellar_autotune)
   int walk_kpop = 0;   // 0: default (2 one-block nodes, 1 two-block nodes)
   int walk_waves = 0;  // 0: default (8)
+  int32_t sort_cap = stellar::kSsCapMax;  // keys per block pass of a sort bucket
+  bool ss_have_spl = false;  // splitters from the previous sort are in place
   // per-step stage events of the current stellar_run (8 per step)
   std::vector<hipEvent_t> evrun;
   int32_t evrun_steps = 0;
@@ -200,6 +211,12 @@ void launch_build(stellar_ctx* c);
 void launch_com(stellar_ctx* c);
 void launch_sort(stellar_ctx* c);
 void launch_forces(stellar_ctx* c, bool zero_counter);
+// Key sort (sort.hip).
+bool sample_sort_fits(int32_t n);
+int32_t sample_sort_buckets(int32_t n);
+int32_t sample_sort_tiles(int32_t n);
+size_t sample_sort_scan_bytes(int32_t n);
+void launch_sample_sort(stellar_ctx* c);
 void launch_update(stellar_ctx* c);
 void launch_output(stellar_ctx* c);
 void launch_zero_counter(stellar_ctx* c);

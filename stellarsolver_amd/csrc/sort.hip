@@ -1,0 +1,307 @@
+// Sort of the bodies' path keys for gfx950: a sample sort that exploits the
+// order the bodies are already in.
+//
+// Between steps the bodies are stored in the previous step's tree order, so
+// the new keys are nearly sorted: half the bodies move fewer than 16
+// positions, 99% fewer than ~2000, and a few tenths of a percent jump far
+// (they crossed a high-level cell boundary, or the box changed).  A general
+// merge or radix sort pays full price for that; here
+//   1. the splitters (nb - 1 keys cutting buckets of about kSsBucket keys)
+//      are the previous sort's quantiles: the sorted keys at every
+//      (n/nb)-th position, read off after each sort.  The keys move a
+//      little between steps, so the buckets stay balanced; after an upload
+//      one workgroup sorts a strided sample of the keys (up to 4 per
+//      bucket) instead.  Splitters only balance the buckets: any sorted set
+//      gives the same result;
+//   2. every key finds its bucket by binary search over the splitters (in
+//      LDS), and per-workgroup bucket counts are scanned;
+//   3. keys and values are scattered to their buckets;
+//   4. one workgroup per bucket sorts it (a block radix sort, stable, with
+//      the bucket's keys in their index order ahead of the padding) and writes
+//      the final order.
+// Equal keys always land in the same bucket (the bucket is the number of
+// splitters below the key), so runs of equal keys end up contiguous, which is
+// all the later tie refinement needs (the order inside a run is not used).
+// A bucket larger than one block sort (cap keys) is sorted in runs that one
+// workgroup then merges (merge path) through the spare buffers: correct for
+// any input, and only reached when the sample misjudges the distribution
+// badly (a first step from an arbitrary body order, say).
+#include <rocprim/block/block_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "engine.h"
+
+namespace stellar {
+
+namespace {
+
+constexpr int kSsOver = 4;              // samples per bucket
+constexpr int kSsMaxSamples = 8192;     // at most (fewer per bucket beyond 2048 buckets)
+constexpr int kSsThreads = 256;       // threads of the sample, bucket and merge workgroups
+constexpr int kSsTileThreads = 1024;  // threads of the counting / scattering workgroups
+constexpr int kSsTile = 4096;     // keys per counting / scattering workgroup
+constexpr int kSsRunItems = kSsCapMax / kSsThreads;         // 8 per thread
+constexpr int kSsSampleItems = kSsMaxSamples / kSsThreads;  // 32 per thread
+template <int IPT>
+using RunSort = rocprim::block_radix_sort<uint64_t, kSsThreads, IPT, uint32_t>;
+template <int IPT>
+using SampleSort = rocprim::block_radix_sort<uint64_t, kSsThreads, IPT>;
+template <class A, class B>
+constexpr size_t max_size() {
+  return sizeof(A) > sizeof(B) ? sizeof(A) : sizeof(B);
+}
+
+// Bucket of a key: the number of splitters below it (0 .. nb-1).
+__device__ __forceinline__ int bucket_of(const uint64_t* sp, int nb, uint64_t key) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (sp[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// 1. strided sample -> sorted -> splitters spl[0 .. nb-2].  The block sort
+// holds IPT samples per thread (the smallest of 2, 4, 8 that fits).
+template <int IPT>
+__device__ void sort_samples(const uint64_t* key, int32_t n, int s, uint64_t* sorted,
+                             void* storage) {
+  uint64_t k[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = threadIdx.x * IPT + j;  // blocked arrangement
+    k[j] = i < s ? key[(int64_t)i * n / s] : ~0ull;
+  }
+  SampleSort<IPT>().sort(k, *reinterpret_cast<typename SampleSort<IPT>::storage_type*>(storage));
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) sorted[threadIdx.x * IPT + j] = k[j];
+}
+
+__global__ __launch_bounds__(kSsThreads) void ss_sample_kernel(const uint64_t* __restrict__ key,
+                                                               int32_t n, int32_t nb,
+                                                               uint64_t* __restrict__ spl) {
+  __shared__ __align__(16) unsigned char store[max_size<typename SampleSort<kSsSampleItems>::storage_type,
+                                                        typename SampleSort<8>::storage_type>()];
+  __shared__ uint64_t sorted[kSsMaxSamples];
+  const int s = nb * kSsOver < kSsMaxSamples ? nb * kSsOver : kSsMaxSamples;
+  if (s <= 8 * kSsThreads)
+    sort_samples<8>(key, n, s, sorted, store);
+  else if (s <= 16 * kSsThreads)
+    sort_samples<16>(key, n, s, sorted, store);
+  else
+    sort_samples<kSsSampleItems>(key, n, s, sorted, store);
+  __syncthreads();
+  for (int j = threadIdx.x; j < nb - 1; j += blockDim.x)
+    spl[j] = sorted[(int)(((int64_t)(j + 1) * s) / nb) - 1];
+}
+
+// Splitters for the next sort: the just-sorted keys at the bucket quantiles.
+__global__ __launch_bounds__(256) void ss_next_splitters_kernel(const uint64_t* __restrict__ key,
+                                                                int32_t n, int32_t nb,
+                                                                uint64_t* __restrict__ spl) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nb - 1) spl[j] = key[(int64_t)(j + 1) * n / nb - 1];
+}
+
+// 2. per-workgroup bucket counts, bucket-major: hist[b * nwg + w].
+__global__ __launch_bounds__(kSsTileThreads) void ss_count_kernel(const uint64_t* __restrict__ key,
+                                                              int32_t n, int32_t nb,
+                                                              const uint64_t* __restrict__ spl,
+                                                              uint32_t* __restrict__ hist) {
+  __shared__ uint64_t sp[kSsMaxBuckets];
+  __shared__ uint32_t cnt[kSsMaxBuckets];
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+    if (j < nb - 1) sp[j] = spl[j];
+    cnt[j] = 0;
+  }
+  __syncthreads();
+  const int64_t t0 = (int64_t)blockIdx.x * kSsTile;
+  const int64_t t1 = t0 + kSsTile < n ? t0 + kSsTile : n;
+  for (int64_t e = t0 + threadIdx.x; e < t1; e += blockDim.x)
+    atomicAdd(&cnt[bucket_of(sp, nb, key[e])], 1u);
+  __syncthreads();
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) hist[(int64_t)j * gridDim.x + blockIdx.x] = cnt[j];
+}
+
+// 3. scatter to the buckets (order inside a bucket is arbitrary).
+__global__ __launch_bounds__(kSsTileThreads) void ss_scatter_kernel(
+    const uint64_t* __restrict__ key, const uint32_t* __restrict__ val, int32_t n, int32_t nb,
+    const uint64_t* __restrict__ spl, const uint32_t* __restrict__ off,
+    uint64_t* __restrict__ key_out, uint32_t* __restrict__ val_out) {
+  __shared__ uint64_t sp[kSsMaxBuckets];
+  __shared__ uint32_t pos[kSsMaxBuckets];
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+    if (j < nb - 1) sp[j] = spl[j];
+    pos[j] = off[(int64_t)j * gridDim.x + blockIdx.x];
+  }
+  __syncthreads();
+  const int64_t t0 = (int64_t)blockIdx.x * kSsTile;
+  const int64_t t1 = t0 + kSsTile < n ? t0 + kSsTile : n;
+  for (int64_t e = t0 + threadIdx.x; e < t1; e += blockDim.x) {
+    const uint64_t k = key[e];
+    const uint32_t p = atomicAdd(&pos[bucket_of(sp, nb, k)], 1u);
+    key_out[p] = k;
+    val_out[p] = val[e];
+  }
+}
+
+// Merges sorted runs [a, a+len) and [a+len, b) of (ks, vs) into (kd, vd) over
+// [a, b), by the whole workgroup: each thread finds where its slice of the
+// output splits the two runs (merge path) and merges it.
+__device__ void wg_merge(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd,
+                         int64_t a, int64_t len, int64_t b) {
+  const int64_t mid = a + len < b ? a + len : b;
+  const int64_t na = mid - a, nbb = b - mid, tot = b - a;
+  const int64_t per = (tot + blockDim.x - 1) / blockDim.x;
+  const int64_t o0 = (int64_t)threadIdx.x * per;
+  if (o0 >= tot) return;
+  const int64_t o1 = o0 + per < tot ? o0 + per : tot;
+  // i + j = o0 with A[i-1] <= B[j] and B[j-1] < A[i]
+  int64_t lo = o0 > nbb ? o0 - nbb : 0, hi = o0 < na ? o0 : na;
+  while (lo < hi) {
+    const int64_t i = (lo + hi) >> 1, j = o0 - i - 1;
+    if (ks[a + i] <= ks[mid + j])
+      lo = i + 1;
+    else
+      hi = i;
+  }
+  int64_t i = lo, j = o0 - lo;
+  for (int64_t o = o0; o < o1; ++o) {
+    const bool takea = j >= nbb || (i < na && ks[a + i] <= ks[mid + j]);
+    if (takea) {
+      kd[a + o] = ks[a + i];
+      vd[a + o] = vs[a + i];
+      ++i;
+    } else {
+      kd[a + o] = ks[mid + j];
+      vd[a + o] = vs[mid + j];
+      ++j;
+    }
+  }
+}
+
+// 4. one workgroup per bucket: (kb, vb) hold the buckets; the sorted result
+// goes to (ko, vo).  Buckets up to `cap` keys take one block sort; larger
+// ones are sorted in runs of `cap` into (ko, vo) and merged back and forth
+// with (kb, vb), ending in (ko, vo).  A run is loaded in index order
+// (blocked) with padding keys ~0 after it; the sort is stable, so a real key
+// ~0 stays ahead of the padding.
+// One block sort of keys [r0, r0 + rm) of (kb, vb) into (ko, vo), rm <= IPT
+// per thread.
+template <int IPT>
+__device__ void sort_run(const uint64_t* kb, const uint32_t* vb, int64_t r0, int rm,
+                         uint64_t* ko, uint32_t* vo, void* storage, unsigned end_bit) {
+  uint64_t k[IPT];
+  uint32_t v[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = threadIdx.x * IPT + j;
+    k[j] = i < rm ? kb[r0 + i] : ~0ull;
+    v[j] = i < rm ? vb[r0 + i] : 0u;
+  }
+  RunSort<IPT>().sort_to_striped(
+      k, v, *reinterpret_cast<typename RunSort<IPT>::storage_type*>(storage), 0u, end_bit);
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = j * kSsThreads + threadIdx.x;  // striped arrangement
+    if (i < rm) {
+      ko[r0 + i] = k[j];
+      vo[r0 + i] = v[j];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kSsThreads) void ss_bucket_kernel(
+    uint64_t* __restrict__ kb, uint32_t* __restrict__ vb, int32_t n, int32_t nb, int32_t nwg,
+    const uint32_t* __restrict__ off, const uint64_t* __restrict__ spl, int32_t cap,
+    uint64_t* __restrict__ ko, uint32_t* __restrict__ vo) {
+  __shared__ typename RunSort<kSsRunItems>::storage_type store;
+  const int b = blockIdx.x;
+  const int64_t s = off[(int64_t)b * nwg];
+  const int64_t e = b + 1 < nb ? (int64_t)off[(int64_t)(b + 1) * nwg] : (int64_t)n;
+  const int64_t m = e - s;
+  if (m <= 0) return;
+  // every key of the bucket lies in [lo, hi] and so shares their common
+  // leading bits, which need no sorting (the padding key ~0 sorts last
+  // either way: its low bits are all ones)
+  const uint64_t lo = b > 0 ? spl[b - 1] : 0ull;
+  const uint64_t hi = b + 1 < nb ? spl[b] : ~0ull;
+  const uint64_t dif = lo ^ hi;
+  const unsigned end_bit = dif ? 64u - (unsigned)__builtin_clzll(dif) : 1u;
+  const int64_t nruns = (m + cap - 1) / cap;
+  for (int64_t r = 0; r < nruns; ++r) {
+    const int64_t r0 = s + r * cap;
+    const int rm = (int)((r0 + cap < e ? r0 + cap : e) - r0);
+    sort_run<kSsRunItems>(kb, vb, r0, rm, ko, vo, &store, end_bit);
+    __syncthreads();
+  }
+  if (nruns == 1) return;
+  // merge runs of length len: (ko, vo) -> (kb, vb) -> (ko, vo) ...
+  bool in_out = true;  // current runs live in (ko, vo)
+  for (int64_t len = cap; len < m; len *= 2) {
+    const uint64_t* ks = in_out ? ko : kb;
+    const uint32_t* vs = in_out ? vo : vb;
+    uint64_t* kd = in_out ? kb : ko;
+    uint32_t* vd = in_out ? vb : vo;
+    for (int64_t a = s; a < e; a += 2 * len) {
+      const int64_t bnd = a + 2 * len < e ? a + 2 * len : e;
+      wg_merge(ks, vs, kd, vd, a, len, bnd);
+      __syncthreads();
+    }
+    in_out = !in_out;
+    __threadfence_block();
+    __syncthreads();
+  }
+  if (!in_out) {
+    for (int64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
+      ko[i] = kb[i];
+      vo[i] = vb[i];
+    }
+  }
+}
+
+}  // namespace
+
+bool sample_sort_fits(int32_t n) { return n >= 1 && (int64_t)n <= kSsMaxN; }
+
+int32_t sample_sort_buckets(int32_t n) {
+  const int64_t nb = ((int64_t)n + kSsBucket - 1) / kSsBucket;
+  return (int32_t)(nb < 1 ? 1 : nb);
+}
+
+int32_t sample_sort_tiles(int32_t n) { return (n + kSsTile - 1) / kSsTile; }
+
+size_t sample_sort_scan_bytes(int32_t n) {
+  if (!sample_sort_fits(n)) return 0;
+  size_t bytes = 0;
+  uint32_t* p = nullptr;
+  const size_t len = (size_t)sample_sort_buckets(n) * (size_t)sample_sort_tiles(n);
+  (void)rocprim::exclusive_scan(nullptr, bytes, p, p, 0u, len, rocprim::plus<uint32_t>());
+  return bytes;
+}
+
+// Sorts (B.key, B.perm) by key; the result is in B.key, B.perm.
+void launch_sample_sort(stellar_ctx* c) {
+  Buffers& B = c->b;
+  const int32_t n = c->n;
+  const int32_t nb = sample_sort_buckets(n);
+  const int32_t nwg = sample_sort_tiles(n);
+  if (!c->ss_have_spl) ss_sample_kernel<<<1, kSsThreads, 0, c->stream>>>(B.key, n, nb, B.ss_spl);
+  ss_count_kernel<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, n, nb, B.ss_spl, B.ss_hist);
+  size_t bytes = B.tmp_bytes;
+  (void)rocprim::exclusive_scan(B.tmp, bytes, B.ss_hist, B.ss_off, 0u,
+                                (size_t)nb * (size_t)nwg, rocprim::plus<uint32_t>(), c->stream);
+  ss_scatter_kernel<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, B.perm, n, nb, B.ss_spl, B.ss_off,
+                                                       B.key_alt, B.perm_alt);
+  // buckets in (key_alt, perm_alt); the sorted order goes back to (key, perm)
+  ss_bucket_kernel<<<nb, kSsThreads, 0, c->stream>>>(B.key_alt, B.perm_alt, n, nb, nwg, B.ss_off,
+                                                     B.ss_spl, c->sort_cap, B.key, B.perm);
+  if (nb > 1)
+    ss_next_splitters_kernel<<<(nb - 1 + 255) / 256, 256, 0, c->stream>>>(B.key, n, nb, B.ss_spl);
+  c->ss_have_spl = true;
+}
+
+}  // namespace stellar

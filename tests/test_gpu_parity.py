@@ -96,6 +96,40 @@ def test_forces_parity(S, oracle, workload, n, group):
     assert (diff / np.maximum(mag, np.percentile(mag, 10))).max() < FORCE_MAX_TOL
 
 
+@pytest.mark.parametrize("cap", [64, 1000])
+def test_sort_multipass_buckets(S, oracle, cap):
+    """The key sort's buckets taken through the multi-pass path (sorted in
+    runs of `cap` keys, then merged): the tree stays bit-exact, and three
+    steps equal those of the default sort bit for bit."""
+    print(f"sort multipass cap={cap}", flush=True)
+    st = ic(2, 65536)
+    eps2 = float(np.float32(EPS[2] ** 2))
+    with make_engine(S, st, eps2) as e:
+        assert S.load_library().stellar_debug_sort_cap(e._h, cap) == 0
+        gbb, gt = gpu_tree(e)
+        e.sort()
+        e.forces()
+        e.update()
+        e.run(2)
+        got = e.download()
+    obb, ot = oracle_tree(oracle, st)
+    assert np.array_equal(gbb, obb)
+    assert tree_equal(gt, ot), first_tree_difference(gt, ot)
+    with make_engine(S, st, eps2) as e:
+        e.run(3)
+        want = e.download()
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+
+
+def test_sort_cap_rejected(S):
+    st = ic(0, 4096)
+    with make_engine(S, st, 1e-4) as e:
+        lib = S.load_library()
+        assert lib.stellar_debug_sort_cap(e._h, 0) == S.STELLAR_ERR_ARG
+        assert lib.stellar_debug_sort_cap(e._h, 2049) == S.STELLAR_ERR_ARG
+
+
 def test_steps_match_oracle(S, oracle):
     """Each GPU step = oracle walk on the GPU's own pre-step state, then the
     oracle update applied to the GPU forces (bit-exact)."""
