@@ -27,7 +27,7 @@ constexpr int64_t kMaxBlocks = int64_t(1) << 26;  // root + cells
 constexpr int kMaxTieRun = 256;
 // Key sort (sort.hip): sample sort for up to kSsMaxN bodies, buckets of at
 // most kSsCapMax keys sorted in one LDS pass.
-constexpr int64_t kSsMaxN = (int64_t)1 << 22;
+constexpr int64_t kSsMaxN = (int64_t)1 << 24;
 constexpr int kSsBucket = 1536;  // target keys per bucket
 constexpr int kSsMaxBuckets = (int)((kSsMaxN + kSsBucket - 1) / kSsBucket);
 constexpr int kSsCapMax = 2048;

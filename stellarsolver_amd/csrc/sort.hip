@@ -36,10 +36,11 @@ namespace stellar {
 namespace {
 
 constexpr int kSsOver = 4;              // samples per bucket
-constexpr int kSsMaxSamples = 8192;     // at most (fewer per bucket beyond 2048 buckets)
+constexpr int kSsMaxSamples = 8192;     // at most: beyond 8192 buckets some splitters repeat
 constexpr int kSsThreads = 256;       // threads of the sample, bucket and merge workgroups
 constexpr int kSsTileThreads = 1024;  // threads of the counting / scattering workgroups
-constexpr int kSsTile = 4096;     // keys per counting / scattering workgroup
+constexpr int kSsTileMin = 4096;  // keys per counting / scattering workgroup (at least)
+constexpr int kSsMaxTiles = 256;  // counting / scattering workgroups (at most)
 constexpr int kSsRunItems = kSsCapMax / kSsThreads;         // 8 per thread
 constexpr int kSsSampleItems = kSsMaxSamples / kSsThreads;  // 32 per thread
 template <int IPT>
@@ -94,8 +95,10 @@ __global__ __launch_bounds__(kSsThreads) void ss_sample_kernel(const uint64_t* _
   else
     sort_samples<kSsSampleItems>(key, n, s, sorted, store);
   __syncthreads();
-  for (int j = threadIdx.x; j < nb - 1; j += blockDim.x)
-    spl[j] = sorted[(int)(((int64_t)(j + 1) * s) / nb) - 1];
+  for (int j = threadIdx.x; j < nb - 1; j += blockDim.x) {
+    const int q = (int)(((int64_t)(j + 1) * s) / nb) - 1;
+    spl[j] = sorted[q > 0 ? q : 0];
+  }
 }
 
 // Splitters for the next sort: the just-sorted keys at the bucket quantiles.
@@ -108,7 +111,7 @@ __global__ __launch_bounds__(256) void ss_next_splitters_kernel(const uint64_t* 
 
 // 2. per-workgroup bucket counts, bucket-major: hist[b * nwg + w].
 __global__ __launch_bounds__(kSsTileThreads) void ss_count_kernel(const uint64_t* __restrict__ key,
-                                                              int32_t n, int32_t nb,
+                                                              int32_t n, int32_t nb, int32_t tile,
                                                               const uint64_t* __restrict__ spl,
                                                               uint32_t* __restrict__ hist) {
   __shared__ uint64_t sp[kSsMaxBuckets];
@@ -118,8 +121,8 @@ __global__ __launch_bounds__(kSsTileThreads) void ss_count_kernel(const uint64_t
     cnt[j] = 0;
   }
   __syncthreads();
-  const int64_t t0 = (int64_t)blockIdx.x * kSsTile;
-  const int64_t t1 = t0 + kSsTile < n ? t0 + kSsTile : n;
+  const int64_t t0 = (int64_t)blockIdx.x * tile;
+  const int64_t t1 = t0 + tile < n ? t0 + tile : n;
   for (int64_t e = t0 + threadIdx.x; e < t1; e += blockDim.x)
     atomicAdd(&cnt[bucket_of(sp, nb, key[e])], 1u);
   __syncthreads();
@@ -129,7 +132,7 @@ __global__ __launch_bounds__(kSsTileThreads) void ss_count_kernel(const uint64_t
 // 3. scatter to the buckets (order inside a bucket is arbitrary).
 __global__ __launch_bounds__(kSsTileThreads) void ss_scatter_kernel(
     const uint64_t* __restrict__ key, const uint32_t* __restrict__ val, int32_t n, int32_t nb,
-    const uint64_t* __restrict__ spl, const uint32_t* __restrict__ off,
+    int32_t tile, const uint64_t* __restrict__ spl, const uint32_t* __restrict__ off,
     uint64_t* __restrict__ key_out, uint32_t* __restrict__ val_out) {
   __shared__ uint64_t sp[kSsMaxBuckets];
   __shared__ uint32_t pos[kSsMaxBuckets];
@@ -138,8 +141,8 @@ __global__ __launch_bounds__(kSsTileThreads) void ss_scatter_kernel(
     pos[j] = off[(int64_t)j * gridDim.x + blockIdx.x];
   }
   __syncthreads();
-  const int64_t t0 = (int64_t)blockIdx.x * kSsTile;
-  const int64_t t1 = t0 + kSsTile < n ? t0 + kSsTile : n;
+  const int64_t t0 = (int64_t)blockIdx.x * tile;
+  const int64_t t1 = t0 + tile < n ? t0 + tile : n;
   for (int64_t e = t0 + threadIdx.x; e < t1; e += blockDim.x) {
     const uint64_t k = key[e];
     const uint32_t p = atomicAdd(&pos[bucket_of(sp, nb, k)], 1u);
@@ -272,7 +275,18 @@ int32_t sample_sort_buckets(int32_t n) {
   return (int32_t)(nb < 1 ? 1 : nb);
 }
 
-int32_t sample_sort_tiles(int32_t n) { return (n + kSsTile - 1) / kSsTile; }
+// Keys per counting / scattering workgroup: kSsTileMin, or more so that at
+// most kSsMaxTiles workgroups (and bucket-count columns) are needed.
+static int32_t sample_sort_tile(int32_t n) {
+  const int64_t per = ((int64_t)n + kSsMaxTiles - 1) / kSsMaxTiles;
+  const int64_t t = (per + 1023) / 1024 * 1024;
+  return (int32_t)(t > kSsTileMin ? t : kSsTileMin);
+}
+
+int32_t sample_sort_tiles(int32_t n) {
+  const int32_t t = sample_sort_tile(n);
+  return (int32_t)(((int64_t)n + t - 1) / t);
+}
 
 size_t sample_sort_scan_bytes(int32_t n) {
   if (!sample_sort_fits(n)) return 0;
@@ -289,12 +303,13 @@ void launch_sample_sort(stellar_ctx* c) {
   const int32_t n = c->n;
   const int32_t nb = sample_sort_buckets(n);
   const int32_t nwg = sample_sort_tiles(n);
+  const int32_t tile = sample_sort_tile(n);
   if (!c->ss_have_spl) ss_sample_kernel<<<1, kSsThreads, 0, c->stream>>>(B.key, n, nb, B.ss_spl);
-  ss_count_kernel<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, n, nb, B.ss_spl, B.ss_hist);
+  ss_count_kernel<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, n, nb, tile, B.ss_spl, B.ss_hist);
   size_t bytes = B.tmp_bytes;
   (void)rocprim::exclusive_scan(B.tmp, bytes, B.ss_hist, B.ss_off, 0u,
                                 (size_t)nb * (size_t)nwg, rocprim::plus<uint32_t>(), c->stream);
-  ss_scatter_kernel<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, B.perm, n, nb, B.ss_spl, B.ss_off,
+  ss_scatter_kernel<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, B.perm, n, nb, tile, B.ss_spl, B.ss_off,
                                                        B.key_alt, B.perm_alt);
   // buckets in (key_alt, perm_alt); the sorted order goes back to (key, perm)
   ss_bucket_kernel<<<nb, kSsThreads, 0, c->stream>>>(B.key_alt, B.perm_alt, n, nb, nwg, B.ss_off,
