@@ -169,7 +169,9 @@ def stage_gbs(stages, n, cells):
     time, in GB/s: what the stage moves at least, not what it fetches."""
     bytes_ = {"bbox": 8 * n, "build": 8 * n + 20 * cells, "com": 12 * n + 12 * cells,
               "sort": 4 * n + 24 * cells + 24 * n, "update": 40 * n}
-    return {k: b / (stages[k] * 1e-3) / 1e9 for k, b in bytes_.items() if stages.get(k)}
+    # a stage under 1 us ran inside another (the update inside the walk)
+    return {k: b / (stages[k] * 1e-3) / 1e9 for k, b in bytes_.items()
+            if stages.get(k, 0.0) > 1e-3}
 
 
 def synced_diagnostics(eng, dt, forces):
