@@ -18,7 +18,13 @@
 //   3. keys and values are scattered to their buckets;
 //   4. one workgroup per bucket sorts it (a block radix sort, stable, with
 //      the bucket's keys in their index order ahead of the padding) and writes
-//      the final order.
+//      the final order.  Keys of a bucket agree above the highest bit in
+//      which its two splitters differ; when more than 32 bits remain, the
+//      block sort takes the 32-bit window below that bit (four 8-bit passes
+//      instead of up to eight) and runs of equal windows -- bodies closer
+//      than the window resolves, a few pairs per step -- are put in full-key
+//      order by insertion afterwards (a run longer than 32 sends the bucket
+//      to the full-key sort).
 // Equal keys always land in the same bucket (the bucket is the number of
 // splitters below the key), so runs of equal keys end up contiguous, which is
 // all the later tie refinement needs (the order inside a run is not used).
@@ -47,6 +53,9 @@ template <int IPT>
 using RunSort = rocprim::block_radix_sort<uint64_t, kSsThreads, IPT, uint32_t>;
 template <int IPT>
 using SampleSort = rocprim::block_radix_sort<uint64_t, kSsThreads, IPT>;
+// 32-bit window of the keys with the run position as value (one-run buckets)
+using WinSort = rocprim::block_radix_sort<uint32_t, kSsThreads, kSsRunItems, uint32_t>;
+constexpr int kSsFixMax = 32;  // longest run of equal windows fixed by one thread
 template <class A, class B>
 constexpr size_t max_size() {
   return sizeof(A) > sizeof(B) ? sizeof(A) : sizeof(B);
@@ -217,11 +226,90 @@ __device__ void sort_run(const uint64_t* kb, const uint32_t* vb, int64_t r0, int
   }
 }
 
+// LDS of a bucket workgroup: the full keys of a one-run bucket, then either
+// block sort's storage or the sorted windows and positions.
+struct BucketLds {
+  uint64_t key[kSsCapMax];
+  union {
+    typename RunSort<kSsRunItems>::storage_type run;
+    typename WinSort::storage_type win;
+    struct {
+      uint32_t w[kSsCapMax];    // sorted windows
+      uint32_t pos[kSsCapMax];  // their positions in the bucket
+    } post;
+  } u;
+  int32_t fallback;
+};
+
+// One block sort of a bucket [r0, r0 + rm), rm <= kSsCapMax, whose keys agree
+// above end_bit > 32: sort the 32-bit windows key >> (end_bit - 32) with the
+// keys' positions, then put each run of equal windows (keys that differ only
+// below the window: bodies closer than a cell of the window's last level)
+// in full-key order by insertion.  Four radix passes instead of up to eight.
+// Returns false, with nothing written, if a run is longer than kSsFixMax
+// (a dense cluster): the caller then sorts the full keys.
+__device__ bool sort_run_window(const uint64_t* kb, const uint32_t* vb, int64_t r0, int rm,
+                                uint64_t* ko, uint32_t* vo, BucketLds& L, unsigned end_bit) {
+  constexpr int IPT = kSsRunItems;
+  const unsigned shift = end_bit - 32u;
+  uint32_t w[IPT], v[IPT];
+  if (threadIdx.x == 0) L.fallback = 0;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = threadIdx.x * IPT + j;  // blocked: stable order = position order
+    const uint64_t k = i < rm ? kb[r0 + i] : ~0ull;
+    L.key[i] = k;
+    w[j] = (uint32_t)(k >> shift);  // padding ~0 sorts after every real key
+    v[j] = (uint32_t)i;
+  }
+  WinSort().sort(w, v, L.u.win, 0u, 32u);
+  __syncthreads();  // the sort's storage is reused below
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int i = threadIdx.x * IPT + j;
+    L.u.post.w[i] = w[j];
+    L.u.post.pos[i] = v[j];
+  }
+  __syncthreads();
+  // the thread holding a run's first entry sorts the whole run
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int p = threadIdx.x * IPT + j;
+    if (p + 1 >= rm) continue;
+    const uint32_t wp = L.u.post.w[p];
+    if (L.u.post.w[p + 1] != wp || (p > 0 && L.u.post.w[p - 1] == wp)) continue;
+    int q = p + 2;
+    while (q < rm && L.u.post.w[q] == wp && q - p <= kSsFixMax) ++q;
+    if (q - p > kSsFixMax) {
+      atomicOr(&L.fallback, 1);
+      continue;
+    }
+    for (int a = p + 1; a < q; ++a) {  // insertion by full key (stable)
+      const uint32_t pa = L.u.post.pos[a];
+      const uint64_t ka = L.key[pa];
+      int b = a;
+      while (b > p && L.key[L.u.post.pos[b - 1]] > ka) {
+        L.u.post.pos[b] = L.u.post.pos[b - 1];
+        --b;
+      }
+      L.u.post.pos[b] = pa;
+    }
+  }
+  __syncthreads();
+  if (L.fallback) return false;
+  for (int i = threadIdx.x; i < rm; i += kSsThreads) {
+    const uint32_t src = L.u.post.pos[i];
+    ko[r0 + i] = L.key[src];
+    vo[r0 + i] = vb[r0 + src];
+  }
+  return true;
+}
+
 __global__ __launch_bounds__(kSsThreads) void ss_bucket_kernel(
     uint64_t* __restrict__ kb, uint32_t* __restrict__ vb, int32_t n, int32_t nb, int32_t nwg,
     const uint32_t* __restrict__ off, const uint64_t* __restrict__ spl, int32_t cap,
     uint64_t* __restrict__ ko, uint32_t* __restrict__ vo) {
-  __shared__ typename RunSort<kSsRunItems>::storage_type store;
+  __shared__ BucketLds L;
   const int b = blockIdx.x;
   const int64_t s = off[(int64_t)b * nwg];
   const int64_t e = b + 1 < nb ? (int64_t)off[(int64_t)(b + 1) * nwg] : (int64_t)n;
@@ -235,10 +323,14 @@ __global__ __launch_bounds__(kSsThreads) void ss_bucket_kernel(
   const uint64_t dif = lo ^ hi;
   const unsigned end_bit = dif ? 64u - (unsigned)__builtin_clzll(dif) : 1u;
   const int64_t nruns = (m + cap - 1) / cap;
+  if (nruns == 1 && end_bit > 32u) {
+    if (sort_run_window(kb, vb, s, (int)m, ko, vo, L, end_bit)) return;
+    __syncthreads();
+  }
   for (int64_t r = 0; r < nruns; ++r) {
     const int64_t r0 = s + r * cap;
     const int rm = (int)((r0 + cap < e ? r0 + cap : e) - r0);
-    sort_run<kSsRunItems>(kb, vb, r0, rm, ko, vo, &store, end_bit);
+    sort_run<kSsRunItems>(kb, vb, r0, rm, ko, vo, &L.u.run, end_bit);
     __syncthreads();
   }
   if (nruns == 1) return;

@@ -122,6 +122,41 @@ def test_sort_multipass_buckets(S, oracle, cap):
         assert np.array_equal(got[k], want[k]), k
 
 
+def test_sort_dense_clusters(S, oracle):
+    """Bodies much closer together than their sort bucket's 32-bit key window
+    resolves: a 12 x 12 grid (a run of equal windows longer than the
+    per-thread fix-up takes, so the bucket falls back to the full-key sort),
+    a 4 x 2 grid and pairs (runs fixed up by insertion).  The tree stays
+    bit-exact."""
+    print("sort dense clusters", flush=True)
+    st = ic(0, 65536)
+    x, y = st["x"].copy(), st["y"].copy()
+    h = np.float32(5e-7)
+    gi, gj = np.meshgrid(np.arange(12), np.arange(12), indexing="ij")
+    big = slice(65536 - 144, 65536)
+    x[big] = np.float32(1.0) + h * gi.ravel().astype(np.float32)
+    y[big] = np.float32(1.0) + h * gj.ravel().astype(np.float32)
+    gi, gj = np.meshgrid(np.arange(4), np.arange(2), indexing="ij")
+    small = slice(65536 - 152, 65536 - 144)
+    x[small] = np.float32(-1.0) + h * gi.ravel().astype(np.float32)
+    y[small] = np.float32(0.5) + h * gj.ravel().astype(np.float32)
+    for k in range(16):  # pairs
+        x[1000 + k] = x[2000 + k] + np.float32(4.0) * np.spacing(x[2000 + k])
+        y[1000 + k] = y[2000 + k]
+    st = dict(st, x=x, y=y)
+    for q in ("vx", "vy"):
+        st[q] = st[q].copy()
+        st[q][65536 - 152:] = 0.0
+    eps2 = float(np.float32(EPS[0] ** 2))
+    with make_engine(S, st, eps2) as e:
+        gbb, gt = gpu_tree(e)
+    obb, ot = oracle_tree(oracle, st)
+    assert np.array_equal(gbb, obb)
+    assert tree_equal(gt, ot), first_tree_difference(gt, ot)
+    for k in ("com_x", "com_y", "com_m"):
+        assert np.array_equal(gt[k], ot[k]), k
+
+
 def test_sort_cap_rejected(S):
     st = ic(0, 4096)
     with make_engine(S, st, 1e-4) as e:
