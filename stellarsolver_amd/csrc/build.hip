@@ -196,13 +196,17 @@ __device__ void deep_key(float px, float py, float4 bb, int l1, uint64_t& k2,
 // Runs of equal keys (bodies of one instance closer than a level-l1 cell):
 // order each run by (deeper levels, body id) with an in-place insertion
 // sort.  Such runs are rare and short; longer than kMaxTieRun is an error.
+// With the sample sort (spl != nullptr) the first nb - 1 threads also read
+// off the next sort's splitters: the sorted keys at the bucket quantiles.
 __global__ __launch_bounds__(256) void ties_kernel(
     const float* __restrict__ x, const float* __restrict__ y,
     const int32_t* __restrict__ id, int32_t n, const uint64_t* __restrict__ key,
     const int32_t* __restrict__ inst_of, const float4* __restrict__ ibbox, int l1,
     uint32_t* __restrict__ perm, uint64_t* __restrict__ key2,
-    uint32_t* __restrict__ key3, DevScalars* __restrict__ dev) {
+    uint32_t* __restrict__ key3, DevScalars* __restrict__ dev, int32_t nb,
+    uint64_t* __restrict__ spl) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (spl && s < nb - 1) spl[s] = key[(int64_t)(s + 1) * n / nb - 1];
   if (s + 1 >= n) return;
   const uint64_t k = key[s];  // the instance is part of the key
   if (key[s + 1] != k || (s > 0 && key[s - 1] == k)) return;
@@ -623,9 +627,11 @@ void launch_build(stellar_ctx* c) {
     if (kb.current() != B.key) std::swap(B.key, B.key_alt);
     if (vb.current() != B.perm) std::swap(B.perm, B.perm_alt);
   }
+  const bool ss = sample_sort_fits(n);
   ties_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, B.id, n, B.key, B.inst_of,
                                                         B.ibbox, l1, B.perm, B.key2, B.key3,
-                                                        B.dev);
+                                                        B.dev, ss ? sample_sort_buckets(n) : 0,
+                                                        ss ? B.ss_spl : nullptr);
   const KeyView K{B.key, B.key2, B.key3, l1, ibits};
   int64_t mtb = 0;
   const MinTree mt = mintree_layout(n, B.lam, B.lmin, &mtb);

@@ -110,14 +110,6 @@ __global__ __launch_bounds__(kSsThreads) void ss_sample_kernel(const uint64_t* _
   }
 }
 
-// Splitters for the next sort: the just-sorted keys at the bucket quantiles.
-__global__ __launch_bounds__(256) void ss_next_splitters_kernel(const uint64_t* __restrict__ key,
-                                                                int32_t n, int32_t nb,
-                                                                uint64_t* __restrict__ spl) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < nb - 1) spl[j] = key[(int64_t)(j + 1) * n / nb - 1];
-}
-
 // 2. per-workgroup bucket counts, bucket-major: hist[b * nwg + w].
 __global__ __launch_bounds__(kSsTileThreads) void ss_count_kernel(const uint64_t* __restrict__ key,
                                                               int32_t n, int32_t nb, int32_t tile,
@@ -406,8 +398,8 @@ void launch_sample_sort(stellar_ctx* c) {
   // buckets in (key_alt, perm_alt); the sorted order goes back to (key, perm)
   ss_bucket_kernel<<<nb, kSsThreads, 0, c->stream>>>(B.key_alt, B.perm_alt, n, nb, nwg, B.ss_off,
                                                      B.ss_spl, c->sort_cap, B.key, B.perm);
-  if (nb > 1)
-    ss_next_splitters_kernel<<<(nb - 1 + 255) / 256, 256, 0, c->stream>>>(B.key, n, nb, B.ss_spl);
+  // the next sort's splitters are read off the sorted keys by the build's
+  // tie kernel, which runs next
   c->ss_have_spl = true;
 }
 
