@@ -367,6 +367,39 @@ def test_quadrupole_parity(S, oracle, workload, n, textbook):
     assert rms_rel(f["ax"], f["ay"], ax, ay) < FORCE_RMS_TOL
 
 
+@pytest.mark.parametrize("kpop,threads", [(1, 512), (2, 512), (4, 256), (1, 1024), (2, 64)])
+@pytest.mark.parametrize("model", ["reference", "textbook+quad", "bmax+quad"])
+def test_walk_shapes_parity(S, oracle, model, kpop, threads):
+    """Every walk shape the tuner may pick (nodes per pop, workgroup size)
+    makes the oracle's decisions: interaction counts exact, forces within the
+    rsqrt tolerance."""
+    print(f"walk shape parity model={model} kpop={kpop} threads={threads}", flush=True)
+    st = ic(2, 65536)
+    eps2 = float(np.float32(EPS[2] ** 2))
+    flags = {"reference": 0,
+             "textbook+quad": S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING,
+             "bmax+quad": S.FLAG_QUADRUPOLE | S.FLAG_BMAX_OPENING}[model]
+    with make_engine(S, st, eps2, flags=flags) as e:
+        e.set_walk_shape(kpop, threads)
+        gbb, gt = gpu_tree(e)
+        e.sort()
+        e.forces()
+        f = e.export_forces()
+    obb, ot = oracle_tree(oracle, st)
+    x, y, m = st["x"], st["y"], st["mass"]
+    kw = {}
+    if model == "bmax+quad":
+        oracle.bmax_exact(ot, x, y, m, 0.5, eps2)
+        kw = {"quad": True, "bmax": True}
+    if model != "reference":
+        oracle.quad_exact(ot, x, y, m)
+    if model == "textbook+quad":
+        kw = {"opening": 1, "quad": True}
+    ax, ay, it = oracle.walk(ot, x, y, m, obb, 0.5, eps2, 64, **kw)
+    assert np.array_equal(f["inter"], it)
+    assert rms_rel(f["ax"], f["ay"], ax, ay) < FORCE_RMS_TOL
+
+
 @pytest.mark.parametrize("theta", [0.3, 0.8])
 @pytest.mark.parametrize("model", ["reference", "textbook+quad"])
 def test_theta_parity(S, oracle, theta, model):
