@@ -47,7 +47,7 @@ HBM_PEAK_GBS = 8000.0
 # --pmc runs of this script: FETCH_SIZE 182,078 KiB x 2.0 + WRITE_SIZE
 # 13,312 KiB x 1.0 per dispatch, the factors being what the same counters read
 # for a 1 GiB device copy (profiles/hbm_traffic_colliding_disks_1M.md).
-WALK_TRAFFIC_BYTES = {(2, 1 << 20, 0.5, 64, "quad", "textbook"): (179173 * 2 + 13312) * 1024}
+WALK_TRAFFIC_BYTES = {(2, 1 << 20, 0.5, 64, "quad", "textbook"): (181757 * 2 + 13312) * 1024}
 
 
 def parse():
