@@ -13,13 +13,12 @@ inline unsigned grid_for(int64_t items, int threads) {
 // sort_kernel (kernels.cu:292-336) lays bodies out in DFS order through
 // sorted[]; the build here already produced that order (perm), so this stage
 // moves the body state itself into it, making every later access coalesced.
+// The build's link kernel already wrote x, y and m in tree order (it reads
+// them through perm for the tree records); this kernel gathers the rest.
 __global__ __launch_bounds__(256) void permute_kernel(
-    int32_t n, const uint32_t* __restrict__ perm, const float* __restrict__ x,
-    const float* __restrict__ y, const float* __restrict__ vx,
-    const float* __restrict__ vy, const float* __restrict__ m,
-    const int32_t* __restrict__ id, float* __restrict__ x2,
-    float* __restrict__ y2, float* __restrict__ vx2, float* __restrict__ vy2,
-    float* __restrict__ m2, int32_t* __restrict__ id2,
+    int32_t n, const uint32_t* __restrict__ perm, const float* __restrict__ vx,
+    const float* __restrict__ vy, const int32_t* __restrict__ id,
+    float* __restrict__ vx2, float* __restrict__ vy2, int32_t* __restrict__ id2,
     DevScalars* __restrict__ dev) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s == 0) {  // for the walk that follows
@@ -28,11 +27,8 @@ __global__ __launch_bounds__(256) void permute_kernel(
   }
   if (s >= n) return;
   const uint32_t p = perm[s];
-  x2[s] = x[p];
-  y2[s] = y[p];
   vx2[s] = vx[p];
   vy2[s] = vy[p];
-  m2[s] = m[p];
   id2[s] = id[p];
 }
 
@@ -270,9 +266,8 @@ void launch_sort(stellar_ctx* c) {
   const int32_t n = c->n;
   if (n == 0) return;
   Buffers& B = c->b;
-  permute_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(
-      n, B.perm, B.x, B.y, B.vx, B.vy, B.m, B.id, B.x2, B.y2, B.vx2, B.vy2,
-      B.m2, B.id2, B.dev);
+  permute_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, B.perm, B.vx, B.vy, B.id,
+                                                           B.vx2, B.vy2, B.id2, B.dev);
   std::swap(B.x, B.x2);
   std::swap(B.y, B.y2);
   std::swap(B.vx, B.vx2);
@@ -306,11 +301,12 @@ void launch_direct_fp64(stellar_ctx* c, int32_t nt, const int32_t* dtargets,
                         double eps2, double* dax, double* day) {
   Buffers& B = c->b;
   const int32_t n = c->n;
-  // positions by id, staged in the sort's spare buffers
+  // positions by id, staged in the sort's spare velocity buffers (the spare
+  // position buffers hold the build's tree-order copy until the sort stage)
   by_id_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, B.x, B.y, B.id,
-                                                         B.x2, B.y2);
+                                                         B.vx2, B.vy2);
   direct_fp64_kernel<<<grid_for(nt, kTile), kTile, 0, c->stream>>>(
-      nt, dtargets, B.inst_of, B.ioff, B.x2, B.y2, B.x, B.y, B.m, eps2, dax, day);
+      nt, dtargets, B.inst_of, B.ioff, B.vx2, B.vy2, B.x, B.y, B.m, eps2, dax, day);
 }
 
 int energy_parts(stellar_ctx* c) { return c->ntiles * kSlices; }

@@ -437,6 +437,10 @@ struct LinkArgs {
   float theta_div;
   float eps2;
   int unsoft;
+  // x, y and m moved into tree order for the sort stage, which swaps them in
+  float* x2;
+  float* y2;
+  float* m2;
 };
 
 __device__ __forceinline__ void set_cell_threshold(const LinkArgs& A, int64_t slot_index,
@@ -509,6 +513,13 @@ __global__ __launch_bounds__(256) void cells_kernel(LinkArgs A) {
 __global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= A.n) return;
+  // body s's record values, read once through the permutation and also
+  // written out in tree order: the sort stage then gathers only vx, vy, id
+  const uint32_t p = A.perm[s];
+  const float bx = A.x[p], by = A.y[p], bm = A.m[p];
+  A.x2[s] = bx;
+  A.y2[s] = by;
+  A.m2[s] = bm;
   const int lp = s > 0 ? A.lam[s - 1] : 0;  // 0 at an instance's first position
   const int nc = s + 1 < A.n ? A.ncell[s] : 0;
   int64_t pid = -1;  // cell id of the level-lp node (-1: the instance's root)
@@ -554,8 +565,7 @@ __global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
       }
     }
   }
-  const uint32_t p = A.perm[s];
-  A.blk[4 * bid + slot] = ChildRec{A.x[p], A.y[p], A.m[p], -1 - s};
+  A.blk[4 * bid + slot] = ChildRec{bx, by, bm, -1 - s};
   if (A.qthr) A.qblk[4 * bid + slot] = make_float4(0.f, 0.f, 0.f, -1.f);
 }
 
@@ -651,7 +661,8 @@ void launch_build(stellar_ctx* c) {
              c->ninst,  mt,
              B.qblk,
              (c->p.flags & (STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_BMAX_OPENING)) ? 1 : 0,
-             B.ibbox,   textbook ? theta * theta : theta, c->p.eps2, textbook ? 1 : 0};
+             B.ibbox,   textbook ? theta * theta : theta, c->p.eps2, textbook ? 1 : 0,
+             B.x2,      B.y2,     B.m2};
   cells_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(A);
   link_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(A);
 }
