@@ -216,7 +216,10 @@ int stellar_autotune(stellar_ctx* ctx, int32_t group_mask, int32_t reps,
                      stellar_tuning* out);
 /* Testing hook, no counterpart in the reference: caps the keys a bucket of
  * the key sort sorts in one block pass (1 .. 2048, default 2048), so that tests
- * can send every bucket through the multi-pass path that larger buckets take. */
+ * can send every bucket through the multi-pass path that larger buckets take;
+ * cap = -1 instead sorts the keys with the device-wide radix sort that
+ * contexts of more than 2^24 bodies use (and cap = 1 .. 2048 returns to the
+ * sample sort). */
 int stellar_debug_sort_cap(stellar_ctx* ctx, int32_t cap);
 
 /* Set the group-64 walk launch shape by hand: nodes_per_pop 1, 2 or 4

@@ -623,7 +623,8 @@ void launch_build(stellar_ctx* c) {
   keys_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, n, B.inst_of, B.ibbox,
                                                         l1, ibits, B.key, B.perm);
   size_t bytes = B.tmp_bytes;
-  if (sample_sort_fits(n)) {
+  const bool ss = sample_sort_fits(n) && !c->radix_sort;
+  if (ss) {
     launch_sample_sort(c);  // sorted data back in (key, perm)
   } else {
     rocprim::double_buffer<uint64_t> kb(B.key, B.key_alt);
@@ -637,7 +638,6 @@ void launch_build(stellar_ctx* c) {
     if (kb.current() != B.key) std::swap(B.key, B.key_alt);
     if (vb.current() != B.perm) std::swap(B.perm, B.perm_alt);
   }
-  const bool ss = sample_sort_fits(n);
   ties_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, B.id, n, B.key, B.inst_of,
                                                         B.ibbox, l1, B.perm, B.key2, B.key3,
                                                         B.dev, ss ? sample_sort_buckets(n) : 0,

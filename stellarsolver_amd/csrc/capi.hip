@@ -945,8 +945,9 @@ int stellar_energy_fp64(stellar_ctx* c, double eps2, double* kinetic,
 
 int stellar_debug_sort_cap(stellar_ctx* c, int32_t cap) {
   if (!c) return STELLAR_ERR_ARG;
-  if (cap < 1 || cap > kSsCapMax) return fail(c, STELLAR_ERR_ARG, "sort cap");
-  c->sort_cap = cap;
+  if (cap != -1 && (cap < 1 || cap > kSsCapMax)) return fail(c, STELLAR_ERR_ARG, "sort cap");
+  c->radix_sort = cap == -1;
+  if (cap > 0) c->sort_cap = cap;
   drop_graphs(c);
   return STELLAR_OK;
 }

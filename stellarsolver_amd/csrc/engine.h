@@ -196,6 +196,7 @@ struct stellar_ctx {
   int walk_waves = 0;  // 0: default (8)
   int32_t sort_cap = stellar::kSsCapMax;  // keys per block pass of a sort bucket
   bool ss_have_spl = false;  // splitters from the previous sort are in place
+  bool radix_sort = false;   // testing: the device-wide radix sort at any size
   // per-step stage events of the current stellar_run (8 per step)
   std::vector<hipEvent_t> evrun;
   int32_t evrun_steps = 0;

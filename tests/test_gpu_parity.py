@@ -157,6 +157,34 @@ def test_sort_dense_clusters(S, oracle):
         assert np.array_equal(gt[k], ot[k]), k
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_radix_sort_path(S, oracle, graph):
+    """The device-wide radix sort that contexts above 2^24 bodies use (forced
+    here at 2^16): the tree is bit-exact, and four steps -- direct, or two
+    replays of a captured two-step graph (rocPRIM's merge-sort variant) --
+    equal those of the sample sort bit for bit."""
+    print(f"radix sort path graph={graph}", flush=True)
+    st = ic(2, 65536)
+    eps2 = float(np.float32(EPS[2] ** 2))
+    lib = S.load_library()
+    with make_engine(S, st, eps2) as e:
+        assert lib.stellar_debug_sort_cap(e._h, -1) == 0
+        gbb, gt = gpu_tree(e)
+    obb, ot = oracle_tree(oracle, st)
+    assert np.array_equal(gbb, obb)
+    assert tree_equal(gt, ot), first_tree_difference(gt, ot)
+    flags = S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING
+    with make_engine(S, st, eps2, flags=flags | (S.FLAG_GRAPH if graph else 0)) as e:
+        assert lib.stellar_debug_sort_cap(e._h, -1) == 0
+        e.run(4)
+        got = e.download()
+    with make_engine(S, st, eps2, flags=flags) as e:
+        e.run(4)
+        want = e.download()
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+
+
 def test_sort_cap_rejected(S):
     st = ic(0, 4096)
     with make_engine(S, st, 1e-4) as e:
