@@ -284,7 +284,7 @@ def run_ensemble(args, S):
     achieved = (inter * OPS_PER_INTERACTION + qinter * OPS_PER_QUAD_TERM) / (walk_ms * 1e-3) / 1e12
     peak = VALU_PEAK_LANE_OPS / 1e12
     out = {
-        "metric": "BH body-steps/s, batched ensemble (config 3 workload)",
+        "metric": "BH body-steps/s, batched ensemble of 256 instances (N_i 1024..65536)",
         "value": batched,
         "unit": "body-steps/s",
         "n_gpus": 1,

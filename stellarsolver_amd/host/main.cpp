@@ -6,7 +6,7 @@
 // -debug, -benchmark, -fullscreen, -iterations=, -timestep=, -gravity=,
 // -dampening= (and -barnes-hut, which the README shows and which selects
 // nothing).  Added: -numbodies= (reference: 512*256), -theta=, -eps=,
-// -group=, -config= (seeded workload 0, 1, 2 or 4), -seed=, -quad,
+// -group=, -workload= (seeded workload 0, 1, 2 or 4), -seed=, -quad,
 // -textbook.  As in the reference, a flag's first character is skipped
 // when matching, so "-x" and "--x"-style prefixes of one character both work.
 #include <cmath>
@@ -71,7 +71,7 @@ int main(int argc, char** argv) {
   const char* eps = cmdLineValue(argc, argv, "eps");
   parameters.eps2 = eps ? (float)(std::atof(eps) * std::atof(eps)) : 0.025f;
   parameters.group = getCmdLineParameterInt(argc, argv, "group", 64);
-  parameters.workload = getCmdLineParameterInt(argc, argv, "config", -1);
+  parameters.workload = getCmdLineParameterInt(argc, argv, "workload", -1);
   const char* seed = cmdLineValue(argc, argv, "seed");
   parameters.seed_given = seed != nullptr;
   parameters.seed = seed ? (uint64_t)std::strtoull(seed, nullptr, 10) : 0;

@@ -41,7 +41,7 @@ def gpu():
 
 @pytest.mark.gpu
 def test_benchmark_lines(gpu):
-    p = run("-benchmark", "-iterations=5", "-numbodies=65536", "-config=2", "-eps=0.01")
+    p = run("-benchmark", "-iterations=5", "-numbodies=65536", "-workload=2", "-eps=0.01")
     assert p.returncode == 0, p.stderr
     lines = [ln for ln in p.stdout.splitlines() if ln.startswith("Timestep:")]
     assert len(lines) == 5
