@@ -28,7 +28,7 @@ namespace stellar {
 
 namespace {
 
-constexpr int kBboxThreads = 256;
+constexpr int kBboxThreads = 512;
 
 // Sort configuration for captured steps: rocPRIM's merge-sort path at every
 // size (its onesweep path, chosen above 2^20 items, clears its counters
