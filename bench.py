@@ -388,6 +388,20 @@ def main():
                  "velocities": "synchronised, v + dt/2 a(x) (the update stores v half a step behind x)",
                  "steps": args.warmup + args.steps,
                  "total_mass": msum}
+    graph_replay = None
+    if not args.graph and args.steps >= 2:
+        # side figure: the same number of steps replayed from captured
+        # two-step graphs (`value` stays the direct launches, whose walk the
+        # HIP events time)
+        eng.set_params(flags=p.flags | S.FLAG_GRAPH)
+        eng.run(2)
+        eng.sync()
+        k = args.steps - args.steps % 2
+        tg0 = time.perf_counter()
+        eng.run(k)
+        tg1 = time.perf_counter()
+        graph_replay = {"value": n * k / (tg1 - tg0), "unit": "body-steps/s",
+                        "ms_per_step": (tg1 - tg0) / k * 1e3, "steps": k}
     eng.close()
 
     walk_ms = stages["forces"]
@@ -435,6 +449,7 @@ def main():
             "quadrupole_interactions": qinter,
         },
         "drift": drift,
+        "graph_replay": graph_replay,
     }
     if not args.no_reference_opening and args.opening != "reference":
         out["reference_opening"] = reference_opening_run(S, args, st, eps2)
