@@ -96,6 +96,10 @@ typedef struct stellar_params {
  * first use for each arrangement of the context's double buffers; an odd
  * last step runs directly).  Stage times are not recorded in graph runs. */
 #define STELLAR_FLAG_GRAPH 8
+/* Record HIP events around the forces stage only (two per step instead of
+ * eight): stellar_stage_times then reports the forces time and zeros.
+ * Ignored when STELLAR_FLAG_STAGE_TIMING is set. */
+#define STELLAR_FLAG_FORCES_TIMING 32
 
 typedef struct stellar_ctx stellar_ctx;
 
@@ -191,9 +195,10 @@ int stellar_export_forces(stellar_ctx* ctx, float* ax, float* ay,
 int stellar_last_interactions(stellar_ctx* ctx, int64_t* total);
 int stellar_last_quad_interactions(stellar_ctx* ctx, int64_t* total);
 
-/* Per-stage device milliseconds of the last step when
- * STELLAR_FLAG_STAGE_TIMING is set: [bbox, build, com, sort, forces,
- * update, output]. */
+/* Per-stage device milliseconds of the last step (mean over the steps of
+ * the last stellar_run) when STELLAR_FLAG_STAGE_TIMING is set: [bbox,
+ * build, com, sort, forces, update, output]; with STELLAR_FLAG_FORCES_TIMING
+ * only the forces entry. */
 int stellar_stage_times(stellar_ctx* ctx, float* ms7);
 
 /* ---- launch-shape autotuner (no counterpart in the reference) ---------- */

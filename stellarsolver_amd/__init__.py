@@ -38,6 +38,7 @@ FLAG_QUADRUPOLE = 2
 FLAG_TEXTBOOK_OPENING = 4
 FLAG_GRAPH = 8
 FLAG_BMAX_OPENING = 16
+FLAG_FORCES_TIMING = 32
 
 IC_DISK = 0
 IC_PLUMMER = 1

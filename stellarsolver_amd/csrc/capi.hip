@@ -102,7 +102,13 @@ int check_device(stellar_ctx* c) {
 }
 
 bool timing(const stellar_ctx* c) {
-  return (c->p.flags & STELLAR_FLAG_STAGE_TIMING) != 0;
+  return (c->p.flags & (STELLAR_FLAG_STAGE_TIMING | STELLAR_FLAG_FORCES_TIMING)) != 0;
+}
+
+// Only the events around the forces stage (marks 4 and 5) are recorded.
+bool forces_only(const stellar_ctx* c) {
+  return (c->p.flags & STELLAR_FLAG_STAGE_TIMING) == 0 &&
+         (c->p.flags & STELLAR_FLAG_FORCES_TIMING) != 0;
 }
 
 // Events of step `step` of the current run (step < 0: the single-step set).
@@ -111,7 +117,7 @@ hipEvent_t* step_events(stellar_ctx* c, int32_t step) {
 }
 
 void mark(stellar_ctx* c, hipEvent_t* ev, int k) {
-  if (ev) (void)hipEventRecord(ev[k], c->stream);
+  if (ev && (!forces_only(c) || k == 4 || k == 5)) (void)hipEventRecord(ev[k], c->stream);
 }
 
 int run_step(stellar_ctx* c, int32_t step) {
@@ -205,9 +211,11 @@ int collect_times(stellar_ctx* c, int32_t steps) {
   const int32_t nsets = steps < 0 ? 1 : steps;
   if (nsets == 0) return STELLAR_OK;
   double acc[7] = {};
+  const bool fo = forces_only(c);
   for (int32_t s = 0; s < nsets; ++s) {
     hipEvent_t* ev = step_events(c, steps < 0 ? -1 : s);
     for (int k = 0; k < 7; ++k) {
+      if (fo && k != 4) continue;
       float ms = 0.f;
       HIP_TRY(c, hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
       acc[k] += ms;

@@ -185,6 +185,22 @@ def test_radix_sort_path(S, oracle, graph):
         assert np.array_equal(got[k], want[k]), k
 
 
+def test_stage_timing_modes(S):
+    """Eight HIP events per step time every stage; the forces-only mode times
+    the walk alone and reports zeros for the rest."""
+    st = ic(2, 65536)
+    eps2 = float(np.float32(EPS[2] ** 2))
+    with make_engine(S, st, eps2, flags=S.FLAG_STAGE_TIMING) as e:
+        e.run(3)
+        full = e.stage_times()
+    with make_engine(S, st, eps2, flags=S.FLAG_FORCES_TIMING) as e:
+        e.run(3)
+        walk = e.stage_times()
+    assert all(full[k] > 0 for k in ("bbox", "build", "com", "sort", "forces", "update"))
+    assert walk["forces"] > 0
+    assert all(walk[k] == 0 for k in walk if k != "forces")
+
+
 def test_sort_cap_rejected(S):
     st = ic(0, 4096)
     with make_engine(S, st, 1e-4) as e:
