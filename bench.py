@@ -408,15 +408,18 @@ def main():
         # side figure: the same number of steps replayed from captured
         # two-step graphs (`value` stays the direct launches, whose walk the
         # HIP events time)
-        eng.set_params(flags=p.flags | S.FLAG_GRAPH)
-        eng.run(2)
-        eng.sync()
-        k = args.steps - args.steps % 2
-        tg0 = time.perf_counter()
-        eng.run(k)
-        tg1 = time.perf_counter()
-        graph_replay = {"value": n * k / (tg1 - tg0), "unit": "body-steps/s",
-                        "ms_per_step": (tg1 - tg0) / k * 1e3, "steps": k}
+        try:
+            eng.set_params(flags=p.flags | S.FLAG_GRAPH)
+            eng.run(2)
+            eng.sync()
+            k = args.steps - args.steps % 2
+            tg0 = time.perf_counter()
+            eng.run(k)
+            tg1 = time.perf_counter()
+            graph_replay = {"value": n * k / (tg1 - tg0), "unit": "body-steps/s",
+                            "ms_per_step": (tg1 - tg0) / k * 1e3, "steps": k}
+        except S.StellarError as err:  # a side figure: report it, keep the line
+            graph_replay = {"error": str(err)}
     eng.close()
 
     walk_ms = stages["forces"]
