@@ -296,6 +296,107 @@ def test_walk_matches_recursive_statement(oracle, group):
     assert np.sqrt(np.mean(rel ** 2)) < 5e-6
 
 
+# ---- the opening threshold, stated from the cell widths ----------------------
+def _anisotropic_state(seed=11, n=400):
+    """Bodies in [-1, 7] x [-0.5, 1.5] with the four extremes present: the box
+    holds the origin, its x extent (8) is four times its y extent (2)."""
+    rng = np.random.default_rng(seed)
+    x = np.concatenate([[-1.0, 7.0, 0.5, 3.0], rng.uniform(-1.0, 7.0, n - 4)]).astype(np.float32)
+    y = np.concatenate([[0.2, -0.1, -0.5, 1.5], rng.uniform(-0.5, 1.5, n - 4)]).astype(np.float32)
+    m = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    return {"x": x, "y": y, "mass": m}
+
+
+def _brute_force_counts(kt, x, y, bb, theta, eps2, group, textbook):
+    """Interactions per walk group from the acceptance rule stated with exact
+    rationals (kernels.cu:349,376,394-406): a cell at level L is W/2^L wide,
+    W = right - left (the box's x extent); a group accepts it when every body
+    has (W/2^L)^2/theta <= d^2 + eps2 (textbook: (W/2^L)^2/theta^2 <= d^2,
+    unsoftened); level-1 cells are always opened, bodies always accepted.
+    Also returns how many decisions lie within 1e-5 (relative) of the
+    threshold, where FP32 rounding could decide instead."""
+    n = x.size
+    W = Fraction(float(bb[1])) - Fraction(float(bb[0]))
+    th = Fraction(float(np.float32(theta)))
+    e2 = Fraction(float(np.float32(eps2)))
+    child, root, srt = kt["child"], kt["root"], kt["sorted"]
+    cx = [Fraction(float(v)) for v in kt["com_x"]]
+    cy = [Fraction(float(v)) for v in kt["com_y"]]
+    counts = np.zeros(n, np.int32)
+    borderline = 0
+    for s0 in range(0, n, group):
+        ids = srt[s0:s0 + group]
+        P = [(Fraction(float(x[b])), Fraction(float(y[b]))) for b in ids]
+        cnt = 0
+        todo = [(int(ch) - n, 1) for ch in root if ch >= n]  # (cell, level), opened
+        cnt += sum(1 for ch in root if 0 <= ch < n)
+        while todo:
+            c, lev = todo.pop()
+            for k in range(4):
+                ch = int(child[4 * c + k])
+                if ch < 0:
+                    continue
+                if ch < n:
+                    cnt += 1
+                    continue
+                c2 = ch - n
+                w = W / 2 ** (lev + 1)
+                thr = w * w / (th * th if textbook else th)
+                d2 = min((cx[c2] - px) ** 2 + (cy[c2] - py) ** 2 for px, py in P)
+                rhs = d2 if textbook else d2 + e2
+                if abs(rhs - thr) <= thr * Fraction(1, 100000):
+                    borderline += 1
+                if thr <= rhs:
+                    cnt += 1
+                else:
+                    todo.append((c2, lev + 1))
+        counts[ids] = cnt
+    return counts, borderline
+
+
+@pytest.mark.parametrize("textbook", [0, 1])
+@pytest.mark.parametrize("group", [1, 32])
+def test_opening_threshold_from_cell_widths(oracle, group, textbook):
+    """The walk's per-level thresholds (root depth from half the box's x
+    extent, a quarter per level) against the widths of the cells themselves,
+    on a box whose x and y extents differ: taking the y extent, or halving
+    instead of quartering per level, changes the counts."""
+    st = _anisotropic_state()
+    x, y, m = st["x"], st["y"], st["mass"]
+    bb, kt = _tree(oracle, st)
+    assert list(bb) == [-1.0, 7.0, -0.5, 1.5]
+    theta, eps2 = 0.5, 1e-4
+    _, _, it = oracle.walk(kt, x, y, m, bb, theta, eps2, group, opening=textbook)
+    bf, borderline = _brute_force_counts(kt, x, y, bb, theta, eps2, group, textbook)
+    assert borderline == 0
+    assert np.array_equal(it, bf)
+    # the rule is not vacuous here: the x-extent threshold accepts cells a
+    # y-extent threshold (16 times smaller) would not
+    assert 1 < it.min() < it.max() < x.size
+
+
+def test_step_is_the_composed_stages(oracle):
+    """or_step (the CPU baseline's timed step) equals bbox -> insertion build
+    -> FP32 CoM -> sort -> group-of-32 walk -> update, composed bit for bit,
+    and does not depend on the thread count."""
+    st = ic(1, 3000)
+    theta, eps2, dt, d = 0.5, 1e-3, 1e-3, 0.9
+    x, y, vx, vy, m = (np.asarray(st[k], np.float32).copy() for k in ("x", "y", "vx", "vy", "mass"))
+    bb = oracle.bbox(x, y)
+    t = oracle.build_insert(x, y, m, bb)
+    oracle.com_ref(t)
+    oracle.sort_ref(t)
+    ct = oracle.canonicalize(t)
+    ax, ay, _ = oracle.walk(ct, x, y, m, bb, theta, eps2, 32)
+    ex = oracle.update(x, y, vx, vy, ax, ay, dt, d)
+    for threads in (1, 4):
+        s = {"x": x.copy(), "y": y.copy(), "vx": vx.copy(), "vy": vy.copy(), "mass": m.copy()}
+        oracle.step(s, theta, eps2, dt, d, group=32, threads=threads)
+        assert np.array_equal(s["ax"], ax) and np.array_equal(s["ay"], ay)
+        for k, v in zip(("x", "y", "vx", "vy"), ex):
+            assert np.array_equal(s[k], v), k
+
+
 def test_walk_error_falls_with_theta_and_group_is_conservative(oracle):
     st = ic(2, 20000)
     bb, kt = _tree(oracle, st)
