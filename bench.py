@@ -38,9 +38,12 @@ VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 # algorithmic VALU issue slots of one body-node interaction: 2 sub, 2 FMA for
 # d^2+eps^2, 1 rsqrt at quarter rate (4 slots), 3 mul, 2 FMA accumulate
 OPS_PER_INTERACTION = 13
-# a cell's quadrupole term adds 14: rs^2, rs^5 (2), rs^7, Q.d (4), d.Q.d (2),
-# the radial factor (2), and one more FMA per component (2)
-OPS_PER_QUAD_TERM = 14
+# a cell's quadrupole term, in the combined form the walk evaluates
+# (rs^5 [(m r + 2.5 (d.Q.d) rs^2) d - Q.d]), issues 11 more than the monopole's
+# 3 multiplies (already in the 13): rs^2, rs^5 (2), Q.d (2 mul + 2 FMA),
+# d.Q.d (mul + FMA), the radial factor (2 mul + FMA), one FMA per component
+# (2), less the monopole's 3 multiplies
+OPS_PER_QUAD_TERM = 11
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per walk launch on the default workload (colliding disks, N = 2^20,
 # theta 0.5, quadrupoles + textbook opening, group 64), from two rocprofv3
@@ -102,7 +105,7 @@ def cpu_baseline(st, args, eps2):
     insertion build cannot separate (it splits until its pool overflows)."""
     from oracle import oracle as O
 
-    threads = args.cpu_threads if args.cpu_threads > 0 else min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads if args.cpu_threads > 0 else host_cores()
     times = []
     for k in range(args.cpu_steps + 1):
         state = {q: st[q].copy() for q in ("x", "y", "vx", "vy", "mass")}
@@ -123,6 +126,20 @@ def cpu_baseline(st, args, eps2):
                   f"1 warm-up, median {med * 1e3:.1f} ms/step, reference criterion, monopoles, "
                   "group 32, OpenMP walk, sequential insertion build",
     }
+
+
+def host_cores() -> int:
+    """Cores this process may run on: its CPU affinity, bounded by
+    OMP_NUM_THREADS when that is set (a host that shares its cores between
+    GPUs sets it to one GPU's share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
 
 
 WORKLOAD_NAMES = {0: "disk", 1: "Plummer a=1", 2: "colliding disks", 4: "deep Plummer a=0.01"}
@@ -471,12 +488,19 @@ def main():
         },
         "drift": drift,
         "graph_replay": graph_replay,
+        "output_buffer": "filled when read (stellar_download_positions), outside the timed "
+                         "steps; the reference fills it every step (about 5 us at N = 2^20)",
     }
     if not args.no_reference_opening and args.opening != "reference":
         out["reference_opening"] = reference_opening_run(S, args, st, eps2)
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(st, args, eps2)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        if "reference_opening" in out:
+            # the same force model on both sides (the CPU port runs the
+            # reference's criterion with monopoles)
+            out["gpu_over_cpu_same_force_model"] = (out["reference_opening"]["value"]
+                                                    / out["cpu_baseline"]["value"])
     print(json.dumps(out))
     return 0
 

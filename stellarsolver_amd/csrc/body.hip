@@ -15,17 +15,31 @@ inline unsigned grid_for(int64_t items, int threads) {
 // moves the body state itself into it, making every later access coalesced.
 // The build's link kernel already wrote x, y and m in tree order (it reads
 // them through perm for the tree records); this kernel gathers the rest.
+// After a failed build (sticky error bits) the tree and perm are incomplete:
+// the kernel then copies the state unchanged into the spare buffers, which
+// the host swaps in as after any sort, so a failed step moves no body.
 __global__ __launch_bounds__(256) void permute_kernel(
-    int32_t n, const uint32_t* __restrict__ perm, const float* __restrict__ vx,
-    const float* __restrict__ vy, const int32_t* __restrict__ id,
-    float* __restrict__ vx2, float* __restrict__ vy2, int32_t* __restrict__ id2,
-    DevScalars* __restrict__ dev) {
+    int32_t n, const uint32_t* __restrict__ perm, const float* __restrict__ x,
+    const float* __restrict__ y, const float* __restrict__ m, const float* __restrict__ vx,
+    const float* __restrict__ vy, const int32_t* __restrict__ id, float* __restrict__ x2,
+    float* __restrict__ y2, float* __restrict__ m2, float* __restrict__ vx2,
+    float* __restrict__ vy2, int32_t* __restrict__ id2, DevScalars* __restrict__ dev) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) {  // for the walk that follows
+  const bool failed = (__builtin_amdgcn_readfirstlane(dev->err) & kBuildErrors) != 0;
+  if (s == 0 && !failed) {  // for the walk that follows
     dev->interactions = 0;
     dev->quad_interactions = 0;
   }
   if (s >= n) return;
+  if (failed) {
+    x2[s] = x[s];
+    y2[s] = y[s];
+    m2[s] = m[s];
+    vx2[s] = vx[s];
+    vy2[s] = vy[s];
+    id2[s] = id[s];
+    return;
+  }
   const uint32_t p = perm[s];
   vx2[s] = vx[p];
   vy2[s] = vy[p];
@@ -34,11 +48,14 @@ __global__ __launch_bounds__(256) void permute_kernel(
 
 // update_kernel (kernels.cu:522-537): v += dt*a; x += d*dt*v, both as fused
 // multiply-adds with d*dt rounded first, as nvcc contracts the reference.
+// Skipped when the step failed (build errors, walk stack overflow): the
+// accelerations are then incomplete, and the state stays as it was.
 __global__ __launch_bounds__(256) void update_kernel(
     int32_t n, float* __restrict__ x, float* __restrict__ y,
     float* __restrict__ vx, float* __restrict__ vy,
     const float* __restrict__ ax, const float* __restrict__ ay, float dt,
-    float d) {
+    float d, const DevScalars* __restrict__ dev) {
+  if (__builtin_amdgcn_readfirstlane(dev->err) & (kBuildErrors | kErrStack)) return;
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const int32_t n4 = n >> 2;
   const float ddt = d * dt;
@@ -266,8 +283,9 @@ void launch_sort(stellar_ctx* c) {
   const int32_t n = c->n;
   if (n == 0) return;
   Buffers& B = c->b;
-  permute_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, B.perm, B.vx, B.vy, B.id,
-                                                           B.vx2, B.vy2, B.id2, B.dev);
+  permute_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, B.perm, B.x, B.y, B.m, B.vx, B.vy,
+                                                           B.id, B.x2, B.y2, B.m2, B.vx2, B.vy2,
+                                                           B.id2, B.dev);
   std::swap(B.x, B.x2);
   std::swap(B.y, B.y2);
   std::swap(B.vx, B.vx2);
@@ -282,7 +300,7 @@ void launch_update(stellar_ctx* c) {
   Buffers& B = c->b;
   const int32_t threads = (n >> 2) > 4 ? (n >> 2) : 4;
   update_kernel<<<grid_for(threads, 256), 256, 0, c->stream>>>(
-      n, B.x, B.y, B.vx, B.vy, B.ax, B.ay, c->p.timestep, c->p.dampening);
+      n, B.x, B.y, B.vx, B.vy, B.ax, B.ay, c->p.timestep, c->p.dampening, B.dev);
 }
 
 void launch_output(stellar_ctx* c) {

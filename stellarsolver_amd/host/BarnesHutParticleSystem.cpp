@@ -7,31 +7,34 @@
 #include <stdexcept>
 #include <vector>
 
-namespace {
-
-stellar_params to_params(const SimulationParameters& p) {
+stellar_params BarnesHutParticleSystem::contextParameters(const SimulationParameters& p,
+                                                          const StellarOptions& opt) {
   stellar_params s;
   stellar_default_params(&s);
-  s.theta = p.theta;
-  s.eps2 = p.eps2;
+  s.theta = opt.theta;
+  s.eps2 = opt.eps2;
   s.timestep = p.timestep;
   s.dampening = p.dampening;
   s.gravity = p.gravity;
-  s.group = p.group;
-  s.flags = (p.quadrupole ? STELLAR_FLAG_QUADRUPOLE : 0) |
-            (p.textbook ? STELLAR_FLAG_TEXTBOOK_OPENING : 0);
+  s.group = opt.group;
+  s.flags = (opt.quadrupole ? STELLAR_FLAG_QUADRUPOLE : 0) |
+            (opt.textbook ? STELLAR_FLAG_TEXTBOOK_OPENING : 0);
   return s;
 }
 
-}  // namespace
-
 BarnesHutParticleSystem::BarnesHutParticleSystem(const SimulationParameters p, const int n)
-    : ParticleSystem(p, n), numParticles_(n), step_(0), ctx_(nullptr), last_ms_(0.f) {
+    : BarnesHutParticleSystem(p, n, StellarOptions()) {}
+
+BarnesHutParticleSystem::BarnesHutParticleSystem(const SimulationParameters p, const int n,
+                                                 const StellarOptions& opt)
+    : ParticleSystem(p, n), options_(opt), numParticles_(n), step_(0), ctx_(nullptr),
+      last_ms_(0.f) {
   create();
 }
 
 BarnesHutParticleSystem::BarnesHutParticleSystem(const BarnesHutParticleSystem& system)
     : ParticleSystem(system.parameters, system.numParticles_),
+      options_(system.options_),
       numParticles_(system.numParticles_),
       step_(system.step_),
       ctx_(nullptr),
@@ -50,6 +53,7 @@ BarnesHutParticleSystem& BarnesHutParticleSystem::operator=(const BarnesHutParti
   if (this != &system) {
     BarnesHutParticleSystem tmp(system);
     std::swap(parameters, tmp.parameters);
+    std::swap(options_, tmp.options_);
     std::swap(numParticles_, tmp.numParticles_);
     std::swap(step_, tmp.step_);
     std::swap(ctx_, tmp.ctx_);
@@ -64,7 +68,7 @@ BarnesHutParticleSystem::~BarnesHutParticleSystem() {
 }
 
 void BarnesHutParticleSystem::create() {
-  const stellar_params sp = to_params(parameters);
+  const stellar_params sp = contextParameters(parameters, options_);
   const int rc = stellar_create(&sp, numParticles_, 0, &ctx_);
   if (rc != STELLAR_OK) {
     ctx_ = nullptr;
@@ -97,15 +101,15 @@ void BarnesHutParticleSystem::update() {
 // BarnesHutParticleSystem::reset (BarnesHutParticleSystem.cpp:350-372).
 void BarnesHutParticleSystem::reset() {
   stellar_ic_params ic;
-  if (parameters.workload < 0) {
+  if (options_.workload < 0) {
     stellar_ic_defaults(0, &ic);
     ic.model = parameters.model == plummer_model          ? STELLAR_IC_REF_PLUMMER
                : parameters.model == colliding_disk_model ? STELLAR_IC_REF_COLLIDING
                                                           : STELLAR_IC_REF_DISK;
   } else {
-    if (stellar_ic_defaults(parameters.workload, &ic) != STELLAR_OK)
+    if (stellar_ic_defaults(options_.workload, &ic) != STELLAR_OK)
       throw std::runtime_error("unknown workload");
-    if (parameters.seed_given) ic.seed = parameters.seed;
+    if (options_.seed_given) ic.seed = options_.seed;
   }
   ic.gravity = parameters.gravity;
   const size_t n = (size_t)numParticles_;

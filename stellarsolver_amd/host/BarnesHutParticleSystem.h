@@ -16,11 +16,15 @@
 #include <vector>
 
 #include "ParticleSystem.h"
+#include "StellarOptions.h"
 #include "stellar.h"
 
 class BarnesHutParticleSystem : public ParticleSystem {
  public:
+  // The reference's constructor: options at their defaults (theta 0.5,
+  // eps2 0.025 as the reference hard-codes them; walk group 64).
   BarnesHutParticleSystem(const SimulationParameters p, const int n);
+  BarnesHutParticleSystem(const SimulationParameters p, const int n, const StellarOptions& opt);
   BarnesHutParticleSystem(const BarnesHutParticleSystem& system);
   BarnesHutParticleSystem& operator=(const BarnesHutParticleSystem& system);
   ~BarnesHutParticleSystem() override;
@@ -35,12 +39,17 @@ class BarnesHutParticleSystem : public ParticleSystem {
   // Milliseconds of the last update() (the value -benchmark prints).
   float lastStepMilliseconds() const { return last_ms_; }
   stellar_ctx* context() { return ctx_; }
+  const StellarOptions& options() const { return options_; }
+  // The library parameters a system built from (p, opt) runs with.
+  static stellar_params contextParameters(const SimulationParameters& p,
+                                          const StellarOptions& opt = StellarOptions());
 
  private:
   void create();
   void check(int status, const char* what) const;
   void runDebugTests();
 
+  StellarOptions options_;
   int numParticles_;
   int step_;
   stellar_ctx* ctx_;

@@ -17,6 +17,7 @@
 
 #include "BarnesHutParticleSystem.h"
 #include "SimulationParameters.h"
+#include "StellarOptions.h"
 
 namespace {
 
@@ -67,21 +68,22 @@ int main(int argc, char** argv) {
   parameters.timestep = getCmdLineParameterFloat(argc, argv, "timestep", 0.001f);
   parameters.gravity = getCmdLineParameterFloat(argc, argv, "gravity", 1.0f);
   parameters.dampening = getCmdLineParameterFloat(argc, argv, "dampening", 1.0f);
-  parameters.theta = getCmdLineParameterFloat(argc, argv, "theta", 0.5f);
+  StellarOptions options;
+  options.theta = getCmdLineParameterFloat(argc, argv, "theta", options.theta);
   const char* eps = cmdLineValue(argc, argv, "eps");
-  parameters.eps2 = eps ? (float)(std::atof(eps) * std::atof(eps)) : 0.025f;
-  parameters.group = getCmdLineParameterInt(argc, argv, "group", 64);
-  parameters.workload = getCmdLineParameterInt(argc, argv, "workload", -1);
+  if (eps) options.eps2 = (float)(std::atof(eps) * std::atof(eps));
+  options.group = getCmdLineParameterInt(argc, argv, "group", options.group);
+  options.workload = getCmdLineParameterInt(argc, argv, "workload", options.workload);
   const char* seed = cmdLineValue(argc, argv, "seed");
-  parameters.seed_given = seed != nullptr;
-  parameters.seed = seed ? (uint64_t)std::strtoull(seed, nullptr, 10) : 0;
-  parameters.quadrupole = checkCmdLineFlag(argc, argv, "quad");
-  parameters.textbook = checkCmdLineFlag(argc, argv, "textbook");
+  options.seed_given = seed != nullptr;
+  if (seed) options.seed = (uint64_t)std::strtoull(seed, nullptr, 10);
+  options.quadrupole = checkCmdLineFlag(argc, argv, "quad");
+  options.textbook = checkCmdLineFlag(argc, argv, "textbook");
   if (parameters.opengl)
     std::cerr << "note: no renderer in this build; -opengl only fetches the output buffer "
                  "every step" << std::endl;
   try {
-    BarnesHutParticleSystem particles(parameters, numbodies);
+    BarnesHutParticleSystem particles(parameters, numbodies, options);
     particles.reset();
     double total_ms = 0.0;
     for (int i = 0; i < (int)parameters.iterations; i++) {

@@ -319,6 +319,44 @@ def test_error_codes(S):
         e.update()  # a full step still works after a refused stage
 
 
+@pytest.mark.parametrize("how", ["step", "run"])
+def test_failed_step_leaves_state_untouched(S, how):
+    """A step whose build overflows the node pool reports STELLAR_ERR_POOL and
+    moves no body: the state downloaded afterwards is the one uploaded (the
+    sort stage copies it unchanged, the update is skipped)."""
+    st = ic(1, 5000)
+    with make_engine(S, st, 0.01, cap=100) as e:
+        with pytest.raises(S.StellarError) as err:
+            e.update() if how == "step" else e.run(3)
+        assert err.value.status == S.STELLAR_ERR_POOL
+        d = e.download()
+    for k in ("x", "y", "vx", "vy", "mass"):
+        assert np.array_equal(d[k], st[k]), k
+
+
+def test_changed_force_model_needs_a_new_tree(S):
+    """The build and centre-of-mass stages store per-child thresholds and
+    moments for the force model: after theta, eps2 or the opening/multipole
+    flags change, stellar_forces refuses the old tree (STELLAR_ERR_STATE);
+    timing flags keep it."""
+    st = ic(2, 8192)
+    with make_engine(S, st, 0.01) as e:
+        for change in ({"theta": 0.4}, {"eps2": 2e-4}, {"flags": S.FLAG_QUADRUPOLE},
+                       {"flags": S.FLAG_TEXTBOOK_OPENING}):
+            e.set_params(theta=0.5, eps2=1e-4, flags=0)
+            e.bbox()
+            e.build()
+            e.com()
+            e.sort()
+            e.set_params(flags=S.FLAG_FORCES_TIMING)
+            e.set_params(flags=0)
+            e.forces()  # timing flags do not touch the tree
+            e.set_params(**change)
+            with pytest.raises(S.StellarError) as err:
+                e.forces()
+            assert err.value.status == S.STELLAR_ERR_STATE, change
+
+
 def test_direct_and_energy_fp64(S, oracle):
     st = ic(0, 4096)
     with make_engine(S, st, 1e-4) as e:
