@@ -38,6 +38,7 @@ extern "C" {
 #define STELLAR_ERR_DEVICE (-6)   /* HIP runtime error                        */
 #define STELLAR_ERR_STATE (-7)    /* stage called out of order                */
 #define STELLAR_ERR_BUCKET (-8)   /* >4 bodies share a deepest-level cell     */
+#define STELLAR_ERR_SIZE (-9)     /* input above an allocation-free maximum    */
 
 /* Deepest cell level: bodies that share all 64 slot codes (coincident or
  * closer than FP32 midpoints can separate) are kept, up to four, in one
@@ -159,6 +160,26 @@ int stellar_step(stellar_ctx* ctx);
 int stellar_run(stellar_ctx* ctx, int32_t steps);
 /* Wait for queued work and report any sticky device error. */
 int stellar_sync(stellar_ctx* ctx);
+
+/* ---- allocation-free bounding box --------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the bounding box of stellar_bbox -- {left, right,
+ * bottom, top} of n bodies with the origin included (kernels.cu:62-114) --
+ * over caller-owned device arrays, with no context.  x, y: device pointers
+ * to n floats; bbox4: device pointer to 4 floats (the result); stream: the
+ * hipStream_t to launch on (NULL: the null stream).  Asynchronous: bbox4
+ * holds the result once the stream's work before the next synchronisation
+ * completes.  The per-tile partial results live in a fixed device array of
+ * the library (STELLAR_BBOX_NOALLOC_MAX_TILES x 16 bytes on each GPU it runs
+ * on, held until the process ends); nothing is allocated or freed, and
+ * n above STELLAR_BBOX_NOALLOC_MAX_BODIES returns STELLAR_ERR_SIZE.  All
+ * calls on one GPU share that array: calls that overlap in time on one GPU
+ * (two streams, two threads) race and must be serialised by the caller. */
+#define STELLAR_BBOX_NOALLOC_MAX_TILES 4096
+#define STELLAR_BBOX_NOALLOC_MAX_BODIES (STELLAR_BBOX_NOALLOC_MAX_TILES * 4096)
+int stellar_bbox_noalloc(const float* x, const float* y, int32_t n, float* bbox4,
+                         void* stream);
 
 /* ---- inspection --------------------------------------------------------- */
 /* Bounding box {left, right, bottom, top} of the last stellar_bbox

@@ -29,6 +29,8 @@ STELLAR_ERR_DEPTH = -5
 STELLAR_ERR_DEVICE = -6
 STELLAR_ERR_STATE = -7
 STELLAR_ERR_BUCKET = -8
+STELLAR_ERR_SIZE = -9
+BBOX_NOALLOC_MAX_BODIES = 4096 * 4096
 STELLAR_MAX_LEVEL = 64
 
 PRECISION_FP32 = 0
@@ -114,6 +116,7 @@ SIGNATURES = {
     "stellar_download": (C.c_int, [_P, _F, _F, _F, _F, _F, _F, _F]),
     "stellar_download_positions": (C.c_int, [_P, _F]),
     "stellar_bbox": (C.c_int, [_P]),
+    "stellar_bbox_noalloc": (C.c_int, [_P, _P, C.c_int32, _P, _P]),
     "stellar_build": (C.c_int, [_P]),
     "stellar_com": (C.c_int, [_P]),
     "stellar_sort": (C.c_int, [_P]),

@@ -22,6 +22,8 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
+#include <mutex>
+
 #include "engine.h"
 
 namespace stellar {
@@ -47,11 +49,14 @@ __device__ __forceinline__ float mx(float a, float v) { return v > a ? v : a; }
 // launches instead of a last-block ticket: the ticket pattern needs a
 // device-scope release per block, which on gfx950 writes back the issuing
 // XCD's L2.
+// Tile bounds come from tstart, or (tstart == nullptr) are the uniform tiles
+// [k*kBboxTile, (k+1)*kBboxTile) of n positions.
 __global__ __launch_bounds__(kBboxThreads) void bbox_kernel(
     const float* __restrict__ x, const float* __restrict__ y,
-    const int32_t* __restrict__ tstart, float4* __restrict__ part) {
+    const int32_t* __restrict__ tstart, int32_t n, float4* __restrict__ part) {
   float l = 0.f, r = 0.f, b = 0.f, t = 0.f;
-  const int32_t lo = tstart[blockIdx.x], hi = tstart[blockIdx.x + 1];
+  const int32_t lo = tstart ? tstart[blockIdx.x] : (int32_t)blockIdx.x * kBboxTile;
+  const int32_t hi = tstart ? tstart[blockIdx.x + 1] : (n - lo < kBboxTile ? n : lo + kBboxTile);
   for (int32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
     const float px = x[i], py = y[i];
     l = mn(l, px);
@@ -600,7 +605,7 @@ void launch_init_scalars(stellar_ctx* c) {
 void launch_bbox(stellar_ctx* c) {
   if (c->n == 0) return;
   Buffers& B = c->b;
-  bbox_kernel<<<(unsigned)c->ntiles, kBboxThreads, 0, c->stream>>>(B.x, B.y, B.tstart,
+  bbox_kernel<<<(unsigned)c->ntiles, kBboxThreads, 0, c->stream>>>(B.x, B.y, B.tstart, c->n,
                                                                     B.bbox_part);
   const unsigned waves_per_block = kBboxThreads / 64;
   bbox_final_kernel<<<(unsigned)((c->ninst + waves_per_block - 1) / waves_per_block),
@@ -673,4 +678,77 @@ size_t mintree_bytes(int32_t n) {
   return (size_t)b;
 }
 
+// ---- allocation-free bounding box (stellar_bbox_noalloc) -----------------
+namespace {
+
+constexpr int kNoAllocTiles = STELLAR_BBOX_NOALLOC_MAX_TILES;
+static_assert((int64_t)STELLAR_BBOX_NOALLOC_MAX_BODIES == (int64_t)kNoAllocTiles * kBboxTile,
+              "the header's body limit is whole tiles");
+
+// Per-tile partial boxes of stellar_bbox_noalloc: the library's fixed device
+// array (one copy on each GPU, at its own address).  Host code never passes
+// this name to a kernel: it resolves each device's address once and uses that.
+static __device__ __align__(256) float4 scratch[kNoAllocTiles];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+float4* scratch_ptr[kMaxDevices] = {};
+
+// Address of `scratch` on the current device (resolved once per device).
+int scratch_on_device(float4** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return STELLAR_ERR_DEVICE;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return STELLAR_ERR_DEVICE;
+    scratch_ptr[dev] = static_cast<float4*>(p);
+  }
+  *out = scratch_ptr[dev];
+  return STELLAR_OK;
+}
+
+// One wavefront folds ntiles partial boxes (none: the origin alone) into out4.
+__global__ __launch_bounds__(64) void bbox_fold_kernel(const float4* __restrict__ part,
+                                                       int32_t ntiles, float* __restrict__ out4) {
+  const int lane = threadIdx.x;
+  float l = 0.f, r = 0.f, b = 0.f, t = 0.f;
+  for (int32_t i = lane; i < ntiles; i += 64) {
+    const float4 q = part[i];
+    l = mn(l, q.x);
+    r = mx(r, q.y);
+    b = mn(b, q.z);
+    t = mx(t, q.w);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    l = mn(l, __shfl_xor(l, off));
+    r = mx(r, __shfl_xor(r, off));
+    b = mn(b, __shfl_xor(b, off));
+    t = mx(t, __shfl_xor(t, off));
+  }
+  if (lane == 0) {
+    out4[0] = l;
+    out4[1] = r;
+    out4[2] = b;
+    out4[3] = t;
+  }
+}
+
+}  // namespace
+
 }  // namespace stellar
+
+extern "C" int stellar_bbox_noalloc(const float* x, const float* y, int32_t n, float* bbox4,
+                                    void* stream) {
+  using namespace stellar;
+  if (n < 0 || !bbox4 || (n > 0 && (!x || !y))) return STELLAR_ERR_ARG;
+  if ((int64_t)n > (int64_t)STELLAR_BBOX_NOALLOC_MAX_BODIES) return STELLAR_ERR_SIZE;
+  float4* part = nullptr;
+  const int rc = scratch_on_device(&part);
+  if (rc != STELLAR_OK) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int32_t ntiles = (int32_t)(((int64_t)n + kBboxTile - 1) / kBboxTile);
+  if (ntiles > 0) bbox_kernel<<<(unsigned)ntiles, kBboxThreads, 0, s>>>(x, y, nullptr, n, part);
+  bbox_fold_kernel<<<1, 64, 0, s>>>(part, ntiles, bbox4);
+  return hipGetLastError() == hipSuccess ? STELLAR_OK : STELLAR_ERR_DEVICE;
+}

@@ -27,6 +27,7 @@ const char* status_text(int s) {
     case STELLAR_ERR_DEVICE: return "HIP runtime error";
     case STELLAR_ERR_STATE: return "stage called out of order";
     case STELLAR_ERR_BUCKET: return "more than 4 bodies share a level-64 cell";
+    case STELLAR_ERR_SIZE: return "input larger than the allocation-free entry point's maximum";
     default: return "unknown status";
   }
 }
