@@ -228,8 +228,11 @@ typedef struct stellar_tuning {
   int32_t nodes_per_pop; /* group 64: tree nodes fetched per round trip      */
   int32_t block_threads; /* group 64: threads per walk workgroup             */
   float walk_ms;         /* mean walk time of the choice when measured       */
+  int32_t schedule;      /* group 64: STELLAR_WALK_STATIC or _QUEUE          */
 } stellar_tuning;
-/* Time the tree walk for every candidate launch shape on the current state
+/* Time the tree walk for every candidate launch shape and schedule (the
+ * static launch and the persistent work queue, see stellar_set_walk_schedule)
+ * on the current state
  * (reps timed walks each, after one untimed) and keep the fastest for this
  * context.  group_mask selects the walk-group sizes tried: 1 = 64, 2 = 32,
  * 4 = 16; 0 means 64 only, which keeps the set of interactions unchanged
@@ -253,6 +256,23 @@ int stellar_debug_sort_cap(stellar_ctx* ctx, int32_t cap);
  * 128, 256, 512 or 1024. */
 int stellar_set_walk_shape(stellar_ctx* ctx, int32_t nodes_per_pop,
                            int32_t block_threads);
+
+/* Scheduling of the group-64 walk (no counterpart in the reference, whose
+ * force kernel is one grid-stride launch, kernels.cu:352-435 with the fixed
+ * shape of Particle_cuda.cu:5-6):
+ *   STELLAR_WALK_STATIC      one workgroup per WAVES groups, each XCD one
+ *                            contiguous range of workgroups (default);
+ *   STELLAR_WALK_QUEUE       persistent: the resident workgroups take units
+ *                            of one group per wavefront from per-XCD work
+ *                            queues, then from the other XCDs' queues;
+ *   STELLAR_WALK_QUEUE_WAVE  the same with every wavefront claiming its own
+ *                            groups.
+ * The set of interactions is the same in every schedule.  Queues apply to
+ * workgroups of 256 or 512 threads; other shapes run the static launch. */
+#define STELLAR_WALK_STATIC 0
+#define STELLAR_WALK_QUEUE 1
+#define STELLAR_WALK_QUEUE_WAVE 2
+int stellar_set_walk_schedule(stellar_ctx* ctx, int32_t schedule);
 
 /* Message for the last non-OK status on this context. */
 const char* stellar_last_error(const stellar_ctx* ctx);

@@ -30,6 +30,7 @@ STELLAR_ERR_DEVICE = -6
 STELLAR_ERR_STATE = -7
 STELLAR_ERR_BUCKET = -8
 STELLAR_ERR_SIZE = -9
+WALK_STATIC, WALK_QUEUE, WALK_QUEUE_WAVE = 0, 1, 2
 BBOX_NOALLOC_MAX_BODIES = 4096 * 4096
 STELLAR_MAX_LEVEL = 64
 
@@ -89,6 +90,7 @@ class StellarTuning(C.Structure):
         ("nodes_per_pop", C.c_int32),
         ("block_threads", C.c_int32),
         ("walk_ms", C.c_float),
+        ("schedule", C.c_int32),
     ]
 
 
@@ -142,6 +144,7 @@ SIGNATURES = {
     "stellar_diagnostics_fp64": (C.c_int, [_P, C.c_double, C.c_double, C.POINTER(StellarDiagnostics)]),
     "stellar_autotune": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(StellarTuning)]),
     "stellar_set_walk_shape": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "stellar_set_walk_schedule": (C.c_int, [_P, C.c_int32]),
     "stellar_debug_sort_cap": (C.c_int, [_P, C.c_int32]),
     "stellar_ic_defaults": (C.c_int, [C.c_int32, C.POINTER(StellarICParams)]),
     "stellar_generate_ic": (C.c_int, [C.POINTER(StellarICParams), C.c_int32, _F, _F, _F, _F, _F]),
@@ -419,7 +422,11 @@ class Engine:
         self._check(self._lib.stellar_autotune(self._h, int(group_mask), int(reps), C.byref(t)), "autotune")
         self.params.group = t.group
         return {"group": t.group, "nodes_per_pop": t.nodes_per_pop,
-                "block_threads": t.block_threads, "walk_ms": t.walk_ms}
+                "block_threads": t.block_threads, "walk_ms": t.walk_ms, "schedule": t.schedule}
+
+    def set_walk_schedule(self, schedule: int) -> None:
+        """WALK_STATIC, WALK_QUEUE or WALK_QUEUE_WAVE (stellar_set_walk_schedule)."""
+        self._check(self._lib.stellar_set_walk_schedule(self._h, int(schedule)), "set_walk_schedule")
 
     def set_walk_shape(self, nodes_per_pop: int, block_threads: int) -> None:
         self._check(self._lib.stellar_set_walk_shape(self._h, int(nodes_per_pop), int(block_threads)),

@@ -26,9 +26,10 @@ __global__ __launch_bounds__(256) void permute_kernel(
     float* __restrict__ vy2, int32_t* __restrict__ id2, DevScalars* __restrict__ dev) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   const bool failed = (__builtin_amdgcn_readfirstlane(dev->err) & kBuildErrors) != 0;
-  if (s == 0 && !failed) {  // for the walk that follows
+  if (s == 0 && !failed) {  // for the walk that follows: counters and queues
     dev->interactions = 0;
     dev->quad_interactions = 0;
+    for (int x = 0; x < kXcds; ++x) dev->wq[x] = 0u;
   }
   if (s >= n) return;
   if (failed) {
@@ -123,6 +124,7 @@ __global__ __launch_bounds__(256) void output_kernel(
 __global__ void zero_counter_kernel(DevScalars* dev) {
   dev->interactions = 0;
   dev->quad_interactions = 0;
+  for (int x = 0; x < kXcds; ++x) dev->wq[x] = 0u;
 }
 
 // Positions scattered to body-id order (for diagnostics by id).

@@ -982,6 +982,16 @@ int stellar_set_walk_shape(stellar_ctx* c, int32_t nodes_per_pop, int32_t block_
   return STELLAR_OK;
 }
 
+int stellar_set_walk_schedule(stellar_ctx* c, int32_t schedule) {
+  if (!c) return STELLAR_ERR_ARG;
+  if (schedule != STELLAR_WALK_STATIC && schedule != STELLAR_WALK_QUEUE &&
+      schedule != STELLAR_WALK_QUEUE_WAVE)
+    return fail(c, STELLAR_ERR_ARG, "walk schedule");
+  c->walk_queue = schedule;
+  drop_graphs(c);
+  return STELLAR_OK;
+}
+
 int stellar_autotune(stellar_ctx* c, int32_t group_mask, int32_t reps, stellar_tuning* out) {
   if (!c || reps < 1 || group_mask < 0 || group_mask > 7) return STELLAR_ERR_ARG;
   if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
@@ -997,6 +1007,7 @@ int stellar_autotune(stellar_ctx* c, int32_t group_mask, int32_t reps, stellar_t
     c->p.group = t.group;
     c->walk_kpop = t.nodes_per_pop;
     c->walk_waves = t.block_threads / 64;
+    c->walk_queue = t.schedule;
     drop_graphs(c);
     if (out) *out = t;
   };
@@ -1024,18 +1035,22 @@ int stellar_autotune(stellar_ctx* c, int32_t group_mask, int32_t reps, stellar_t
   if (mask & 1)
     for (int kp : {1, 2, 4})
       for (int w : {1, 2, 4, 8, 16})
-        if (!((quad || (c->p.flags & STELLAR_FLAG_BMAX_OPENING)) && kp == 4))
-          cand.push_back(stellar_tuning{64, kp, 64 * w, 0.f});
-  if (mask & 2) cand.push_back(stellar_tuning{32, 1, 256, 0.f});
-  if (mask & 4) cand.push_back(stellar_tuning{16, 1, 256, 0.f});
+        if (!((quad || (c->p.flags & STELLAR_FLAG_BMAX_OPENING)) && kp == 4)) {
+          cand.push_back(stellar_tuning{64, kp, 64 * w, 0.f, STELLAR_WALK_STATIC});
+          if (w == 4 || w == 8)
+            cand.push_back(stellar_tuning{64, kp, 64 * w, 0.f, STELLAR_WALK_QUEUE});
+        }
+  if (mask & 2) cand.push_back(stellar_tuning{32, 1, 256, 0.f, STELLAR_WALK_STATIC});
+  if (mask & 4) cand.push_back(stellar_tuning{16, 1, 256, 0.f, STELLAR_WALK_STATIC});
   const stellar_params saved = c->p;
-  const int skp = c->walk_kpop, sw = c->walk_waves;
+  const int skp = c->walk_kpop, sw = c->walk_waves, sq = c->walk_queue;
   stellar_tuning best{};
   best.walk_ms = 3.4e38f;
   for (const stellar_tuning& t : cand) {
     c->p.group = t.group;
     c->walk_kpop = t.nodes_per_pop;
     c->walk_waves = t.block_threads / 64;
+    c->walk_queue = t.schedule;
     launch_forces(c, true);  // untimed
     HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
     for (int32_t r = 0; r < reps; ++r) launch_forces(c, true);
@@ -1046,6 +1061,7 @@ int stellar_autotune(stellar_ctx* c, int32_t group_mask, int32_t reps, stellar_t
       c->p = saved;
       c->walk_kpop = skp;
       c->walk_waves = sw;
+      c->walk_queue = sq;
       return rc;
     }
     float ms = 0.f;

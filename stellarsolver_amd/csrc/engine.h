@@ -90,7 +90,8 @@ struct DevScalars {
   float bbox[4];         // unused (per-instance boxes live in Buffers::ibbox)
   unsigned long long interactions;       // body-node interactions of the last walk
   unsigned long long quad_interactions;  // of which with a cell's quadrupole term
-  uint32_t pad[52];
+  uint32_t wq[8];  // walk work queues: next unit of each XCD's range (zeroed before each walk)
+  uint32_t pad[44];
 };
 static_assert(sizeof(DevScalars) == 256, "one 256-byte line");
 
@@ -194,6 +195,10 @@ struct stellar_ctx {
   // wavefronts per workgroup (stellar_autotune)
   int walk_kpop = 0;   // 0: default (2 one-block nodes, 1 two-block nodes)
   int walk_waves = 0;  // 0: default (8)
+  // walk scheduling: 0 one launch over static XCD ranges; 1 persistent
+  // workgroups claiming units of one group per wavefront from per-XCD queues;
+  // 2 the same with each wavefront claiming its own groups
+  int walk_queue = 0;
   int32_t sort_cap = stellar::kSsCapMax;  // keys per block pass of a sort bucket
   bool ss_have_spl = false;  // splitters from the previous sort are in place
   bool radix_sort = false;   // testing: the device-wide radix sort at any size

@@ -33,6 +33,9 @@
 //
 // QUAD adds each accepted cell's quadrupole term (a second 64-byte block of
 // {Qxx, Qxy, Qyy, threshold} per node); bodies stay monopoles.
+#include <map>
+#include <mutex>
+
 #include "engine.h"
 
 namespace stellar {
@@ -131,12 +134,16 @@ constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
 // 1 + 3*kMaxLevel; popping KPOP at once can hold more, so pushes past the
 // end are dropped and the walk stops with kErrStack.
 constexpr int kStack64 = 1024;
+// SGPRs per wavefront at which a SIMD still holds eight walk wavefronts on
+// MI355X (measured: 81 SGPRs let only six in, whatever the compiler's
+// occupancy estimate says).
+constexpr int kWalkSgprs = 80;
 
 // One walk group on one wavefront: the traversal of group g, its forces and
 // interaction counts.  The tree arrays are separate const __restrict__
 // arguments: that is what lets the compiler read them through the scalar
 // data cache.
-template <bool QUAD, bool THR, int KPOP>
+template <bool QUAD, bool THR, int KPOP, int STACK = kStack64>
 __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
                                              const float4* __restrict__ qblk,
                                              const WalkArgs& A, int64_t g,
@@ -242,12 +249,12 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
         if (__builtin_expect(__builtin_amdgcn_ballot_w64(!(t <= r)) == 0, 1)) {
           acc.term(QUAD && code > 0, cmv, r, dx, dy, qm[kQ ? jj : 0][k]);
         } else {
-          if (lane == 0 && sp < kStack64) st[sp] = (uint32_t)code | child_lvl;
+          if (lane == 0 && sp < STACK) st[sp] = (uint32_t)code | child_lvl;
           ++sp;
         }
       }
     }
-    if (sp > kStack64) {
+    if (sp > STACK) {
       if (lane == 0) atomicOr(&A.dev->err, kErrStack);
       break;
     }
@@ -271,7 +278,7 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
 // Workgroups are mapped so that each XCD walks one contiguous range of
 // groups (xcd_block), which its L2 serves.
 template <bool QUAD, bool THR, int KPOP, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_num_sgpr(kWalkSgprs))) void walk64_kernel(
     const ChildRec* __restrict__ blk, const float4* __restrict__ qblk,
     WalkArgs A) {
   __shared__ uint32_t stk[WAVES][kStack64];
@@ -280,6 +287,58 @@ __global__ __launch_bounds__(64 * WAVES) void walk64_kernel(
   if (g >= A.ngroups) return;
   if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
   walk64_group<QUAD, THR, KPOP>(blk, qblk, A, g, stk[wave]);
+}
+
+// Persistent form: exactly the resident workgroups, each taking work from
+// per-XCD queues.  XCD x owns a contiguous range of units (as the one-shot
+// launch gives it a contiguous range of workgroups); a workgroup first drains
+// its own XCD's queue, then the other XCDs' in turn, so an XCD that finishes
+// early takes the last units of the slower ones.
+//   CLAIM_WAVE = false: a unit is WAVES consecutive groups, claimed by the
+//     workgroup (its wavefronts start each unit together and share node
+//     visits through the scalar cache);
+//   CLAIM_WAVE = true: every wavefront claims one group at a time, so no
+//     wavefront waits for the others of its workgroup.
+// The queues (dev->wq) are zeroed before each walk; claims past a range's
+// end are harmless, and every workgroup stops after one failed claim on each
+// of the eight queues.  Which workgroup sits on which XCD follows the
+// round-robin dispatch (workgroup b on XCD b % 8).
+// The claim word takes the last stack entry of the last wavefront, so that the
+// workgroup's LDS stays at 4 KiB per wavefront (four 512-thread workgroups fit
+// a CU).
+constexpr int kStackQ = kStack64 - 1;
+template <bool QUAD, bool THR, int KPOP, int WAVES, bool CLAIM_WAVE>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_num_sgpr(kWalkSgprs))) void walk64_queue_kernel(
+    const ChildRec* __restrict__ blk, const float4* __restrict__ qblk, WalkArgs A) {
+  __shared__ uint32_t stk[WAVES * kStack64];
+  uint32_t& claim = stk[WAVES * kStack64 - 1];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
+  const int64_t units = CLAIM_WAVE ? A.ngroups : (A.ngroups + WAVES - 1) / WAVES;
+  const int x0 = (int)(blockIdx.x % kXcds);
+  int k = 0;
+  while (k < kXcds) {
+    const int x = (x0 + k) % kXcds;
+    const int64_t lo = units * x / kXcds, hi = units * (x + 1) / kXcds;
+    int64_t u;
+    if (CLAIM_WAVE) {
+      uint32_t v = 0;
+      if (lane == 0) v = atomicAdd(&A.dev->wq[x], 1u);
+      u = lo + (int64_t)__builtin_amdgcn_readfirstlane(v);
+    } else {
+      if (threadIdx.x == 0) claim = atomicAdd(&A.dev->wq[x], 1u);
+      __syncthreads();
+      u = lo + (int64_t)claim;
+      __syncthreads();  // every wavefront has read the claim before the next one
+    }
+    if (u >= hi) {
+      ++k;
+      continue;
+    }
+    const int64_t g = CLAIM_WAVE ? u : u * WAVES + wave;
+    if (g < A.ngroups) walk64_group<QUAD, THR, KPOP, kStackQ>(blk, qblk, A, g, stk + wave * kStack64);
+  }
 }
 
 // ----------------------------------------------------------- G = 32, 16 --
@@ -397,10 +456,55 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
   }
 }
 
+// Workgroups of a kernel that are resident at once on the whole device.
+int resident_blocks(const void* kernel, int threads) {
+  static std::mutex mu;
+  static std::map<const void*, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(kernel);
+  if (it != cache.end()) return it->second;
+  int dev = 0, per_cu = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = kXcds;
+  const int nb = per_cu * cus;
+  cache[kernel] = nb;
+  return nb;
+}
+
+template <bool QUAD, bool THR, int KPOP, int WAVES, bool CLAIM_WAVE>
+void launch64_queue(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
+  auto k = walk64_queue_kernel<QUAD, THR, KPOP, WAVES, CLAIM_WAVE>;
+  const int64_t units = CLAIM_WAVE ? ngroups : (ngroups + WAVES - 1) / WAVES;
+  int64_t grid = resident_blocks(reinterpret_cast<const void*>(k), 64 * WAVES);
+  if (getenv("PROBE_GRID")) grid = atoi(getenv("PROBE_GRID")) / WAVES;
+  if (grid > units) grid = units;
+  k<<<(unsigned)grid, 64 * WAVES, 0, c->stream>>>(A.blk, A.qblk, A);
+}
+
 template <bool QUAD, bool THR, int KPOP>
 void launch64(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
   // default shape (measured on MI355X, N = 2^20): 8 wavefronts per workgroup
   const int w = c->walk_waves > 0 ? c->walk_waves : 8;
+  if (c->walk_queue != 0 && (w == 4 || w == 8)) {
+    const bool per_wave = c->walk_queue == 2;
+    if (w == 8) {
+      if (per_wave)
+        launch64_queue<QUAD, THR, KPOP, 8, true>(c, ngroups, A);
+      else
+        launch64_queue<QUAD, THR, KPOP, 8, false>(c, ngroups, A);
+    } else {
+      if (per_wave)
+        launch64_queue<QUAD, THR, KPOP, 4, true>(c, ngroups, A);
+      else
+        launch64_queue<QUAD, THR, KPOP, 4, false>(c, ngroups, A);
+    }
+    return;
+  }
   const unsigned grid = (unsigned)((ngroups + w - 1) / w);
   switch (w) {
     case 1:

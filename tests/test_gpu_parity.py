@@ -449,13 +449,17 @@ def test_quadrupole_parity(S, oracle, workload, n, textbook):
     assert rms_rel(f["ax"], f["ay"], ax, ay) < FORCE_RMS_TOL
 
 
-@pytest.mark.parametrize("kpop,threads", [(1, 512), (2, 512), (4, 256), (1, 1024), (2, 64)])
+@pytest.mark.parametrize("kpop,threads,schedule", [(1, 512, 0), (2, 512, 0), (4, 256, 0), (1, 1024, 0),
+                                                   (2, 64, 0), (1, 512, 1), (2, 256, 1),
+                                                   (1, 512, 2), (2, 256, 2)])
 @pytest.mark.parametrize("model", ["reference", "textbook+quad", "bmax+quad"])
-def test_walk_shapes_parity(S, oracle, model, kpop, threads):
-    """Every walk shape the tuner may pick (nodes per pop, workgroup size)
-    makes the oracle's decisions: interaction counts exact, forces within the
-    rsqrt tolerance."""
-    print(f"walk shape parity model={model} kpop={kpop} threads={threads}", flush=True)
+def test_walk_shapes_parity(S, oracle, model, kpop, threads, schedule):
+    """Every walk shape and schedule the tuner may pick (nodes per pop,
+    workgroup size, static launch or persistent work queues) makes the
+    oracle's decisions: interaction counts exact, forces within the rsqrt
+    tolerance."""
+    print(f"walk shape parity model={model} kpop={kpop} threads={threads} schedule={schedule}",
+          flush=True)
     st = ic(2, 65536)
     eps2 = float(np.float32(EPS[2] ** 2))
     flags = {"reference": 0,
@@ -463,10 +467,12 @@ def test_walk_shapes_parity(S, oracle, model, kpop, threads):
              "bmax+quad": S.FLAG_QUADRUPOLE | S.FLAG_BMAX_OPENING}[model]
     with make_engine(S, st, eps2, flags=flags) as e:
         e.set_walk_shape(kpop, threads)
+        e.set_walk_schedule(schedule)
         gbb, gt = gpu_tree(e)
         e.sort()
         e.forces()
         f = e.export_forces()
+        total = e.last_interactions()
     obb, ot = oracle_tree(oracle, st)
     x, y, m = st["x"], st["y"], st["mass"]
     kw = {}
@@ -479,6 +485,7 @@ def test_walk_shapes_parity(S, oracle, model, kpop, threads):
         kw = {"opening": 1, "quad": True}
     ax, ay, it = oracle.walk(ot, x, y, m, obb, 0.5, eps2, 64, **kw)
     assert np.array_equal(f["inter"], it)
+    assert total == int(it.astype(np.int64).sum())  # every group walked once
     assert rms_rel(f["ax"], f["ay"], ax, ay) < FORCE_RMS_TOL
 
 
