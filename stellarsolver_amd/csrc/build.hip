@@ -122,37 +122,6 @@ __global__ void init_scalars_kernel(DevScalars* dev) {
   dev->quad_interactions = 0;
 }
 
-// One level of box subdivision along one axis (kernels.cu:144-157): the
-// FP32 sum lo+hi is halved in double, p < mid is decided in double and the
-// new bound is mid rounded to FP32.  When |lo+hi| >= 2^-125 the halving is
-// exact in FP32, so the FP32 path gives the identical decision and bound.
-__device__ __forceinline__ uint32_t split1(float p, float& lo, float& hi) {
-  const float s = lo + hi;
-  bool lt;
-  float mid;
-  if (fabsf(s) >= 0x1p-125f) {
-    mid = s * 0.5f;
-    lt = p < mid;
-  } else {
-    const double md = 0.5 * (double)s;
-    lt = (double)p < md;
-    mid = (float)md;
-  }
-  if (lt)
-    hi = mid;
-  else
-    lo = mid;
-  return lt ? 1u : 0u;
-}
-
-// Slot code of the next level: +1 west (x < mid), +2 south (y < mid).
-__device__ __forceinline__ uint32_t split2(float px, float py, float& l,
-                                           float& r, float& b, float& t) {
-  const uint32_t cx = split1(px, l, r);
-  const uint32_t cy = split1(py, b, t);
-  return cx | (cy << 1);
-}
-
 // Sort keys.  With one instance a key holds slot codes 1..32.  With several,
 // its top ibits bits hold the instance index, so that one sort orders
 // (instance, path) and every instance keeps its own position range, and the
@@ -174,14 +143,7 @@ __global__ __launch_bounds__(256) void keys_kernel(
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t inst = inst_of[i];
-  const float4 bb = ibbox[inst];
-  float l = bb.x, r = bb.y, b = bb.z, t = bb.w;
-  const float px = x[i], py = y[i];
-  uint64_t k = 0;
-#pragma unroll 4
-  for (int lev = 0; lev < l1; ++lev) k = (k << 2) | split2(px, py, l, r, b, t);
-  if (ibits > 0) k |= (uint64_t)inst << (64 - ibits);
-  key[i] = k;
+  key[i] = path_key(x[i], y[i], ibbox[inst], inst, l1, ibits);
   perm[i] = (uint32_t)i;
 }
 
@@ -293,15 +255,30 @@ MinTree mintree_layout(int32_t n, const uint8_t* lam, const uint8_t* lmin,
   return t;
 }
 
-// Cells opened by pair (s, s+1): levels lam(s-1)+1 .. lam(s).  Also writes
-// lam and, per wavefront of 64 positions, level 1 of the minimum tree, and
-// clears tree blocks [0, nclear) with coalesced 16-byte stores (the cells
-// kernel clears any block from n on).
-__global__ __launch_bounds__(256) void lambda_kernel(
-    int32_t n, KeyView K, const int32_t* __restrict__ inst_of,
-    int32_t* __restrict__ ncell, uint8_t* __restrict__ lam,
-    uint8_t* __restrict__ lmin1, float4* __restrict__ blk, int64_t nclear) {
+// Cells opened by pair (s, s+1): levels lam(s-1)+1 .. lam(s).  Writes lam,
+// the exclusive prefix of the cell counts within each block of kScanBlock
+// positions (clocal) and each block's total (bsum; the minimum-tree kernel
+// scans the totals, so coff(s) = bsum[s / kScanBlock] + clocal[s] is the id
+// of the first cell pair s opens), and, per wavefront of 64 positions, level
+// 1 of the minimum tree.  Also clears tree blocks [0, nclear) with coalesced
+// 16-byte stores (nclear = n: a tree has fewer cells than bodies unless many
+// bodies pair up closely; the minimum-tree kernel clears any blocks beyond).
+constexpr int kScanBlock = 256;
+
+__global__ __launch_bounds__(kScanBlock) void lambda_kernel(
+    int32_t n, KeyView K, const int32_t* __restrict__ inst_of, int32_t* __restrict__ clocal,
+    int32_t* __restrict__ bsum, uint8_t* __restrict__ lam, uint8_t* __restrict__ lmin1,
+    float4* __restrict__ blk, int64_t nclear) {
+  __shared__ int32_t wsum[kScanBlock / 64];
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  {
+    const int64_t r0 = 4 * (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t q = r0 + (int64_t)r * blockDim.x;
+      if (q < 4 * nclear) blk[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
   int v = 0, lm = 0;
   if (s + 1 < n) {
     // a pair that straddles two instances shares no level (lam = 0, like the
@@ -311,26 +288,68 @@ __global__ __launch_bounds__(256) void lambda_kernel(
     const int prev = s > 0 && inst_of[s - 1] == is ? common_levels(K, s - 1, s) : 0;
     v = lm > prev ? lm - prev : 0;
   }
-  if (s < n) {
-    ncell[s] = v;
-    lam[s] = (uint8_t)lm;
-  }
-  const int64_t r0 = 4 * (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int64_t q = r0 + (int64_t)r * blockDim.x;
-    if (q < 4 * nclear) blk[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-  }
+  if (s < n) lam[s] = (uint8_t)lm;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lmin1) {
     int w = s < n ? lm : 255;
     for (int off = 32; off > 0; off >>= 1) w = min(w, __shfl_xor(w, off));
-    if ((threadIdx.x & 63) == 0 && s < n) lmin1[s >> 6] = (uint8_t)w;
+    if (lane == 0 && s < n) lmin1[s >> 6] = (uint8_t)w;
   }
+  int incl = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int o = __shfl_up(incl, off);
+    if (lane >= off) incl += o;
+  }
+  if (lane == 63) wsum[wv] = incl;
+  __syncthreads();
+  int before = incl - v, tot = 0;
+  for (int w = 0; w < kScanBlock / 64; ++w) {
+    if (w < wv) before += wsum[w];
+    tot += wsum[w];
+  }
+  if (s < n) clocal[s] = before;
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
 // Levels 2..top of the minimum tree (one workgroup; each entry reads its 64
-// children as four 16-byte loads).
-__global__ __launch_bounds__(1024) void mintree_kernel(MinTree t) {
+// children as four 16-byte loads), and the exclusive scan of the lambda
+// kernel's per-block cell counts (in place, nblk of them).
+__global__ __launch_bounds__(1024) void mintree_kernel(MinTree t, int32_t* __restrict__ bsum,
+                                                       int32_t nblk, float4* __restrict__ blk,
+                                                       int64_t cleared, int64_t max_blocks,
+                                                       int32_t nroot) {
+  {
+    __shared__ int32_t part[1024 / 64];
+    const int per = (nblk + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int b0 = (int)threadIdx.x * per;
+    const int b1 = b0 + per < nblk ? b0 + per : nblk;
+    int mine = 0;
+    for (int j = b0; j < b1; ++j) mine += bsum[j];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int incl = mine;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(incl, off);
+      if (lane >= off) incl += o;
+    }
+    if (lane == 63) part[wv] = incl;
+    __syncthreads();
+    int run = incl - mine;
+    for (int w = 0; w < wv; ++w) run += part[w];
+    for (int j = b0; j < b1; ++j) {
+      const int c = bsum[j];
+      bsum[j] = run;
+      run += c;
+    }
+    // blocks beyond those the lambda kernel cleared, up to the last cell's
+    // (the last thread's running total is the cell count)
+    __shared__ int32_t total;
+    if (threadIdx.x == blockDim.x - 1) total = run;
+    __syncthreads();
+    int64_t used = (int64_t)total + nroot;
+    if (used > max_blocks) used = max_blocks;
+    for (int64_t q = 4 * cleared + threadIdx.x; q < 4 * used; q += blockDim.x)
+      blk[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   for (int k = 2; k <= t.top; ++k) {
     const uint8_t* src = t.lv[k - 1];
     uint8_t* dst = const_cast<uint8_t*>(t.lv[k]);
@@ -416,8 +435,9 @@ struct LinkArgs {
   int32_t n;
   int64_t cap;
   KeyView K;
-  const int32_t* coff;
-  const int32_t* ncell;
+  int32_t* coff;
+  const int32_t* bpre;    // exclusive prefix of each block's cell count
+  const int32_t* clocal;  // exclusive prefix within the block
   const uint8_t* lam;
   const uint32_t* perm;
   const float* x;
@@ -433,9 +453,8 @@ struct LinkArgs {
   const int32_t* ioff;
   int32_t nroot;  // number of instances = root blocks; cell c is block nroot + c
   MinTree mt;
-  // second block per node (quadrupoles / bmax): child slot thresholds, a
-  // cell's level threshold (the bmax pass replaces it), -1 for a body, which
-  // every body of a group accepts
+  // second block per node (quadrupoles / bmax): a cell child's opening
+  // threshold (its level threshold; the bmax pass replaces it)
   float4* qblk;
   int qthr;
   const float4* ibbox;
@@ -453,69 +472,32 @@ __device__ __forceinline__ void set_cell_threshold(const LinkArgs& A, int64_t sl
   reinterpret_cast<float*>(A.qblk + slot_index)[3] = A.unsoft ? t + A.eps2 : t;
 }
 
-// One thread per adjacent pair s: creates the cells that pair opens, at
-// levels lam(s-1)+1 .. lam(s), with ids coff[s] + k, and links each cell but
-// the first into the one before it (its parent).  The thread of the last
-// position of an instance (no pair) records the root's metadata; that of
-// position n-1 also the cell count.
-__global__ __launch_bounds__(256) void cells_kernel(LinkArgs A) {
-  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= A.n) return;
-  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-  const int32_t inst = A.inst_of[s];
-  if (s == A.n - 1 || A.inst_of[s + 1] != inst) {
-    if (s == A.n - 1) {
-      const int64_t total = (int64_t)A.coff[s] + A.ncell[s];
-      if (total > A.cap) atomicOr(&A.dev->err, kErrPool);
-      A.dev->ncells = (int32_t)(total < A.cap ? total : A.cap);
-    }
-    A.cstart[inst] = A.ioff[inst];
-    A.clevel[inst] = 0;
-    A.cparent[inst] = -1;
-    return;
-  }
-  const int nc = A.ncell[s];
-  if (nc == 0) return;
-  const int lp = s > 0 ? A.lam[s - 1] : 0;
-  const int64_t base = A.coff[s];
-  // level threshold (before eps2) of the cell at level lp + 1 + k, k = 1 ...
-  float lt = 0.f;
-  if (A.qthr && nc > 1) lt = walk_level_threshold(walk_root_depth(A.ibbox[inst], A.theta_div),
-                                                  lp + 1, 0.f, false);
-  for (int k = 0; k < nc; ++k) {
-    const int64_t c = base + k;
-    if (c >= A.cap) break;
-    const int32_t b = (int32_t)(c + A.nroot);
-    const int L = lp + 1 + k;
-    if (b >= A.n) {  // blocks below n were cleared by the lambda kernel
-      float4* d = reinterpret_cast<float4*>(A.blk + 4 * (int64_t)b);
-      d[0] = zero;
-      d[1] = zero;
-      d[2] = zero;
-      d[3] = zero;
-    }
-    A.cstart[b] = s;
-    A.clevel[b] = (uint8_t)L;
-    const int dg = digit_at(A.K, s, L);
-    A.cslot[b] = (uint8_t)dg;
-    if (k > 0) {
-      A.cparent[b] = b - 1;
-      A.blk[4 * (int64_t)(b - 1) + dg] = ChildRec{0.f, 0.f, 0.f, b};
-      if (A.qthr) {
-        lt = 0.25f * lt;
-        set_cell_threshold(A, 4 * (int64_t)(b - 1) + dg, lt);
-      }
-    }
-  }
+// Id of the first cell pair a opens.
+__device__ __forceinline__ int64_t coff_at(const LinkArgs& A, int32_t a) {
+  return (int64_t)A.bpre[a / kScanBlock] + A.clocal[a];
 }
 
-// One thread per tree position s: links body s, and the first cell s opens,
-// into their parents.  The node at level lam(s-1) holding s-1 and s starts
-// at a = psv(s-1) + 1 and was opened by pair a.  Body s sits in the deepest
-// node holding another body: s's own last cell when lam(s) > lam(s-1), else
-// that node; its slot is its code one level down, or its rank in a level-64
-// cell.
-__global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
+__device__ __forceinline__ void put_cell(const LinkArgs& A, int64_t blk, int slot, int64_t child,
+                                         int child_level, float depth0) {
+  A.blk[4 * blk + slot] = ChildRec{0.f, 0.f, 0.f, (int32_t)child};
+  if (A.qthr)
+    set_cell_threshold(A, 4 * blk + slot, walk_level_threshold(depth0, child_level, 0.f, false));
+}
+
+// The tree's node blocks and the cells' metadata; one thread per tree
+// position s.  Nodes are the instance's root (level 0, block inst) and its
+// cells (block nroot + id); their blocks were cleared by the lambda and
+// minimum-tree kernels, so a slot nobody writes is empty.  With lp = lam(s-1)
+// (0 at the instance's first position) and ls = lam(s), position s
+//   A. creates the nodes that start at s -- levels lp+1 .. ls, and the root
+//      at the instance's first position -- and writes each one's first child
+//      (the next node created, or body s);
+//   B. unless it is the instance's first position, starts a new child of the
+//      node at level lp that holds s-1 and s (found by a previous-smaller
+//      search over lam) and writes it: its first created cell, or body s.
+// A slot's digit is the position's slot code one level down, or in a level-64
+// cell (bodies closer than 64 levels resolve) its rank.
+__global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= A.n) return;
   // body s's record values, read once through the permutation and also
@@ -525,53 +507,78 @@ __global__ __launch_bounds__(256) void link_kernel(LinkArgs A) {
   A.x2[s] = bx;
   A.y2[s] = by;
   A.m2[s] = bm;
-  const int lp = s > 0 ? A.lam[s - 1] : 0;  // 0 at an instance's first position
-  const int nc = s + 1 < A.n ? A.ncell[s] : 0;
-  int64_t pid = -1;  // cell id of the level-lp node (-1: the instance's root)
-  int32_t a = 0;
-  if (lp > 0) {
+  const ChildRec body{bx, by, bm, -1 - s};
+  const int32_t inst = A.inst_of[s];
+  const int32_t i0 = A.ioff[inst];
+  const bool first = s == i0;
+  const int lp = first ? 0 : A.lam[s - 1];
+  const int ls = A.lam[s];
+  const int64_t cs = coff_at(A, s);
+  A.coff[s] = (int32_t)cs;
+  if (s == A.n - 1) {
+    if (cs > A.cap) atomicOr(&A.dev->err, kErrPool);
+    A.dev->ncells = (int32_t)(cs < A.cap ? cs : A.cap);
+  }
+  const float depth0 = A.qthr ? walk_root_depth(A.ibbox[inst], A.theta_div) : 0.f;
+  // block of the node created at s at level L (-1: beyond the pool)
+  auto created = [&](int L) -> int64_t {
+    if (L == 0) return inst;
+    const int64_t id = cs + (L - lp - 1);
+    return id < A.cap ? id + A.nroot : -1;
+  };
+
+  // B's node: level lp, holding s-1 and s, starting at position a (its
+  // opening pair), la = lam(a-1)
+  int32_t a = i0;
+  int la = 0;
+  int64_t pblk = inst;
+  if (!first && lp > 0) {
     a = psv(A.mt, s - 1, (uint32_t)lp) + 1;
-    const int la = a > 0 ? A.lam[a - 1] : 0;
-    pid = (int64_t)A.coff[a] + (lp - la - 1);
+    la = a > 0 ? A.lam[a - 1] : 0;
+    const int64_t pid = coff_at(A, a) + (lp - la - 1);
+    pblk = pid < A.cap ? pid + A.nroot : -1;
   }
-  const bool pok = pid < A.cap;
-  const int32_t pblk = pid < 0 ? A.inst_of[s] : (int32_t)(pid + A.nroot);
-  if (nc > 0) {
-    const int64_t c0 = A.coff[s];
-    if (c0 < A.cap && pok) {
-      const int32_t b = (int32_t)(c0 + A.nroot);
-      A.cparent[b] = pblk;
-      const int dg = digit_at(A.K, s, lp + 1);
-      A.blk[4 * (int64_t)pblk + dg] = ChildRec{0.f, 0.f, 0.f, b};
-      if (A.qthr)
-        set_cell_threshold(A, 4 * (int64_t)pblk + dg,
-                           walk_level_threshold(walk_root_depth(A.ibbox[A.inst_of[s]], A.theta_div),
-                                                lp + 1, 0.f, false));
-    }
-  }
-  int64_t bid;  // block of body s's parent
-  int slot;
-  if (nc > 0) {
-    const int64_t c = (int64_t)A.coff[s] + nc - 1;
-    if (c >= A.cap) return;
-    bid = c + A.nroot;
-    const int lev = lp + nc;
-    slot = lev < kMaxLevel ? digit_at(A.K, s, lev + 1) : 0;
-  } else {
-    if (!pok) return;
-    bid = pblk;
-    if (lp < kMaxLevel) {
-      slot = digit_at(A.K, s, lp + 1);
-    } else {
-      slot = s - a;
-      if (slot > 3) {
-        atomicOr(&A.dev->err, kErrBucket);
-        return;
+
+  // A: nodes created at s
+  if (first || ls > lp) {
+    for (int L = first ? 0 : lp + 1; L <= ls; ++L) {
+      const int64_t b = created(L);
+      if (b < 0) break;
+      if (L == 0) {
+        A.cstart[inst] = i0;
+        A.clevel[inst] = 0;
+        A.cparent[inst] = -1;
+      } else {
+        A.cstart[b] = s;
+        A.clevel[b] = (uint8_t)L;
+        A.cslot[b] = (uint8_t)digit_at(A.K, s, L);
+        A.cparent[b] = (int32_t)(L == lp + 1 && !first ? pblk : created(L - 1));
+      }
+      const int d = L < kMaxLevel ? digit_at(A.K, s, L + 1) : 0;
+      if (L < ls) {
+        const int64_t cb = created(L + 1);
+        if (cb >= 0) put_cell(A, b, d, cb, L + 1, depth0);
+      } else {
+        A.blk[4 * b + d] = body;
       }
     }
   }
-  A.blk[4 * bid + slot] = ChildRec{bx, by, bm, -1 - s};
-  if (A.qthr) A.qblk[4 * bid + slot] = make_float4(0.f, 0.f, 0.f, -1.f);
+
+  // B: s starts a new child of the node at level lp
+  if (!first && pblk >= 0) {
+    const int dcur = lp < kMaxLevel ? digit_at(A.K, s, lp + 1) : s - a;
+    if (dcur > 3) {
+      atomicOr(&A.dev->err, kErrBucket);
+    } else {
+      if (ls > lp) {
+        const int64_t cb = created(lp + 1);
+        if (cb >= 0) put_cell(A, pblk, dcur, cb, lp + 1, depth0);
+      } else {
+        A.blk[4 * pblk + dcur] = body;
+      }
+    }
+  }
+
 }
 
 inline unsigned grid_for(int64_t items, int threads) {
@@ -593,8 +600,6 @@ size_t build_scratch_bytes(int32_t n) {
   (void)rocprim::exclusive_scan(nullptr, b, q, q, 0, (size_t)n,
                                 rocprim::plus<int32_t>());
   if (a2 > a) a = a2;
-  const size_t c = sample_sort_scan_bytes(n);
-  if (c > a) a = c;
   return (a > b ? a : b) + 256;
 }
 
@@ -625,12 +630,18 @@ void launch_build(stellar_ctx* c) {
   if (n == 0) return;
   const int ibits = key_instance_bits(c->ninst);
   const int l1 = 32 - ibits / 2;
-  keys_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, n, B.inst_of, B.ibbox,
-                                                        l1, ibits, B.key, B.perm);
   size_t bytes = B.tmp_bytes;
   const bool ss = sample_sort_fits(n) && !c->radix_sort;
+  // with the previous sort's splitters at hand, the keys kernel of the key
+  // sort also counts the keys per bucket
+  const bool fused = ss && c->ss_have_spl;
+  if (fused)
+    launch_keys_count(c, l1, ibits);
+  else
+    keys_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, n, B.inst_of, B.ibbox,
+                                                          l1, ibits, B.key, B.perm);
   if (ss) {
-    launch_sample_sort(c);  // sorted data back in (key, perm)
+    launch_sample_sort(c, fused);  // sorted data back in (key, perm)
   } else {
     rocprim::double_buffer<uint64_t> kb(B.key, B.key_alt);
     rocprim::double_buffer<uint32_t> vb(B.perm, B.perm_alt);
@@ -650,17 +661,18 @@ void launch_build(stellar_ctx* c) {
   const KeyView K{B.key, B.key2, B.key3, l1, ibits};
   int64_t mtb = 0;
   const MinTree mt = mintree_layout(n, B.lam, B.lmin, &mtb);
-  const int64_t nclear = (int64_t)n < c->cap + c->ninst ? (int64_t)n : c->cap + c->ninst;
-  lambda_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(
-      n, K, B.inst_of, B.ncell, B.lam, mt.top >= 1 ? const_cast<uint8_t*>(mt.lv[1]) : nullptr,
-      reinterpret_cast<float4*>(B.blk), nclear);
-  if (mt.top >= 2) mintree_kernel<<<1, 1024, 0, c->stream>>>(mt);
-  bytes = B.tmp_bytes;
-  (void)rocprim::exclusive_scan(B.tmp, bytes, B.ncell, B.coff, 0, (size_t)n,
-                                rocprim::plus<int32_t>(), c->stream);
+  const int32_t nblk = (int32_t)grid_for(n, kScanBlock);
+  const int64_t max_blocks = c->cap + c->ninst;
+  const int64_t nclear = (int64_t)n < max_blocks ? (int64_t)n : max_blocks;
+  lambda_kernel<<<nblk, kScanBlock, 0, c->stream>>>(
+      n, K, B.inst_of, B.ncell, B.bsum, B.lam,
+      mt.top >= 1 ? const_cast<uint8_t*>(mt.lv[1]) : nullptr, reinterpret_cast<float4*>(B.blk),
+      nclear);
+  mintree_kernel<<<1, 1024, 0, c->stream>>>(mt, B.bsum, nblk, reinterpret_cast<float4*>(B.blk),
+                                            nclear, max_blocks, c->ninst);
   const float theta = c->p.theta;
   const bool textbook = (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) != 0;
-  LinkArgs A{n,         c->cap,   K,       B.coff,    B.ncell,  B.lam,
+  LinkArgs A{n,         c->cap,   K,       B.coff,    B.bsum,   B.ncell,  B.lam,
              B.perm,    B.x,      B.y,     B.m,       B.blk,    B.cstart,
              B.cparent, B.clevel, B.cslot, B.dev,     B.inst_of, B.ioff,
              c->ninst,  mt,
@@ -668,8 +680,7 @@ void launch_build(stellar_ctx* c) {
              (c->p.flags & (STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_BMAX_OPENING)) ? 1 : 0,
              B.ibbox,   textbook ? theta * theta : theta, c->p.eps2, textbook ? 1 : 0,
              B.x2,      B.y2,     B.m2};
-  cells_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(A);
-  link_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(A);
+  link_kernel<<<nblk, kScanBlock, 0, c->stream>>>(A);
 }
 
 size_t mintree_bytes(int32_t n) {

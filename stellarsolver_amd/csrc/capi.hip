@@ -71,13 +71,13 @@ void free_all(stellar_ctx* c) {
   void* ptrs[] = {B.x,     B.y,      B.vx,      B.vy,     B.m,      B.id,
                   B.x2,    B.y2,     B.vx2,     B.vy2,    B.m2,     B.id2,
                   B.ax,    B.ay,     B.inter,   B.key,    B.key_alt, B.perm,
-                  B.perm_alt, B.key2, B.ncell,  B.coff,   B.tmp,    B.blk,
+                  B.perm_alt, B.key2, B.ncell,  B.coff,   B.tmp,    B.blk, B.bsum,
                   B.cstart, B.lam, B.cparent, B.clevel, B.cslot,
                   B.csum,  B.out,    B.dev,
                   B.bbox_part, B.scount, B.choff,
                   B.lmin,  B.qblk,  B.spine,   B.clist, B.clen,
                   B.key3,  B.ioff,  B.inst_of, B.ibbox, B.tstart, B.itile,
-                  B.gtab[0], B.gtab[1], B.gtab[2], B.ss_spl, B.ss_hist, B.ss_off};
+                  B.gtab[0], B.gtab[1], B.gtab[2], B.ss_spl, B.ss_tot, B.ss_bstart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (B.host_dev) (void)hipHostFree(B.host_dev);
@@ -338,15 +338,15 @@ int create_ctx(const stellar_params* p, int32_t ninst, const int32_t* counts,
   chk(dmalloc(&B.perm, N));
   chk(dmalloc(&B.perm_alt, N));
   if (sample_sort_fits((int32_t)N)) {
-    const size_t hb = (size_t)sample_sort_buckets((int32_t)N) * (size_t)sample_sort_tiles((int32_t)N);
     chk(dmalloc(&B.ss_spl, (size_t)kSsMaxBuckets));
-    chk(dmalloc(&B.ss_hist, hb));
-    chk(dmalloc(&B.ss_off, hb));
+    chk(dmalloc(&B.ss_tot, 2 * (size_t)kSsMaxBuckets));
+    chk(dmalloc(&B.ss_bstart, (size_t)kSsMaxBuckets + 1));
   }
   chk(dmalloc(&B.key2, N));
   chk(dmalloc(&B.key3, N));
   chk(dmalloc(&B.ncell, N));
   chk(dmalloc(&B.coff, N));
+  chk(dmalloc(&B.bsum, N / 256 + 2));
   chk(dmalloc(&B.lam, (size_t)N + 64));
   chk(dmalloc(&B.lmin, mintree_bytes((int32_t)N)));
   B.tmp_bytes = build_scratch_bytes((int32_t)N);
@@ -398,6 +398,9 @@ int create_ctx(const stellar_params* p, int32_t ninst, const int32_t* counts,
   chk(hipMemsetAsync(B.blk, 0, 4 * NB * sizeof(ChildRec), c->stream));
   chk(hipMemsetAsync(B.qblk, 0, 4 * NB * sizeof(float4), c->stream));
   chk(hipMemsetAsync(B.ibbox, 0, (size_t)ninst * sizeof(float4), c->stream));
+  // the key sort's bucket totals and fill levels start at zero (its bucket
+  // kernel clears them again after each use)
+  if (B.ss_tot) chk(hipMemsetAsync(B.ss_tot, 0, 2 * (size_t)kSsMaxBuckets * sizeof(uint32_t), c->stream));
   chk(hipMemsetAsync(B.ax, 0, N * sizeof(float), c->stream));
   chk(hipMemsetAsync(B.ay, 0, N * sizeof(float), c->stream));
   chk(hipMemsetAsync(B.inter, 0, N * sizeof(int32_t), c->stream));

@@ -54,6 +54,50 @@ __device__ __forceinline__ float walk_level_threshold(float depth0, int level, f
   return unsoft ? thr + eps2 : thr;
 }
 
+// One level of box subdivision along one axis (kernels.cu:144-157): the
+// FP32 sum lo+hi is halved in double, p < mid is decided in double and the
+// new bound is mid rounded to FP32.  When |lo+hi| >= 2^-125 the halving is
+// exact in FP32, so the FP32 path gives the identical decision and bound.
+__device__ __forceinline__ uint32_t split1(float p, float& lo, float& hi) {
+  const float s = lo + hi;
+  bool lt;
+  float mid;
+  if (fabsf(s) >= 0x1p-125f) {
+    mid = s * 0.5f;
+    lt = p < mid;
+  } else {
+    const double md = 0.5 * (double)s;
+    lt = (double)p < md;
+    mid = (float)md;
+  }
+  if (lt)
+    hi = mid;
+  else
+    lo = mid;
+  return lt ? 1u : 0u;
+}
+
+// Slot code of the next level: +1 west (x < mid), +2 south (y < mid).
+__device__ __forceinline__ uint32_t split2(float px, float py, float& l,
+                                           float& r, float& b, float& t) {
+  const uint32_t cx = split1(px, l, r);
+  const uint32_t cy = split1(py, b, t);
+  return cx | (cy << 1);
+}
+
+// Sort key of one body: the slot codes of levels 1..l1 below the instance's
+// box, with the instance index in the top ibits bits (ibits = 0: one
+// instance, l1 = 32).
+__device__ __forceinline__ uint64_t path_key(float px, float py, float4 bb, int32_t inst, int l1,
+                                             int ibits) {
+  float l = bb.x, r = bb.y, b = bb.z, t = bb.w;
+  uint64_t k = 0;
+#pragma unroll 4
+  for (int lev = 0; lev < l1; ++lev) k = (k << 2) | split2(px, py, l, r, b, t);
+  if (ibits > 0) k |= (uint64_t)inst << (64 - ibits);
+  return k;
+}
+
 // MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs
 // (workgroup b to XCD b % 8), each XCD with its own L2.  xcd_block maps
 // hardware workgroup ids to logical ones so that each XCD gets one
@@ -117,12 +161,13 @@ struct Buffers {
   uint64_t *key = nullptr, *key_alt = nullptr;  // level 1..32 codes
   uint32_t *perm = nullptr, *perm_alt = nullptr;
   uint64_t* ss_spl = nullptr;   // sample sort: splitters
-  uint32_t* ss_hist = nullptr;  // sample sort: per-workgroup bucket counts
-  uint32_t* ss_off = nullptr;   // their exclusive scan
+  uint32_t* ss_tot = nullptr;     // sample sort: bucket totals, then fill levels
+  uint32_t* ss_bstart = nullptr;  // bucket starts (nb + 1)
   uint64_t* key2 = nullptr;   // 32 deeper levels (valid inside tie runs)
   uint32_t* key3 = nullptr;   // the deepest levels when the key holds instances
   int32_t* ncell = nullptr;   // cells opened by each adjacent pair
   int32_t* coff = nullptr;    // exclusive scan of ncell
+  int32_t* bsum = nullptr;    // cells opened per block of 256 positions, then their scan
   uint8_t* lam = nullptr;     // common-prefix depth of each adjacent pair
   uint8_t* lmin = nullptr;    // minimum tree over lam (levels 1..top)
   void* tmp = nullptr;        // rocprim scratch
@@ -221,8 +266,8 @@ void launch_forces(stellar_ctx* c, bool zero_counter);
 bool sample_sort_fits(int32_t n);
 int32_t sample_sort_buckets(int32_t n);
 int32_t sample_sort_tiles(int32_t n);
-size_t sample_sort_scan_bytes(int32_t n);
-void launch_sample_sort(stellar_ctx* c);
+void launch_sample_sort(stellar_ctx* c, bool keys_counted);
+void launch_keys_count(stellar_ctx* c, int l1, int ibits);
 void launch_update(stellar_ctx* c);
 void launch_output(stellar_ctx* c);
 void launch_zero_counter(stellar_ctx* c);

@@ -14,8 +14,13 @@
 //      bucket) instead.  Splitters only balance the buckets: any sorted set
 //      gives the same result;
 //   2. every key finds its bucket by binary search over the splitters (in
-//      LDS), and per-workgroup bucket counts are scanned;
-//   3. keys and values are scattered to their buckets;
+//      LDS); each workgroup counts its tile's keys per bucket and adds the
+//      counts to the bucket totals (global atomics: a tile of nearly sorted
+//      keys touches a few buckets).  Between steps this is done by the kernel
+//      that computes the keys;
+//   3. every scattering workgroup scans the totals into the buckets' starts,
+//      reserves a range in each bucket it writes to (one atomic per bucket
+//      and workgroup) and scatters its keys and values there;
 //   4. one workgroup per bucket sorts it (a block radix sort, stable, with
 //      the bucket's keys in their index order ahead of the padding) and writes
 //      the final order.  Keys of a bucket agree above the highest bit in
@@ -33,7 +38,6 @@
 // any input, and only reached when the sample misjudges the distribution
 // badly (a first step from an arbitrary body order, say).
 #include <rocprim/block/block_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 
 #include "engine.h"
 
@@ -110,40 +114,108 @@ __global__ __launch_bounds__(kSsThreads) void ss_sample_kernel(const uint64_t* _
   }
 }
 
-// 2. per-workgroup bucket counts, bucket-major: hist[b * nwg + w].
-__global__ __launch_bounds__(kSsTileThreads) void ss_count_kernel(const uint64_t* __restrict__ key,
-                                                              int32_t n, int32_t nb, int32_t tile,
-                                                              const uint64_t* __restrict__ spl,
-                                                              uint32_t* __restrict__ hist) {
-  __shared__ uint64_t sp[kSsMaxBuckets];
-  __shared__ uint32_t cnt[kSsMaxBuckets];
+// Splitters into LDS and zeroed per-bucket counts (nb <= kSsMaxBuckets).
+__device__ __forceinline__ void load_splitters(const uint64_t* __restrict__ spl, int32_t nb,
+                                               uint64_t* sp, uint32_t* cnt) {
   for (int j = threadIdx.x; j < nb; j += blockDim.x) {
     if (j < nb - 1) sp[j] = spl[j];
     cnt[j] = 0;
   }
   __syncthreads();
+}
+
+// The tile's per-bucket counts (LDS) added to the bucket totals.
+__device__ __forceinline__ void add_counts(const uint32_t* cnt, int32_t nb,
+                                           uint32_t* __restrict__ tot) {
+  __syncthreads();
+  for (int j = threadIdx.x; j < nb; j += blockDim.x)
+    if (cnt[j]) atomicAdd(&tot[j], cnt[j]);
+}
+
+// 2. bucket totals of the keys (the first sort after an upload, whose
+// splitters are sampled from keys computed beforehand).
+__global__ __launch_bounds__(kSsTileThreads) void ss_count_kernel(const uint64_t* __restrict__ key,
+                                                              int32_t n, int32_t nb, int32_t tile,
+                                                              const uint64_t* __restrict__ spl,
+                                                              uint32_t* __restrict__ tot) {
+  __shared__ uint64_t sp[kSsMaxBuckets];
+  __shared__ uint32_t cnt[kSsMaxBuckets];
+  load_splitters(spl, nb, sp, cnt);
   const int64_t t0 = (int64_t)blockIdx.x * tile;
   const int64_t t1 = t0 + tile < n ? t0 + tile : n;
   for (int64_t e = t0 + threadIdx.x; e < t1; e += blockDim.x)
     atomicAdd(&cnt[bucket_of(sp, nb, key[e])], 1u);
-  __syncthreads();
-  for (int j = threadIdx.x; j < nb; j += blockDim.x) hist[(int64_t)j * gridDim.x + blockIdx.x] = cnt[j];
+  add_counts(cnt, nb, tot);
 }
 
-// 3. scatter to the buckets (order inside a bucket is arbitrary).
-__global__ __launch_bounds__(kSsTileThreads) void ss_scatter_kernel(
-    const uint64_t* __restrict__ key, const uint32_t* __restrict__ val, int32_t n, int32_t nb,
-    int32_t tile, const uint64_t* __restrict__ spl, const uint32_t* __restrict__ off,
-    uint64_t* __restrict__ key_out, uint32_t* __restrict__ val_out) {
+}  // namespace
+
+// 1+2. keys of a tile (as the build's keys kernel computes them) and their
+// bucket totals, when the splitters are already known (every sort after the
+// first).
+__global__ __launch_bounds__(kSsTileThreads) void ss_keys_count_kernel(
+    const float* __restrict__ x, const float* __restrict__ y, int32_t n,
+    const int32_t* __restrict__ inst_of, const float4* __restrict__ ibbox, int l1, int ibits,
+    int32_t nb, int32_t tile, const uint64_t* __restrict__ spl, uint64_t* __restrict__ key,
+    uint32_t* __restrict__ perm, uint32_t* __restrict__ tot) {
   __shared__ uint64_t sp[kSsMaxBuckets];
-  __shared__ uint32_t pos[kSsMaxBuckets];
-  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
-    if (j < nb - 1) sp[j] = spl[j];
-    pos[j] = off[(int64_t)j * gridDim.x + blockIdx.x];
-  }
-  __syncthreads();
+  __shared__ uint32_t cnt[kSsMaxBuckets];
+  load_splitters(spl, nb, sp, cnt);
   const int64_t t0 = (int64_t)blockIdx.x * tile;
   const int64_t t1 = t0 + tile < n ? t0 + tile : n;
+  for (int64_t e = t0 + threadIdx.x; e < t1; e += blockDim.x) {
+    const int32_t inst = inst_of[e];
+    const uint64_t k = path_key(x[e], y[e], ibbox[inst], inst, l1, ibits);
+    key[e] = k;
+    perm[e] = (uint32_t)e;
+    atomicAdd(&cnt[bucket_of(sp, nb, k)], 1u);
+  }
+  add_counts(cnt, nb, tot);
+}
+
+namespace {
+
+// 3. scatter to the buckets (order inside a bucket is arbitrary).  Every
+// workgroup scans the totals itself (workgroup 0 also writes the bucket
+// starts for the bucket sort), then reserves its range in each bucket its
+// tile writes to.  cur[] (the buckets' fill levels) starts at zero.
+__global__ __launch_bounds__(kSsTileThreads) void ss_scatter_kernel(
+    const uint64_t* __restrict__ key, const uint32_t* __restrict__ val, int32_t n, int32_t nb,
+    int32_t tile, const uint64_t* __restrict__ spl, const uint32_t* __restrict__ tot,
+    uint32_t* __restrict__ cur, uint32_t* __restrict__ bstart, uint64_t* __restrict__ key_out,
+    uint32_t* __restrict__ val_out) {
+  __shared__ uint64_t sp[kSsMaxBuckets];
+  __shared__ uint32_t pos[kSsMaxBuckets];  // tile counts, then write cursors
+  __shared__ uint32_t part[kSsTileThreads / 64];
+  load_splitters(spl, nb, sp, pos);
+  const int64_t t0 = (int64_t)blockIdx.x * tile;
+  const int64_t t1 = t0 + tile < n ? t0 + tile : n;
+  for (int64_t e = t0 + threadIdx.x; e < t1; e += blockDim.x)
+    atomicAdd(&pos[bucket_of(sp, nb, key[e])], 1u);
+  // exclusive scan of the totals: thread t sums buckets [t*per, (t+1)*per)
+  const int per = (nb + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int b0 = (int)threadIdx.x * per;
+  const int b1 = b0 + per < nb ? b0 + per : nb;
+  uint32_t mine = 0;
+  for (int j = b0; j < b1; ++j) mine += tot[j];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t incl = mine;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = __shfl_up(incl, off);
+    if (lane >= off) incl += o;
+  }
+  if (lane == 63) part[wv] = incl;
+  __syncthreads();
+  uint32_t run = incl - mine;
+  for (int w = 0; w < wv; ++w) run += part[w];
+  for (int j = b0; j < b1; ++j) {
+    if (blockIdx.x == 0) bstart[j] = run;
+    const uint32_t c = pos[j];
+    pos[j] = c ? run + atomicAdd(&cur[j], c) : 0u;
+    run += tot[j];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) bstart[nb] = (uint32_t)n;
+  __syncthreads();
   for (int64_t e = t0 + threadIdx.x; e < t1; e += blockDim.x) {
     const uint64_t k = key[e];
     const uint32_t p = atomicAdd(&pos[bucket_of(sp, nb, k)], 1u);
@@ -297,14 +369,21 @@ __device__ bool sort_run_window(const uint64_t* kb, const uint32_t* vb, int64_t 
   return true;
 }
 
+// The bucket's totals and fill level are cleared for the next sort here (the
+// scatter that read them has finished).
 __global__ __launch_bounds__(kSsThreads) void ss_bucket_kernel(
-    uint64_t* __restrict__ kb, uint32_t* __restrict__ vb, int32_t n, int32_t nb, int32_t nwg,
-    const uint32_t* __restrict__ off, const uint64_t* __restrict__ spl, int32_t cap,
-    uint64_t* __restrict__ ko, uint32_t* __restrict__ vo) {
+    uint64_t* __restrict__ kb, uint32_t* __restrict__ vb, int32_t nb,
+    const uint32_t* __restrict__ bstart, uint32_t* __restrict__ tot, uint32_t* __restrict__ cur,
+    const uint64_t* __restrict__ spl, int32_t cap, uint64_t* __restrict__ ko,
+    uint32_t* __restrict__ vo) {
   __shared__ BucketLds L;
   const int b = blockIdx.x;
-  const int64_t s = off[(int64_t)b * nwg];
-  const int64_t e = b + 1 < nb ? (int64_t)off[(int64_t)(b + 1) * nwg] : (int64_t)n;
+  const int64_t s = bstart[b];
+  const int64_t e = bstart[b + 1];
+  if (threadIdx.x == 0) {
+    tot[b] = 0u;
+    cur[b] = 0u;
+  }
   const int64_t m = e - s;
   if (m <= 0) return;
   // every key of the bucket lies in [lo, hi] and so shares their common
@@ -372,35 +451,37 @@ int32_t sample_sort_tiles(int32_t n) {
   return (int32_t)(((int64_t)n + t - 1) / t);
 }
 
-size_t sample_sort_scan_bytes(int32_t n) {
-  if (!sample_sort_fits(n)) return 0;
-  size_t bytes = 0;
-  uint32_t* p = nullptr;
-  const size_t len = (size_t)sample_sort_buckets(n) * (size_t)sample_sort_tiles(n);
-  (void)rocprim::exclusive_scan(nullptr, bytes, p, p, 0u, len, rocprim::plus<uint32_t>());
-  return bytes;
-}
-
-// Sorts (B.key, B.perm) by key; the result is in B.key, B.perm.
-void launch_sample_sort(stellar_ctx* c) {
+// Sorts (B.key, B.perm) by key; the result is in B.key, B.perm.  keys_done:
+// the keys were computed, and their bucket totals added, by
+// ss_keys_count_kernel (launch_build does that when splitters are known).
+void launch_sample_sort(stellar_ctx* c, bool keys_counted) {
   Buffers& B = c->b;
   const int32_t n = c->n;
   const int32_t nb = sample_sort_buckets(n);
   const int32_t nwg = sample_sort_tiles(n);
   const int32_t tile = sample_sort_tile(n);
-  if (!c->ss_have_spl) ss_sample_kernel<<<1, kSsThreads, 0, c->stream>>>(B.key, n, nb, B.ss_spl);
-  ss_count_kernel<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, n, nb, tile, B.ss_spl, B.ss_hist);
-  size_t bytes = B.tmp_bytes;
-  (void)rocprim::exclusive_scan(B.tmp, bytes, B.ss_hist, B.ss_off, 0u,
-                                (size_t)nb * (size_t)nwg, rocprim::plus<uint32_t>(), c->stream);
-  ss_scatter_kernel<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, B.perm, n, nb, tile, B.ss_spl, B.ss_off,
-                                                       B.key_alt, B.perm_alt);
+  uint32_t* tot = B.ss_tot;
+  uint32_t* cur = B.ss_tot + kSsMaxBuckets;
+  if (!keys_counted) {
+    if (!c->ss_have_spl) ss_sample_kernel<<<1, kSsThreads, 0, c->stream>>>(B.key, n, nb, B.ss_spl);
+    ss_count_kernel<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, n, nb, tile, B.ss_spl, tot);
+  }
+  ss_scatter_kernel<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, B.perm, n, nb, tile, B.ss_spl, tot,
+                                                           cur, B.ss_bstart, B.key_alt, B.perm_alt);
   // buckets in (key_alt, perm_alt); the sorted order goes back to (key, perm)
-  ss_bucket_kernel<<<nb, kSsThreads, 0, c->stream>>>(B.key_alt, B.perm_alt, n, nb, nwg, B.ss_off,
+  ss_bucket_kernel<<<nb, kSsThreads, 0, c->stream>>>(B.key_alt, B.perm_alt, nb, B.ss_bstart, tot, cur,
                                                      B.ss_spl, c->sort_cap, B.key, B.perm);
   // the next sort's splitters are read off the sorted keys by the build's
   // tie kernel, which runs next
   c->ss_have_spl = true;
+}
+
+void launch_keys_count(stellar_ctx* c, int l1, int ibits) {
+  Buffers& B = c->b;
+  const int32_t n = c->n;
+  ss_keys_count_kernel<<<sample_sort_tiles(n), kSsTileThreads, 0, c->stream>>>(
+      B.x, B.y, n, B.inst_of, B.ibbox, l1, ibits, sample_sort_buckets(n), sample_sort_tile(n),
+      B.ss_spl, B.key, B.perm, B.ss_tot);
 }
 
 }  // namespace stellar

@@ -15,7 +15,7 @@
 //     last body, so their votes change nothing and need no mask;
 //   * the stack (in LDS, one region per wavefront, written by lane 0) holds
 //     node index and level in one word; with a second block per node
-//     (quadrupoles, bmax) each child's threshold comes with it, otherwise
+//     (quadrupoles, bmax) each cell child's threshold comes with it, otherwise
 //     the threshold dp of a level comes from a per-lane table (v_readlane);
 //   * the eight wavefronts of a workgroup take consecutive groups and each
 //     XCD one contiguous range of workgroups (scalar-cache and L2 reuse);
@@ -243,8 +243,10 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
         const float r = dist2(dx, dy, eps2);
         // one vote for bodies and cells: a body always passes (threshold -1
         // against r >= eps2 > 0), so acceptance is a single scalar test
-        const int32_t ti = kQ ? __float_as_int(qm[kQ ? jj : 0][k].w)
-                              : (code < 0 ? __float_as_int(-1.0f) : __float_as_int(dp[jj]));
+        // (a body's slot in the second block is not written: the code decides)
+        const int32_t ti = code < 0 ? __float_as_int(-1.0f)
+                                    : (kQ ? __float_as_int(qm[kQ ? jj : 0][k].w)
+                                          : __float_as_int(dp[jj]));
         const float t = __int_as_float(__builtin_amdgcn_readfirstlane(ti));
         if (__builtin_expect(__builtin_amdgcn_ballot_w64(!(t <= r)) == 0, 1)) {
           acc.term(QUAD && code > 0, cmv, r, dx, dy, qm[kQ ? jj : 0][k]);

@@ -72,6 +72,39 @@ def test_tree_bit_exact(S, oracle, workload, n):
         assert np.array_equal(gt[k], ot[k]), k
 
 
+@pytest.mark.parametrize("quad", [False, True])
+def test_tree_over_stale_blocks(S, oracle, quad):
+    """Nothing clears the tree blocks between builds (every child slot is
+    written exactly once per build): a tree built where earlier steps and a
+    different body set left their trees is bit-exact against the oracle."""
+    print(f"tree over stale blocks quad={quad}", flush=True)
+    n = 65536
+    flags = (S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING) if quad else 0
+    eps2 = float(np.float32(EPS[2] ** 2))
+    with make_engine(S, ic(2, n), eps2, flags=flags) as e:
+        e.run(3)
+        for w in (1, 0, 2):
+            st = ic(w, n)
+            e.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
+            e.run(2)
+            d = e.download()
+            gbb, gt = gpu_tree(e)
+            cur = {"x": d["x"], "y": d["y"], "mass": d["mass"]}
+            obb, ot = oracle_tree(oracle, cur)
+            assert np.array_equal(gbb, obb)
+            assert tree_equal(gt, ot), (w, first_tree_difference(gt, ot))
+            for k in ("com_x", "com_y", "com_m"):
+                assert np.array_equal(gt[k], ot[k]), (w, k)
+            e.sort()
+            e.forces()
+            f = e.export_forces()
+            kw = {"opening": 1, "quad": True} if quad else {}
+            if quad:
+                oracle.quad_exact(ot, cur["x"], cur["y"], cur["mass"])
+            _, _, it = oracle.walk(ot, cur["x"], cur["y"], cur["mass"], obb, 0.5, eps2, 64, **kw)
+            assert np.array_equal(f["inter"], it), w
+
+
 @pytest.mark.parametrize("group", [64, 32, 16])
 @pytest.mark.parametrize("workload,n", [(0, 4096), (2, 65536), (1, 65536)])
 def test_forces_parity(S, oracle, workload, n, group):
