@@ -86,6 +86,9 @@ def parse():
                     help="only copy 1 GiB device-to-device three times (a plain copy of known "
                          "size that scales rocprofv3's FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--graph", action="store_true", help="replay captured two-step graphs")
+    ap.add_argument("--precision", choices=("fp32", "mixed"), default="fp32",
+                    help="force path: FP32 (default) or mixed (accepted cells' terms in packed "
+                         "FP16 with Kahan-compensated FP32 sums; needs --multipole quad)")
     return ap.parse_args()
 
 
@@ -241,9 +244,19 @@ def ensemble_members(S, ninst, seed=3):
     return [int(c) for c in counts], sts
 
 
+def precision_of(S, args):
+    return S.PRECISION_MIXED if args.precision == "mixed" else S.PRECISION_FP32
+
+
+def dtype_of(args):
+    # arithmetic of the path: FP32 throughout, or packed FP16 cell terms in
+    # FP32 sums
+    return "f16+f32" if args.precision == "mixed" else "f32"
+
+
 def run_ensemble(args, S):
     eps2 = float(np.float32(args.eps * args.eps))
-    prec = S.PRECISION_FP32
+    prec = precision_of(S, args)
     counts, sts = ensemble_members(S, args.instances)
     ntot = int(sum(counts))
     cat = {k: np.concatenate([s[k] for s in sts]) for k in ("mass", "x", "y", "vx", "vy")}
@@ -313,12 +326,12 @@ def run_ensemble(args, S):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": dtype_of(args),
         "data": "synthetic",
         "config": {"workload": f"ensemble of {len(counts)} instances, N_i log-uniform "
                                "[1024, 65536] (x64), disk/Plummer/colliding disks",
                    "n_bodies": ntot, "theta": args.theta, "eps": args.eps, "dt": args.dt,
-                   "precision": "fp32", "graph": True, "tuning": tuning,
+                   "precision": args.precision, "graph": True, "tuning": tuning,
                    "multipoles": multipoles(args), "opening": opening_text(args)},
         "plain_form_body_steps_per_s": plain,
         "plain_form": f"{len(counts)} single-instance contexts run one after another, "
@@ -363,7 +376,7 @@ def main():
     n = args.n
     eps2 = float(np.float32(args.eps * args.eps))
     st = S.generate_ic(S.ic_params(args.workload), n)
-    prec = S.PRECISION_FP32
+    prec = precision_of(S, args)
     p = S.default_params(theta=args.theta, eps2=eps2, timestep=args.dt, dampening=1.0,
                          group=args.group, precision=prec,
                          flags=S.FLAG_STAGE_TIMING | model_flags(S, args)
@@ -453,11 +466,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": dtype_of(args),
         "data": "synthetic",
         "config": {"workload": WORKLOAD_NAMES.get(args.workload, f"workload {args.workload}"),
                    "n_bodies": n, "theta": args.theta, "eps": args.eps, "dt": args.dt,
-                   "group": args.group, "precision": "fp32", "seed": args.workload,
+                   "group": args.group, "precision": args.precision, "seed": args.workload,
                    "graph": bool(args.graph),
                    "opening": opening_text(args), "multipoles": multipoles(args)},
         "rms_rel_force_err": err_rms,

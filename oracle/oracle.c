@@ -486,6 +486,63 @@ static inline void acc_node(float* fx, float* fy, float dx, float dy, float r,
   *fy = fmaf(rs5, fmaf(c, dy, -py), *fy);
 }
 
+/* ---- FP16 arithmetic (mixed precision), emulated in double -------------- */
+/* x rounded to an FP16 value: nearest even (to_zero = 0) or toward zero,
+ * with subnormals (quantum 2^-24); nearest-even overflow gives +-inf, toward
+ * zero saturates at +-65504. */
+static double f16_round(double x, int to_zero) {
+  if (x == 0.0 || x != x) return x;
+  const double ax = fabs(x);
+  int e;
+  (void)frexp(ax, &e); /* ax = m 2^e, m in [0.5, 1): leading bit 2^(e-1) */
+  const int lead = e - 1 < -14 ? -14 : e - 1;
+  const double q = ldexp(1.0, lead - 10);
+  double r = ax / q;
+  r = to_zero ? trunc(r) : nearbyint(r);
+  double v = r * q;
+  if (v > 65504.0) v = to_zero ? 65504.0 : INFINITY;
+  return x < 0 ? -v : v;
+}
+static inline double h16(double x) { return f16_round(x, 0); }
+double or_f16_round(double x, int32_t to_zero) { return f16_round(x, to_zero); }
+static inline double hmul(double a, double b) { return h16(a * b); }
+static inline double hfma(double a, double b, double c) { return h16(fma(a, b, c)); }
+
+/* Per-body state of the mixed far field. */
+typedef struct mixacc {
+  float sx, sy, cx, cy;     /* Kahan sums */
+  float pex, pey, pk, pt;   /* the held cell */
+} mixacc;
+
+static inline void kahan(float* sum, float* comp, float v) {
+  const float yv = v - *comp;
+  const float t = *sum + yv;
+  *comp = (t - *sum) - yv;
+  *sum = t;
+}
+
+/* Two accepted cells (a: held, b: current) for one body. */
+static void mix_pair(mixacc* M, const float* qa, float ex, float ey, float k, float t,
+                     const float* qb) {
+  const double EXa = f16_round(M->pex, 1), EXb = f16_round(ex, 1);
+  const double EYa = f16_round(M->pey, 1), EYb = f16_round(ey, 1);
+  const double Ta = f16_round(M->pt, 1), Tb = f16_round(t, 1);
+  double BX[2], BY[2];
+  const double EX[2] = {EXa, EXb}, EY[2] = {EYa, EYb}, T[2] = {Ta, Tb};
+  const float* Q[2] = {qa, qb};
+  for (int h = 0; h < 2; ++h) {
+    const double qxx = Q[h][0], qxy = Q[h][1], qyy = Q[h][2];
+    const double qex = hfma(qxx, EX[h], hmul(qxy, EY[h]));
+    const double qey = hfma(qxy, EX[h], hmul(qyy, EY[h]));
+    const double s = hfma(EX[h], qex, hmul(EY[h], qey));
+    const double cc = hmul(s, 2.5);
+    BX[h] = hfma(T[h], hfma(cc, EX[h], -qex), EX[h]);
+    BY[h] = hfma(T[h], hfma(cc, EY[h], -qey), EY[h]);
+  }
+  kahan(&M->sx, &M->cx, fmaf(M->pk, (float)BX[0], k * (float)BX[1]));
+  kahan(&M->sy, &M->cy, fmaf(M->pk, (float)BY[0], k * (float)BY[1]));
+}
+
 int or_walk(int32_t n, const float* x, const float* y, const float* mass,
             const float* bbox, const int32_t* sorted, const int32_t* root,
             const int32_t* child, int64_t ncells, const float* ccx,
@@ -498,6 +555,14 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
   const float depth0 = depth0_of(bbox, p->theta, p->opening);
   const float* cq = p->cq;
   const int32_t ngroups = (n + G - 1) / G;
+  const int mixed = p->mixed && p->cqh && p->clevel;
+  float s2 = 0.f;
+  if (mixed) {
+    const float w = bbox[1] - bbox[0], h = bbox[3] - bbox[2];
+    const float sz = w > h ? w : h;
+    s2 = sz * sz;
+  }
+  static const float qzero[3] = {0.f, 0.f, 0.f};
   int err = OR_OK;
 #pragma omp parallel num_threads(nthreads(p->threads))
   {
@@ -506,6 +571,9 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
     float* fx = py + G;
     float* fy = fx + G;
     float* rr = fy + G;
+    mixacc* mx = (mixacc*)malloc(sizeof(mixacc) * (size_t)G);
+    const float* pq = qzero; /* the held cell's Qh (the whole group holds one) */
+    int pend = 0;
     int32_t stk[OR_STACK];
     float dep[OR_STACK];
 #pragma omp for schedule(dynamic, 8)
@@ -519,7 +587,10 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
         py[j] = y[b];
         fx[j] = 0.f;
         fy[j] = 0.f;
+        memset(&mx[j], 0, sizeof(mixacc));
       }
+      pend = 0;
+      pq = qzero;
       int sp = 0;
       for (int k = 0; k < 4; ++k) {
         const int32_t ch = root[k];
@@ -562,7 +633,31 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
             rr[j] = dist2(dx, dy, eps2);
             if (!(th <= rr[j])) all = 0;
           }
-          if (ch < n || all) {
+          if (mixed && ch >= n && all) {
+            const float* qc = p->cqh + 3 * (int64_t)(ch - n);
+            const float ws2 = ldexpf(s2, -2 * p->clevel[ch - n]);
+            for (int32_t j = 0; j < gs; ++j) {
+              const float dx = cxv - px[j], dy = cyv - py[j];
+              const float rs = rsqrt_ref(rr[j]);
+              const float u = rs * rs;
+              const float ex = dx * rs, ey = dy * rs, k = cmv * u, t = u * ws2;
+              if (pend) {
+                mix_pair(&mx[j], pq, ex, ey, k, t, qc);
+              } else {
+                mx[j].pex = ex;
+                mx[j].pey = ey;
+                mx[j].pk = k;
+                mx[j].pt = t;
+              }
+            }
+            if (pend) {
+              pend = 0;
+            } else {
+              pend = 1;
+              pq = qc;
+            }
+            ++cnt;
+          } else if (ch < n || all) {
             for (int32_t j = 0; j < gs; ++j) {
               const float dx = cxv - px[j], dy = cyv - py[j];
               acc_node(&fx[j], &fy[j], dx, dy, rr[j], cmv, q);
@@ -580,6 +675,13 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
           }
         }
       }
+      if (mixed) {
+        for (int32_t j = 0; j < gs; ++j) {
+          if (pend) mix_pair(&mx[j], pq, 0.f, 0.f, 0.f, 0.f, qzero);
+          fx[j] = fx[j] + mx[j].sx;
+          fy[j] = fy[j] + mx[j].sy;
+        }
+      }
       for (int32_t j = 0; j < gs; ++j) {
         const int32_t b = sorted[s0 + j];
         ax[b] = fx[j];
@@ -588,6 +690,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
       }
     }
     free(px);
+    free(mx);
   }
   return err;
 }
@@ -698,10 +801,11 @@ int or_walk_recursive(int32_t n, const float* x, const float* y,
   return OR_OK;
 }
 
-/* Quadrupole moments, bottom-up in FP64 (see oracle.h). */
-void or_quad_exact(int32_t n, int64_t ncells, const int32_t* child,
-                   const float* x, const float* y, const float* mass,
-                   float* cq) {
+/* FP64 moments per cell, bottom-up in slot order: S[6c..6c+5] = sum m,
+ * sum m x, sum m y, Qxx, Qxy, Qyy (traceless, about the FP64 centre of
+ * mass).  The caller frees S. */
+static double* quad_moments(int32_t n, int64_t ncells, const int32_t* child,
+                            const float* x, const float* y, const float* mass) {
   double* S = (double*)malloc(sizeof(double) * 6 * (size_t)(ncells + 1));
   for (int64_t c = ncells - 1; c >= 0; --c) {
     double cm[4], cx[4], cy[4];
@@ -758,9 +862,38 @@ void or_quad_exact(int32_t n, int64_t ncells, const int32_t* child,
     o[3] = qxx;
     o[4] = qxy;
     o[5] = qyy;
-    cq[3 * c + 0] = (float)qxx;
-    cq[3 * c + 1] = (float)qxy;
-    cq[3 * c + 2] = (float)qyy;
+  }
+  return S;
+}
+
+/* Quadrupole moments, bottom-up in FP64 (see oracle.h). */
+void or_quad_exact(int32_t n, int64_t ncells, const int32_t* child,
+                   const float* x, const float* y, const float* mass,
+                   float* cq) {
+  double* S = quad_moments(n, ncells, child, x, y, mass);
+  for (int64_t c = 0; c < ncells; ++c) {
+    cq[3 * c + 0] = (float)S[6 * c + 3];
+    cq[3 * c + 1] = (float)S[6 * c + 4];
+    cq[3 * c + 2] = (float)S[6 * c + 5];
+  }
+  free(S);
+}
+
+void or_quad_mixed(int32_t n, int64_t ncells, const int32_t* child,
+                   const int32_t* clevel, const float* x, const float* y,
+                   const float* mass, const float* bbox, float* qh) {
+  double* S = quad_moments(n, ncells, child, x, y, mass);
+  const float w = bbox[1] - bbox[0], h = bbox[3] - bbox[2];
+  const float sz = w > h ? w : h;
+  const float s2 = sz * sz;
+  for (int64_t c = 0; c < ncells; ++c) {
+    const float ws2 = ldexpf(s2, -2 * clevel[c]);
+    const double den = S[6 * c] * (double)ws2;
+    for (int k = 0; k < 3; ++k) {
+      float v = den != 0.0 ? (float)(S[6 * c + 3 + k] / den) : 0.f;
+      v = v > 65504.f ? 65504.f : (v < -65504.f ? -65504.f : v);
+      qh[3 * c + k] = (float)h16((double)v);
+    }
   }
   free(S);
 }
@@ -978,7 +1111,7 @@ int or_step(int32_t n, float* x, float* y, float* vx, float* vy,
   t[4] = now_s();
   {
     /* the walk reads the pool layout: cells at n.. with CoM in cx/cy/cm */
-    or_walk_params wp = {p->theta, p->eps2, p->group, p->threads, 0, 0, NULL, NULL};
+    or_walk_params wp = {p->theta, p->eps2, p->group, p->threads, 0, 0, NULL, NULL, NULL, NULL};
     rc = or_walk(n, x, y, mass, bbox, sorted, child, child + 4 * (size_t)n,
                  ncells, cx + n, cy + n, cm + n, &wp, ax, ay, NULL);
     if (rc) goto out;

@@ -100,10 +100,13 @@ typedef struct or_walk_params {
   int32_t threads;   /* OpenMP threads, <=0 = all */
   int32_t opening;   /* 0: w^2/theta <= r (reference);
                         1: w^2/theta^2 + eps2 <= r (textbook, unsoftened d) */
-  int32_t pad;
+  int32_t mixed;     /* 1: accepted cells' terms in FP16 (see or_quad_mixed),
+                        needs cqh and clevel */
   const float* cq;   /* NULL: monopole; else 3 quadrupole terms per cell */
   const float* cthr; /* NULL: per-level thresholds; else one per cell
                         (or_bmax_exact), compared with r like dp */
+  const float* cqh;  /* mixed: Qh per cell (3 FP16 values, or_quad_mixed) */
+  const int32_t* clevel; /* mixed: level of each cell */
 } or_walk_params;
 
 int or_walk(int32_t n, const float* x, const float* y, const float* mass,
@@ -111,6 +114,23 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
             const int32_t* child, int64_t ncells, const float* ccx,
             const float* ccy, const float* ccm, const or_walk_params* p,
             float* ax, float* ay, int32_t* inter);
+
+/* Mixed precision (not in the reference).  An accepted cell's term
+ * m d/r^3 - Q.d/r^5 + (5/2)(d.Q.d) d/r^7 is k B with k = m rs^2 (FP32) and
+ *   B = e + t (-Qh.e + (5/2)(e.Qh.e) e),  e = d rs, t = rs^2 ws^2,
+ * rs = 1/sqrt(|d|^2 + eps2), ws^2 = ldexpf(S*S, -2 L) for a cell at level L
+ * (S the larger extent of the box, FP32), Qh = Q / (m ws^2) rounded to FP32
+ * and then to FP16 (nearest even, clamped to +-65504).  B is evaluated in
+ * FP16 arithmetic (each operation rounded to nearest even; e, t converted
+ * toward zero), two accepted cells at a time in visiting order (the last
+ * with zeros), and fmaf(k_a, B_a, k_b * B_b) enters an FP32 Kahan sum per
+ * body; bodies stay FP32 monopoles.  or_quad_mixed writes Qh (3 per cell, as
+ * floats holding FP16 values). */
+/* x rounded to FP16 (to_zero = 0: nearest even; 1: toward zero). */
+double or_f16_round(double x, int32_t to_zero);
+void or_quad_mixed(int32_t n, int64_t ncells, const int32_t* child,
+                   const int32_t* clevel, const float* x, const float* y,
+                   const float* mass, const float* bbox, float* qh);
 
 /* Opening thresholds from the bottom-up bound bmax (the largest distance
  * from a cell's FP64 centre of mass to any of its bodies, bounded through

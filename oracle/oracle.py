@@ -34,7 +34,8 @@ _I64 = C.POINTER(C.c_int64)
 
 class WalkParams(C.Structure):
     _fields_ = [("theta", C.c_float), ("eps2", C.c_float), ("group", C.c_int32), ("threads", C.c_int32),
-                ("opening", C.c_int32), ("pad", C.c_int32), ("cq", _F), ("cthr", _F)]
+                ("opening", C.c_int32), ("mixed", C.c_int32), ("cq", _F), ("cthr", _F), ("cqh", _F),
+                ("clevel", _I)]
 
 
 class StepParams(C.Structure):
@@ -53,6 +54,8 @@ _SIG = {
     "or_com_exact": (None, [C.c_int32, C.c_int64, _I, _F, _F, _F, _F, _F, _F]),
     "or_bmax_exact": (None, [C.c_int32, C.c_int64, _I, _F, _F, _F, C.c_float, C.c_float, _F]),
     "or_quad_exact": (None, [C.c_int32, C.c_int64, _I, _F, _F, _F, _F]),
+    "or_quad_mixed": (None, [C.c_int32, C.c_int64, _I, _I, _F, _F, _F, _F, _F]),
+    "or_f16_round": (C.c_double, [C.c_double, C.c_int32]),
     "or_walk": (C.c_int, [C.c_int32, _F, _F, _F, _F, _I, _I, _I, C.c_int64, _F, _F, _F,
                           C.POINTER(WalkParams), _F, _F, _I]),
     "or_walk_recursive": (C.c_int, [C.c_int32, _F, _F, _F, _F, _I, _I, _I, C.c_int64, _F, _F, _F,
@@ -216,15 +219,32 @@ def quad_exact(tree: dict, x, y, mass) -> None:
     tree["quad"] = q[: 3 * c]
 
 
-def _wp(tree, theta, eps2, group, threads, opening, quad, bmax=False):
+def _wp(tree, theta, eps2, group, threads, opening, quad, bmax=False, mixed=False):
     cq = None
     if quad:
         cq = tree["quad"] if tree["quad"].size else np.zeros(3, np.float32)
     th = None
     if bmax:
         th = tree["thr"] if tree["thr"].size else np.zeros(1, np.float32)
-    p = WalkParams(theta, eps2, group, threads, opening, 0, _f(cq), _f(th))
-    return p, (cq, th)
+    qh = lv = None
+    if mixed:
+        qh = tree["qh"] if tree["qh"].size else np.zeros(3, np.float32)
+        lv = np.ascontiguousarray(tree["level"], dtype=np.int32)
+        lv = lv if lv.size else np.zeros(1, np.int32)
+    p = WalkParams(theta, eps2, group, threads, opening, 1 if mixed else 0, _f(cq), _f(th), _f(qh),
+                   _i(lv))
+    return p, (cq, th, qh, lv)
+
+
+def quad_mixed(tree: dict, x, y, mass, bb) -> None:
+    """Qh = Q / (m ws^2) per cell, rounded to FP16, into tree["qh"]."""
+    x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
+    c = tree["ncells"]
+    q = np.zeros(3 * max(c, 1), np.float32)
+    lv = np.ascontiguousarray(tree["level"], dtype=np.int32)
+    lib().or_quad_mixed(x.size, c, _i(tree["child"]), _i(lv if lv.size else np.zeros(1, np.int32)),
+                        _f(x), _f(y), _f(mass), _f(bb), _f(q))
+    tree["qh"] = q[: 3 * c]
 
 
 def bmax_exact(tree: dict, x, y, mass, theta: float, eps2: float) -> None:
@@ -238,12 +258,13 @@ def bmax_exact(tree: dict, x, y, mass, theta: float, eps2: float) -> None:
 
 
 def walk(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0,
-         opening: int = 0, quad: bool = False, bmax: bool = False):
+         opening: int = 0, quad: bool = False, bmax: bool = False, mixed: bool = False):
     """Group-vote walk (kernels.cu:340-435) over a canonical tree; by body id.
-    bmax: per-cell thresholds tree["thr"] (bmax_exact) instead of levels."""
+    bmax: per-cell thresholds tree["thr"] (bmax_exact) instead of levels.
+    mixed: accepted cells' terms in FP16 from tree["qh"] (quad_mixed)."""
     x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
     n = x.size
-    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad, bmax)
+    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad, bmax, mixed)
     ax = np.zeros(n, np.float32)
     ay = np.zeros(n, np.float32)
     it = np.zeros(n, np.int32)

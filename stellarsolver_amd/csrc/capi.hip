@@ -57,7 +57,12 @@ bool params_ok(const stellar_params* p) {
   if (!p) return false;
   if (!(p->theta > 0.f) || !(p->eps2 > 0.f)) return false;
   if (p->group != 64 && p->group != 32 && p->group != 16) return false;
-  if (p->precision != STELLAR_PRECISION_FP32) return false;  // MIXED: not implemented
+  if (p->precision != STELLAR_PRECISION_FP32 && p->precision != STELLAR_PRECISION_MIXED)
+    return false;
+  // the mixed far field is the quadrupole term of group-64 walks
+  if (p->precision == STELLAR_PRECISION_MIXED &&
+      (p->group != 64 || !(p->flags & STELLAR_FLAG_QUADRUPOLE)))
+    return false;
   return true;
 }
 
@@ -1034,7 +1039,8 @@ int stellar_autotune(stellar_ctx* c, int32_t group_mask, int32_t reps, stellar_t
   if (rc) return rc;
   const bool quad = (c->p.flags & STELLAR_FLAG_QUADRUPOLE) != 0;
   std::vector<stellar_tuning> cand;
-  const int mask = group_mask == 0 ? 1 : group_mask;
+  // the mixed far field exists for group-64 walks only
+  const int mask = group_mask == 0 || c->p.precision == STELLAR_PRECISION_MIXED ? 1 : group_mask;
   if (mask & 1)
     for (int kp : {1, 2, 4})
       for (int w : {1, 2, 4, 8, 16})

@@ -52,7 +52,19 @@ struct ComArgs {
   int bmax;       // per-cell opening thresholds bmax^2/t2 + eps2 in qblk .w
   double t2;      // theta^2
   double eps2;
+  // mixed precision: qblk .x/.y hold Qh = Q / (m ws^2) as FP16 bits (Qxx | Qxy
+  // << 16, Qyy), ws = S / 2^L with S the larger extent of the instance's box
+  int mixed;
+  const int32_t* cstart;
+  const int32_t* inst_of;
+  const float4* ibbox;
 };
+
+// FP16 bits of v (round to nearest even), clamped to the finite range.
+__device__ __forceinline__ uint32_t half_bits(float v) {
+  v = v > 65504.f ? 65504.f : (v < -65504.f ? -65504.f : v);
+  return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v);
+}
 
 template <bool EXT>
 struct ChunkLds {
@@ -180,7 +192,21 @@ __device__ void publish_cell(const ComArgs& A, const ChunkLds<EXT>& L, int i, in
                              int32_t p, int s) {
   const double* o = L.mom[i];
   A.blk[4 * (int64_t)p + s] = ChildRec{(float)o[7], (float)o[8], (float)o[0], b};
-  if (A.bmax) {
+  if (A.mixed) {
+    // Qh of a cell at level lev: Q / (m ws^2), ws^2 = S^2 / 4^lev in FP32
+    const float4 bb = A.ibbox[A.inst_of[A.cstart[b]]];
+    const float w = bb.y - bb.x, h = bb.w - bb.z;
+    const float sz = w > h ? w : h;
+    const float ws2 = __builtin_ldexpf(sz * sz, -2 * (int)L.lev[i]);
+    const double den = o[0] * (double)ws2;
+    const float qxx = den != 0.0 ? (float)(o[3] / den) : 0.f;
+    const float qxy = den != 0.0 ? (float)(o[4] / den) : 0.f;
+    const float qyy = den != 0.0 ? (float)(o[5] / den) : 0.f;
+    uint32_t* q = reinterpret_cast<uint32_t*>(A.qblk + 4 * (int64_t)p + s);
+    q[0] = half_bits(qxx) | (half_bits(qxy) << 16);
+    q[1] = half_bits(qyy);
+    if (A.bmax) reinterpret_cast<float*>(q)[3] = (float)((o[6] * o[6]) / A.t2 + A.eps2);
+  } else if (A.bmax) {
     const float thr = (float)((o[6] * o[6]) / A.t2 + A.eps2);
     A.qblk[4 * (int64_t)p + s] = make_float4((float)o[3], (float)o[4], (float)o[5], thr);
   } else if (A.quad) {
@@ -444,7 +470,8 @@ void launch_com(stellar_ctx* c) {
   ComArgs A{B.blk,  B.qblk, B.cparent, B.cslot, B.clevel, B.csum, B.dev,
             (c->p.flags & STELLAR_FLAG_QUADRUPOLE) ? 1 : 0, c->ninst,
             (c->p.flags & STELLAR_FLAG_BMAX_OPENING) ? 1 : 0,
-            (double)c->p.theta * (double)c->p.theta, (double)c->p.eps2};
+            (double)c->p.theta * (double)c->p.theta, (double)c->p.eps2,
+            c->p.precision == STELLAR_PRECISION_MIXED ? 1 : 0, B.cstart, B.inst_of, B.ibbox};
   // pass 1 over all cells; each later list holds at most kSlots cells per
   // chunk of the pass before, so grids are sized from that bound and chunks
   // beyond the actual length exit at once

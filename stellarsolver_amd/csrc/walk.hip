@@ -126,6 +126,77 @@ struct Acc {
   }
 };
 
+// ------------------------------------------------ mixed-precision far field --
+// STELLAR_PRECISION_MIXED: an accepted cell's term m d/r^3 - Q.d/r^5 +
+// (5/2)(d.Q.d) d/r^7 is written as k B with k = m rs^2 (FP32) and the
+// dimensionless bracket
+//   B = e + t (-Qh.e + (5/2)(e.Qh.e) e),  e = d rs, t = rs^2 ws^2,
+// where rs = 1/sqrt(|d|^2 + eps2), ws = S / 2^L is the cell's size at level L
+// (S the larger extent of the instance's box) and Qh = Q / (m ws^2) is stored
+// by the centre-of-mass pass as three FP16 values.  |e| <= 1 and t, |Qh| are
+// O(1), so B is evaluated in packed FP16 (v_pk_fma_f16), two accepted cells
+// per instruction: a cell is held until the next one comes, and the last is
+// paired with zeros.  The conversions to FP16 round toward zero (one
+// v_cvt_pkrtz_f16_f32 per pair).  k B enters an FP32 sum with Kahan
+// compensation; bodies (the near field) stay FP32 monopoles as in the plain
+// walk, and the acceptance vote is the plain walk's, in FP32.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+struct MixedAcc {
+  float sx = 0.f, sy = 0.f, cx = 0.f, cy = 0.f;  // compensated far-field sums
+  bool pend = false;
+  float pex = 0.f, pey = 0.f, pk = 0.f, pt = 0.f;  // the held cell (per lane)
+  uint32_t pq0 = 0u, pq1 = 0u;                     // its Qh halves (uniform)
+
+  __device__ __forceinline__ static void kahan(float& sum, float& comp, float v) {
+    const float y = v - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+  }
+  // q0 = Qxx | Qxy << 16, q1 = Qyy (FP16 bits)
+  __device__ __forceinline__ void pair(float ex, float ey, float k, float t, uint32_t q0,
+                                       uint32_t q1) {
+    const half2_t EX = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(pex, ex));
+    const half2_t EY = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(pey, ey));
+    const half2_t T = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(pt, t));
+    const half2_t QXX = __builtin_bit_cast(half2_t, (pq0 & 0xffffu) | (q0 << 16));
+    const half2_t QXY = __builtin_bit_cast(half2_t, (pq0 >> 16) | (q0 & 0xffff0000u));
+    const half2_t QYY = __builtin_bit_cast(half2_t, (pq1 & 0xffffu) | (q1 << 16));
+    const half2_t QEX = __builtin_elementwise_fma(QXX, EX, QXY * EY);
+    const half2_t QEY = __builtin_elementwise_fma(QXY, EX, QYY * EY);
+    const half2_t S = __builtin_elementwise_fma(EX, QEX, EY * QEY);
+    const half2_t C = S * (half2_t)(_Float16)2.5f;
+    const half2_t BX = __builtin_elementwise_fma(T, __builtin_elementwise_fma(C, EX, -QEX), EX);
+    const half2_t BY = __builtin_elementwise_fma(T, __builtin_elementwise_fma(C, EY, -QEY), EY);
+    kahan(sx, cx, __builtin_fmaf(pk, (float)BX.x, k * (float)BX.y));
+    kahan(sy, cy, __builtin_fmaf(pk, (float)BY.x, k * (float)BY.y));
+  }
+  // One accepted cell (r = |d|^2 + eps2, ws2 = ws^2 of its level, q0/q1 its Qh).
+  __device__ __forceinline__ void cell(float cm, float r, float dx, float dy, float ws2,
+                                       uint32_t q0, uint32_t q1) {
+    const float rs = __builtin_amdgcn_rsqf(r);
+    const float u = rs * rs;
+    const float ex = dx * rs, ey = dy * rs, k = cm * u, t = u * ws2;
+    if (pend) {
+      pair(ex, ey, k, t, q0, q1);
+      pend = false;
+    } else {
+      pex = ex;
+      pey = ey;
+      pk = k;
+      pt = t;
+      pq0 = q0;
+      pq1 = q1;
+      pend = true;
+    }
+  }
+  __device__ __forceinline__ void flush() {
+    if (pend) pair(0.f, 0.f, 0.f, 0.f, 0u, 0u);
+    pend = false;
+  }
+};
+
 // ---------------------------------------------------------------- G = 64 --
 // Stack word: block index in the low 26 bits, (level - 1) in the top 6.
 constexpr int kIdxBits = 26;
@@ -143,11 +214,12 @@ constexpr int kWalkSgprs = 80;
 // interaction counts.  The tree arrays are separate const __restrict__
 // arguments: that is what lets the compiler read them through the scalar
 // data cache.
-template <bool QUAD, bool THR, int KPOP, int STACK = kStack64>
+template <bool QUAD, bool THR, int KPOP, int STACK = kStack64, bool MIXED = false>
 __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
                                              const float4* __restrict__ qblk,
                                              const WalkArgs& A, int64_t g,
                                              uint32_t* st) {
+  static_assert(!MIXED || (QUAD && KPOP == 1), "the mixed far field pairs cells in visiting order");
   const float eps2 = A.eps2;
   const int lane = threadIdx.x & 63;
   const int32_t s0 = __builtin_amdgcn_readfirstlane(A.gtab[g]);
@@ -175,6 +247,16 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
 
   int sp = 0;
   Acc acc;
+  MixedAcc mix;
+  // mixed: the square of the larger box extent (a cell at level L has size
+  // S / 2^L)
+  float s2 = 0.f;
+  if (MIXED) {
+    const float4 bb = A.ibbox[inst];
+    const float w = bb.y - bb.x, h = bb.w - bb.z;
+    const float sz = w > h ? w : h;
+    s2 = sz * sz;
+  }
 
   const ChildRec* root = blk + 4 * (int64_t)inst;
 #pragma unroll
@@ -233,6 +315,8 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
     for (int jj = KPOP - 1; jj >= 0; --jj) {
       if (jj >= take) continue;
       const uint32_t child_lvl = (lvl[jj] + 1u) << kIdxBits;
+      // mixed: ws^2 of the children (level lvl + 2)
+      const float ws2 = MIXED ? __builtin_ldexpf(s2, -2 * (int)(lvl[jj] + 2u)) : 0.f;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int32_t code = q[jj][k].w;
@@ -249,7 +333,14 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
                                           : __float_as_int(dp[jj]));
         const float t = __int_as_float(__builtin_amdgcn_readfirstlane(ti));
         if (__builtin_expect(__builtin_amdgcn_ballot_w64(!(t <= r)) == 0, 1)) {
-          acc.term(QUAD && code > 0, cmv, r, dx, dy, qm[kQ ? jj : 0][k]);
+          if (MIXED && code > 0) {
+            ++acc.cnt;
+            ++acc.qcnt;
+            const float4 q = qm[kQ ? jj : 0][k];
+            mix.cell(cmv, r, dx, dy, ws2, __float_as_uint(q.x), __float_as_uint(q.y));
+          } else {
+            acc.term(QUAD && code > 0, cmv, r, dx, dy, qm[kQ ? jj : 0][k]);
+          }
         } else {
           if (lane == 0 && sp < STACK) st[sp] = (uint32_t)code | child_lvl;
           ++sp;
@@ -260,6 +351,11 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
       if (lane == 0) atomicOr(&A.dev->err, kErrStack);
       break;
     }
+  }
+  if (MIXED) {
+    mix.flush();
+    acc.fx = acc.fx + mix.sx;
+    acc.fy = acc.fy + mix.sy;
   }
   if (valid) {
     A.ax[s] = acc.fx;
@@ -279,7 +375,7 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
 // at about the same time, which the CU's scalar cache serves once.
 // Workgroups are mapped so that each XCD walks one contiguous range of
 // groups (xcd_block), which its L2 serves.
-template <bool QUAD, bool THR, int KPOP, int WAVES>
+template <bool QUAD, bool THR, int KPOP, int WAVES, bool MIXED = false>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_num_sgpr(kWalkSgprs))) void walk64_kernel(
     const ChildRec* __restrict__ blk, const float4* __restrict__ qblk,
     WalkArgs A) {
@@ -288,7 +384,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_num_sgpr(kWalkSgp
   const int64_t g = (int64_t)xcd_block(blockIdx.x, gridDim.x) * WAVES + wave;
   if (g >= A.ngroups) return;
   if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
-  walk64_group<QUAD, THR, KPOP>(blk, qblk, A, g, stk[wave]);
+  walk64_group<QUAD, THR, KPOP, kStack64, MIXED>(blk, qblk, A, g, stk[wave]);
 }
 
 // Persistent form: exactly the resident workgroups, each taking work from
@@ -309,7 +405,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_num_sgpr(kWalkSgp
 // workgroup's LDS stays at 4 KiB per wavefront (four 512-thread workgroups fit
 // a CU).
 constexpr int kStackQ = kStack64 - 1;
-template <bool QUAD, bool THR, int KPOP, int WAVES, bool CLAIM_WAVE>
+template <bool QUAD, bool THR, int KPOP, int WAVES, bool CLAIM_WAVE, bool MIXED = false>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_num_sgpr(kWalkSgprs))) void walk64_queue_kernel(
     const ChildRec* __restrict__ blk, const float4* __restrict__ qblk, WalkArgs A) {
   __shared__ uint32_t stk[WAVES * kStack64];
@@ -339,7 +435,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_num_sgpr(kWalkSgp
       continue;
     }
     const int64_t g = CLAIM_WAVE ? u : u * WAVES + wave;
-    if (g < A.ngroups) walk64_group<QUAD, THR, KPOP, kStackQ>(blk, qblk, A, g, stk + wave * kStack64);
+    if (g < A.ngroups)
+      walk64_group<QUAD, THR, KPOP, kStackQ, MIXED>(blk, qblk, A, g, stk + wave * kStack64);
   }
 }
 
@@ -478,17 +575,16 @@ int resident_blocks(const void* kernel, int threads) {
   return nb;
 }
 
-template <bool QUAD, bool THR, int KPOP, int WAVES, bool CLAIM_WAVE>
+template <bool QUAD, bool THR, int KPOP, int WAVES, bool CLAIM_WAVE, bool MIXED>
 void launch64_queue(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
-  auto k = walk64_queue_kernel<QUAD, THR, KPOP, WAVES, CLAIM_WAVE>;
+  auto k = walk64_queue_kernel<QUAD, THR, KPOP, WAVES, CLAIM_WAVE, MIXED>;
   const int64_t units = CLAIM_WAVE ? ngroups : (ngroups + WAVES - 1) / WAVES;
   int64_t grid = resident_blocks(reinterpret_cast<const void*>(k), 64 * WAVES);
-  if (getenv("PROBE_GRID")) grid = atoi(getenv("PROBE_GRID")) / WAVES;
   if (grid > units) grid = units;
   k<<<(unsigned)grid, 64 * WAVES, 0, c->stream>>>(A.blk, A.qblk, A);
 }
 
-template <bool QUAD, bool THR, int KPOP>
+template <bool QUAD, bool THR, int KPOP, bool MIXED = false>
 void launch64(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
   // default shape (measured on MI355X, N = 2^20): 8 wavefronts per workgroup
   const int w = c->walk_waves > 0 ? c->walk_waves : 8;
@@ -496,33 +592,33 @@ void launch64(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
     const bool per_wave = c->walk_queue == 2;
     if (w == 8) {
       if (per_wave)
-        launch64_queue<QUAD, THR, KPOP, 8, true>(c, ngroups, A);
+        launch64_queue<QUAD, THR, KPOP, 8, true, MIXED>(c, ngroups, A);
       else
-        launch64_queue<QUAD, THR, KPOP, 8, false>(c, ngroups, A);
+        launch64_queue<QUAD, THR, KPOP, 8, false, MIXED>(c, ngroups, A);
     } else {
       if (per_wave)
-        launch64_queue<QUAD, THR, KPOP, 4, true>(c, ngroups, A);
+        launch64_queue<QUAD, THR, KPOP, 4, true, MIXED>(c, ngroups, A);
       else
-        launch64_queue<QUAD, THR, KPOP, 4, false>(c, ngroups, A);
+        launch64_queue<QUAD, THR, KPOP, 4, false, MIXED>(c, ngroups, A);
     }
     return;
   }
   const unsigned grid = (unsigned)((ngroups + w - 1) / w);
   switch (w) {
     case 1:
-      walk64_kernel<QUAD, THR, KPOP, 1><<<grid, 64, 0, c->stream>>>(A.blk, A.qblk, A);
+      walk64_kernel<QUAD, THR, KPOP, 1, MIXED><<<grid, 64, 0, c->stream>>>(A.blk, A.qblk, A);
       break;
     case 2:
-      walk64_kernel<QUAD, THR, KPOP, 2><<<grid, 128, 0, c->stream>>>(A.blk, A.qblk, A);
+      walk64_kernel<QUAD, THR, KPOP, 2, MIXED><<<grid, 128, 0, c->stream>>>(A.blk, A.qblk, A);
       break;
     case 8:
-      walk64_kernel<QUAD, THR, KPOP, 8><<<grid, 512, 0, c->stream>>>(A.blk, A.qblk, A);
+      walk64_kernel<QUAD, THR, KPOP, 8, MIXED><<<grid, 512, 0, c->stream>>>(A.blk, A.qblk, A);
       break;
     case 16:
-      walk64_kernel<QUAD, THR, KPOP, 16><<<grid, 1024, 0, c->stream>>>(A.blk, A.qblk, A);
+      walk64_kernel<QUAD, THR, KPOP, 16, MIXED><<<grid, 1024, 0, c->stream>>>(A.blk, A.qblk, A);
       break;
     default:
-      walk64_kernel<QUAD, THR, KPOP, 4><<<grid, 256, 0, c->stream>>>(A.blk, A.qblk, A);
+      walk64_kernel<QUAD, THR, KPOP, 4, MIXED><<<grid, 256, 0, c->stream>>>(A.blk, A.qblk, A);
       break;
   }
 }
@@ -558,6 +654,15 @@ template <int G>
 void launch_g(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
   const bool quad = (c->p.flags & STELLAR_FLAG_QUADRUPOLE) != 0;
   const bool thr = (c->p.flags & STELLAR_FLAG_BMAX_OPENING) != 0;
+  if (G == 64 && c->p.precision == STELLAR_PRECISION_MIXED) {
+    // quadrupoles are required (params_ok); one node per pop keeps the
+    // pairing of accepted cells in visiting order
+    if (thr)
+      launch64<true, true, 1, true>(c, ngroups, A);
+    else
+      launch64<true, false, 1, true>(c, ngroups, A);
+    return;
+  }
   if (quad && thr)
     launch_shape<G, true, true>(c, ngroups, A);
   else if (quad)

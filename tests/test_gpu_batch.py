@@ -87,11 +87,14 @@ def test_batch_trees_bit_exact(S, oracle):
                 assert np.array_equal(t[k], kt[k]), k
 
 
-@pytest.mark.parametrize("group", [64, 32, 16])
-def test_batch_steps_match_single_contexts(S, group):
+@pytest.mark.parametrize("group,mixed", [(64, False), (32, False), (16, False), (64, True)])
+def test_batch_steps_match_single_contexts(S, group, mixed):
     sts = members(S, COUNTS, base_seed=11)
     steps = 3
-    with batch_engine(S, sts, group=group) as e:
+    kw = {"group": group}
+    if mixed:  # the mixed-precision far field (quadrupoles required)
+        kw.update(precision=S.PRECISION_MIXED, flags=S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING)
+    with batch_engine(S, sts, **kw) as e:
         e.run(steps)
         got = e.download()
         e.bbox()
@@ -105,7 +108,7 @@ def test_batch_steps_match_single_contexts(S, group):
     tot = 0
     for i, st in enumerate(sts):
         print(f"instance {i} n={st['x'].size}", flush=True)
-        with single_engine(S, st, group=group) as s:
+        with single_engine(S, st, **kw) as s:
             s.run(steps)
             want = s.download()
             s.bbox()

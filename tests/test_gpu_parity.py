@@ -74,9 +74,9 @@ def test_tree_bit_exact(S, oracle, workload, n):
 
 @pytest.mark.parametrize("quad", [False, True])
 def test_tree_over_stale_blocks(S, oracle, quad):
-    """Nothing clears the tree blocks between builds (every child slot is
-    written exactly once per build): a tree built where earlier steps and a
-    different body set left their trees is bit-exact against the oracle."""
+    """The blocks a build fills are cleared by that build, not only when the
+    context is created: a tree built where earlier steps and a different body
+    set left their trees is bit-exact against the oracle."""
     print(f"tree over stale blocks quad={quad}", flush=True)
     n = 65536
     flags = (S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING) if quad else 0
@@ -103,6 +103,50 @@ def test_tree_over_stale_blocks(S, oracle, quad):
                 oracle.quad_exact(ot, cur["x"], cur["y"], cur["mass"])
             _, _, it = oracle.walk(ot, cur["x"], cur["y"], cur["mass"], obb, 0.5, eps2, 64, **kw)
             assert np.array_equal(f["inter"], it), w
+
+
+# Mixed precision: the GPU and the oracle compute identical FP16 inputs except
+# where the hardware rsqrt's last bit moves a value across an FP16 rounding
+# boundary (rare), so the forces agree to FP32-walk precision in rms.
+MIXED_RMS_TOL = 2e-5
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("workload,n", [(2, 65536), (1, 65536), (0, 4096), (4, 65536)])
+def test_mixed_precision_parity(S, oracle, workload, n, schedule):
+    """STELLAR_PRECISION_MIXED: interaction counts equal the oracle's (the vote
+    is the FP32 walk's), forces within MIXED_RMS_TOL of the oracle's FP16
+    restatement, and against the FP64 direct sum at most 4e-4 (rms relative)
+    less accurate than the FP32 quadrupole walk of the same tree."""
+    print(f"mixed parity workload={workload} n={n} schedule={schedule}", flush=True)
+    st = ic(workload, n)
+    eps2 = float(np.float32(EPS[workload] ** 2))
+    flags = S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING
+    with make_engine(S, st, eps2, flags=flags, precision=S.PRECISION_MIXED) as e:
+        e.set_walk_schedule(schedule)
+        gpu_tree(e)
+        e.sort()
+        e.forces()
+        f = e.export_forces()
+        tg = np.arange(0, n, max(1, n // 2048), dtype=np.int32)
+        dx, dy = e.direct_fp64(tg, eps2=eps2)
+    x, y, m = st["x"], st["y"], st["mass"]
+    obb, ot = oracle_tree(oracle, st)
+    oracle.quad_exact(ot, x, y, m)
+    oracle.quad_mixed(ot, x, y, m, obb)
+    ax, ay, it = oracle.walk(ot, x, y, m, obb, 0.5, eps2, 64, opening=1, quad=True, mixed=True)
+    assert np.array_equal(f["inter"], it)
+    assert rms_rel(f["ax"], f["ay"], ax, ay) < MIXED_RMS_TOL
+    qx, qy, _ = oracle.walk(ot, x, y, m, obb, 0.5, eps2, 64, opening=1, quad=True)
+    fp32_err = rms_rel(qx[tg], qy[tg], dx, dy)
+    assert rms_rel(f["ax"][tg], f["ay"][tg], dx, dy) < fp32_err + 4e-4
+
+
+def test_mixed_precision_needs_quadrupoles_and_group_64(S):
+    st = ic(2, 4096)
+    for kw in ({"flags": S.FLAG_TEXTBOOK_OPENING}, {"flags": S.FLAG_QUADRUPOLE, "group": 32}):
+        with pytest.raises(S.StellarError):
+            make_engine(S, st, 1e-4, precision=S.PRECISION_MIXED, **kw)
 
 
 @pytest.mark.parametrize("group", [64, 32, 16])

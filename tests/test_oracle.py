@@ -667,3 +667,56 @@ def test_bmax_error_falls_with_theta(oracle):
         ax, ay, _ = oracle.walk(kt, x, y, m, bb, th, 1e-4, 64, bmax=True)
         errs.append(np.sqrt(np.mean(((ax[tg] - dx) ** 2 + (ay[tg] - dy) ** 2) / (dx ** 2 + dy ** 2))))
     assert errs[0] > errs[1] > errs[2]
+
+
+# ---- mixed precision (FP16 far field) ---------------------------------------
+def test_f16_rounding_against_numpy_and_brute_force(oracle):
+    """The oracle's FP16 rounding: nearest-even against numpy's float16
+    conversion, toward-zero against the largest-magnitude FP16 value not
+    above |x| (found by search over every finite FP16 value)."""
+    lib = oracle.lib()
+    rng = np.random.default_rng(21)
+    xs = np.concatenate([rng.normal(0, 1, 4000), rng.normal(0, 1e-5, 2000),
+                         rng.uniform(-7e4, 7e4, 2000), 2.0 ** rng.integers(-26, 17, 500),
+                         [0.0, -0.0, 65504.0, 65519.99, 65520.0, 6e-8, 2.98e-8, 5.96e-8]])
+    finite = np.arange(65536, dtype=np.uint16).view(np.float16)
+    finite = np.unique(finite[np.isfinite(finite)].astype(np.float64))
+    for x in xs:
+        got = lib.or_f16_round(float(x), 0)
+        want = float(np.float64(x).astype(np.float16))
+        assert got == want or (np.isinf(got) and np.isinf(want)), (x, got, want)
+        rz = lib.or_f16_round(float(x), 1)
+        ax = min(abs(x), 65504.0)
+        want_z = finite[np.searchsorted(finite, ax, side="right") - 1]
+        assert rz == np.copysign(want_z, x) or (x == 0 and rz == 0), (x, rz, want_z)
+
+
+def test_mixed_far_field_matches_fp32_quadrupoles(oracle):
+    """The FP16 bracket k B with B = e + t (-Qh.e + (5/2)(e.Qh.e) e) is the
+    monopole + quadrupole term: the mixed walk makes the FP32 walk's decisions
+    (same counts) and its forces differ from the FP32 quadrupole walk's by no
+    more than FP16 rounding allows; a wrong sign or factor in the quadrupole
+    bracket (about 1e-2 of a term) or a level off by one between the stored
+    Qh and the walk's ws^2 (a factor 4) does not fit in that bound."""
+    st = ic(2, 20000)
+    x, y, m = st["x"], st["y"], st["mass"]
+    bb, kt = _tree(oracle, st)
+    oracle.quad_exact(kt, x, y, m)
+    oracle.quad_mixed(kt, x, y, m, bb)
+    assert np.abs(kt["qh"]).max() < 4.0
+    eps2 = 1e-4
+    fx, fy, fi = oracle.walk(kt, x, y, m, bb, 0.5, eps2, 64, opening=1, quad=True)
+    mx, my, mi = oracle.walk(kt, x, y, m, bb, 0.5, eps2, 64, opening=1, quad=True, mixed=True)
+    assert np.array_equal(fi, mi)
+    d = np.sqrt(np.mean(((mx - fx) ** 2 + (my - fy) ** 2)
+                        / (fx.astype(np.float64) ** 2 + fy.astype(np.float64) ** 2)))
+    assert 1e-6 < d < 6e-4, d
+    dx, dy = oracle.direct_fp64(x, y, m, eps2)
+    err = np.sqrt(np.mean(((mx - dx) ** 2 + (my - dy) ** 2) / (dx ** 2 + dy ** 2)))
+    assert err < 1e-3, err
+    # the same walk with the quadrupole bracket dropped (Qh = 0) is the
+    # monopole walk, an order of magnitude less accurate
+    kq = dict(kt, qh=np.zeros_like(kt["qh"]))
+    zx, zy, _ = oracle.walk(kq, x, y, m, bb, 0.5, eps2, 64, opening=1, quad=True, mixed=True)
+    zerr = np.sqrt(np.mean(((zx - dx) ** 2 + (zy - dy) ** 2) / (dx ** 2 + dy ** 2)))
+    assert zerr > 5 * err
