@@ -149,8 +149,9 @@ def test_mixed_precision_needs_quadrupoles_and_group_64(S):
             make_engine(S, st, 1e-4, precision=S.PRECISION_MIXED, **kw)
 
 
-@pytest.mark.parametrize("group", [64, 32, 16])
-@pytest.mark.parametrize("workload,n", [(0, 4096), (2, 65536), (1, 65536)])
+@pytest.mark.parametrize("group,workload,n",
+                         [(g, w, n) for g in (64, 32, 16) for w, n in ((0, 4096), (2, 65536), (1, 65536))]
+                         + [(64, 1, 262144)])
 def test_forces_parity(S, oracle, workload, n, group):
     print(f"force parity workload={workload} n={n} group={group}", flush=True)
     st = ic(workload, n)
@@ -167,7 +168,10 @@ def test_forces_parity(S, oracle, workload, n, group):
     assert total == int(it.astype(np.int64).sum())
     mag = np.hypot(ax.astype(np.float64), ay.astype(np.float64))
     diff = np.hypot(f["ax"] - ax.astype(np.float64), f["ay"] - ay.astype(np.float64))
-    assert np.sqrt(np.mean((diff / mag) ** 2)) < FORCE_RMS_TOL
+    # at 2^18 Plummer bodies the dense core's bodies sum ~2x the terms, with
+    # more cancellation: the same per-term rsqrt rounding shows as ~2e-6
+    tol = FORCE_RMS_TOL if n <= 65536 else 2.5 * FORCE_RMS_TOL
+    assert np.sqrt(np.mean((diff / mag) ** 2)) < tol
     # per body, against a floor of the 10th-percentile magnitude: a body whose
     # terms nearly cancel has a tiny |a| and no meaningful relative error
     assert (diff / np.maximum(mag, np.percentile(mag, 10))).max() < FORCE_MAX_TOL
