@@ -101,6 +101,15 @@ typedef struct stellar_params {
  * eight): stellar_stage_times then reports the forces time and zeros.
  * Ignored when STELLAR_FLAG_STAGE_TIMING is set. */
 #define STELLAR_FLAG_FORCES_TIMING 32
+/* Treat a body alone in a root slot as the reference does (kernels.cu:
+ * 373-387, 397-398): it is pushed like a cell and its child slots are read.
+ * Those are empty for every body but body 0 (of each instance), so such a
+ * body exerts no force; body 0's slots are the root's own (child[0..3]), so
+ * the root's children are visited again with the next level's threshold and
+ * the accepted ones counted twice.  Without the flag the lone body is
+ * accumulated like any other body.  For the level-threshold walks only (not
+ * with STELLAR_FLAG_QUADRUPOLE or STELLAR_FLAG_BMAX_OPENING). */
+#define STELLAR_FLAG_REFERENCE_ROOT 64
 
 typedef struct stellar_ctx stellar_ctx;
 

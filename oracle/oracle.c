@@ -595,6 +595,14 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
       for (int k = 0; k < 4; ++k) {
         const int32_t ch = root[k];
         if (ch < 0) continue;
+        if (ch < n && p->ref_root) {
+          /* pushed like a cell; -1 marks the root's own slots (body 0) */
+          if (ch == 0) {
+            stk[sp] = -1;
+            dep[sp++] = depth0;
+          }
+          continue;
+        }
         if (ch < n) {
           for (int32_t j = 0; j < gs; ++j) {
             const float dx = x[ch] - px[j], dy = y[ch] - py[j];
@@ -611,8 +619,9 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
         --sp;
         const int32_t node = stk[sp];
         const float dp = 0.25f * dep[sp];
+        const int32_t* kids = node < 0 ? root : child + 4 * (int64_t)(node - n);
         for (int k = 0; k < 4; ++k) {
-          const int32_t ch = child[4 * (int64_t)(node - n) + k];
+          const int32_t ch = kids[k];
           if (ch < 0) continue;
           float cxv, cyv, cmv;
           const float* q = NULL;
@@ -698,6 +707,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
 /* Same acceptance rule stated recursively, accumulated in FP64. */
 typedef struct recctx {
   int32_t n, gs, opening;
+  const int32_t* root;
   const float *x, *y, *mass, *ccx, *ccy, *ccm, *cq, *cthr;
   const int32_t* child;
   float eps2;
@@ -729,10 +739,11 @@ static void rec_acc(recctx* R, float cxv, float cyv, float cmv, const float* q) 
   R->cnt++;
 }
 
+/* node < 0: the root's own slots (a lone body 0 in the reference's handling) */
 static void rec_open(recctx* R, int32_t node, float depth) {
   const float dp = 0.25f * depth;
   for (int k = 0; k < 4; ++k) {
-    const int32_t ch = R->child[4 * (int64_t)(node - R->n) + k];
+    const int32_t ch = node < 0 ? R->root[k] : R->child[4 * (int64_t)(node - R->n) + k];
     if (ch < 0) continue;
     if (ch < R->n) {
       rec_acc(R, R->x[ch], R->y[ch], R->mass[ch], NULL);
@@ -778,15 +789,18 @@ int or_walk_recursive(int32_t n, const float* x, const float* y,
         px[G + j] = y[sorted[s0 + j]];
         f[j] = f[G + j] = 0.0;
       }
-      recctx R = {n, gs, p->opening, x, y, mass, ccx, ccy, ccm, p->cq, p->cthr, child, p->eps2,
-                  px, px + G, f, f + G, 0};
+      recctx R = {n, gs, p->opening, root, x, y, mass, ccx, ccy, ccm, p->cq, p->cthr, child,
+                  p->eps2, px, px + G, f, f + G, 0};
       for (int k = 0; k < 4; ++k) {
         const int32_t ch = root[k];
         if (ch < 0) continue;
-        if (ch < n)
+        if (ch < n && p->ref_root) {
+          if (ch == 0) rec_open(&R, -1, depth0);
+        } else if (ch < n) {
           rec_acc(&R, x[ch], y[ch], mass[ch], NULL);
-        else
+        } else {
           rec_open(&R, ch, depth0);
+        }
       }
       for (int32_t j = 0; j < gs; ++j) {
         const int32_t b = sorted[s0 + j];
@@ -1111,7 +1125,7 @@ int or_step(int32_t n, float* x, float* y, float* vx, float* vy,
   t[4] = now_s();
   {
     /* the walk reads the pool layout: cells at n.. with CoM in cx/cy/cm */
-    or_walk_params wp = {p->theta, p->eps2, p->group, p->threads, 0, 0, NULL, NULL, NULL, NULL};
+    or_walk_params wp = {p->theta, p->eps2, p->group, p->threads, 0, 0, NULL, NULL, NULL, NULL, 0, 0};
     rc = or_walk(n, x, y, mass, bbox, sorted, child, child + 4 * (size_t)n,
                  ncells, cx + n, cy + n, cm + n, &wp, ax, ay, NULL);
     if (rc) goto out;

@@ -107,6 +107,11 @@ typedef struct or_walk_params {
                         (or_bmax_exact), compared with r like dp */
   const float* cqh;  /* mixed: Qh per cell (3 FP16 values, or_quad_mixed) */
   const int32_t* clevel; /* mixed: level of each cell */
+  int32_t ref_root;  /* 1: a body alone in a root slot is pushed like a cell
+                        (kernels.cu:373-387): body 0's child slots are the
+                        root's (the root's children are visited again one
+                        level down), any other body's are empty */
+  int32_t pad2;
 } or_walk_params;
 
 int or_walk(int32_t n, const float* x, const float* y, const float* mass,

@@ -35,7 +35,7 @@ _I64 = C.POINTER(C.c_int64)
 class WalkParams(C.Structure):
     _fields_ = [("theta", C.c_float), ("eps2", C.c_float), ("group", C.c_int32), ("threads", C.c_int32),
                 ("opening", C.c_int32), ("mixed", C.c_int32), ("cq", _F), ("cthr", _F), ("cqh", _F),
-                ("clevel", _I)]
+                ("clevel", _I), ("ref_root", C.c_int32), ("pad2", C.c_int32)]
 
 
 class StepParams(C.Structure):
@@ -219,7 +219,7 @@ def quad_exact(tree: dict, x, y, mass) -> None:
     tree["quad"] = q[: 3 * c]
 
 
-def _wp(tree, theta, eps2, group, threads, opening, quad, bmax=False, mixed=False):
+def _wp(tree, theta, eps2, group, threads, opening, quad, bmax=False, mixed=False, ref_root=False):
     cq = None
     if quad:
         cq = tree["quad"] if tree["quad"].size else np.zeros(3, np.float32)
@@ -232,7 +232,7 @@ def _wp(tree, theta, eps2, group, threads, opening, quad, bmax=False, mixed=Fals
         lv = np.ascontiguousarray(tree["level"], dtype=np.int32)
         lv = lv if lv.size else np.zeros(1, np.int32)
     p = WalkParams(theta, eps2, group, threads, opening, 1 if mixed else 0, _f(cq), _f(th), _f(qh),
-                   _i(lv))
+                   _i(lv), 1 if ref_root else 0, 0)
     return p, (cq, th, qh, lv)
 
 
@@ -258,13 +258,14 @@ def bmax_exact(tree: dict, x, y, mass, theta: float, eps2: float) -> None:
 
 
 def walk(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0,
-         opening: int = 0, quad: bool = False, bmax: bool = False, mixed: bool = False):
+         opening: int = 0, quad: bool = False, bmax: bool = False, mixed: bool = False,
+         ref_root: bool = False):
     """Group-vote walk (kernels.cu:340-435) over a canonical tree; by body id.
     bmax: per-cell thresholds tree["thr"] (bmax_exact) instead of levels.
     mixed: accepted cells' terms in FP16 from tree["qh"] (quad_mixed)."""
     x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
     n = x.size
-    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad, bmax, mixed)
+    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad, bmax, mixed, ref_root)
     ax = np.zeros(n, np.float32)
     ay = np.zeros(n, np.float32)
     it = np.zeros(n, np.int32)
@@ -281,10 +282,10 @@ def walk(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, thre
 
 
 def walk_recursive(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0,
-                   opening: int = 0, quad: bool = False, bmax: bool = False):
+                   opening: int = 0, quad: bool = False, bmax: bool = False, ref_root: bool = False):
     x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
     n = x.size
-    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad, bmax)
+    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad, bmax, ref_root=ref_root)
     ax = np.zeros(n, np.float64)
     ay = np.zeros(n, np.float64)
     it = np.zeros(n, np.int32)

@@ -63,6 +63,10 @@ bool params_ok(const stellar_params* p) {
   if (p->precision == STELLAR_PRECISION_MIXED &&
       (p->group != 64 || !(p->flags & STELLAR_FLAG_QUADRUPOLE)))
     return false;
+  // the reference's lone-root-body handling is a level-threshold walk
+  if ((p->flags & STELLAR_FLAG_REFERENCE_ROOT) &&
+      (p->flags & (STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_BMAX_OPENING)))
+    return false;
   return true;
 }
 
@@ -463,8 +467,8 @@ int stellar_set_params(stellar_ctx* c, const stellar_params* p) {
   // the build and centre-of-mass stages store per-child opening thresholds
   // and moments that depend on these: a tree built under the old values must
   // not be walked under the new ones, so the stages start again from bbox
-  constexpr int32_t kTreeFlags =
-      STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_TEXTBOOK_OPENING | STELLAR_FLAG_BMAX_OPENING;
+  constexpr int32_t kTreeFlags = STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_TEXTBOOK_OPENING |
+                                 STELLAR_FLAG_BMAX_OPENING | STELLAR_FLAG_REFERENCE_ROOT;
   if (((p->flags ^ c->p.flags) & kTreeFlags) || p->theta != c->p.theta ||
       p->eps2 != c->p.eps2 || p->precision != c->p.precision)
     c->stage = kNone;

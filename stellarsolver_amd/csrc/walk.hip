@@ -27,9 +27,12 @@
 // Which children are accepted depends only on the child and the group, never
 // on the visiting order, so every variant makes the same decisions as the CPU
 // restatement (bit-exact interaction counts); the order of summation differs.
-// One deliberate difference from the reference: a body alone in a root slot
-// is accumulated; the reference pushes it like a cell and then finds no
-// children (kernels.cu:373-387, 397-398), so that body exerts no force.
+// One deliberate difference from the reference, unless
+// STELLAR_FLAG_REFERENCE_ROOT is set: a body alone in a root slot is
+// accumulated; the reference pushes it like a cell and then reads its child
+// slots (kernels.cu:373-387, 397-398), so that body exerts no force -- and
+// body 0, whose child slots are the root's, makes the group revisit the
+// root's children one level down.
 //
 // QUAD adds each accepted cell's quadrupole term (a second 64-byte block of
 // {Qxx, Qxy, Qyy, threshold} per node); bodies stay monopoles.
@@ -61,6 +64,12 @@ struct WalkArgs {
   const int32_t* inst_of;
   const float4* ibbox;
   int unsoft;  // level thresholds compare with the unsoftened distance
+  // STELLAR_FLAG_REFERENCE_ROOT: a body alone in a root slot is handled as
+  // the reference handles it (see walk64_group); needs the body ids and the
+  // instances' first ids
+  int ref_root;
+  const int32_t* id;
+  const int32_t* ioff;
 };
 
 // r = dx*dx + dy*dy + eps2 as dx*dx + (dy*dy + eps2), two fused
@@ -264,6 +273,20 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
     const ChildRec c = root[k];
     if (c.code == 0) continue;
     if (c.code < 0) {
+      if (!kQ && A.ref_root) {
+        // the reference pushes a root slot's lone body like a cell
+        // (kernels.cu:373-387) and then reads that body's child slots
+        // (:397-398): empty for every body but body 0, whose slots are the
+        // root's own (child[0..3]), so the root's children are visited once
+        // more with the next level's threshold; any other lone body exerts no
+        // force
+        const int32_t b = __builtin_amdgcn_readfirstlane(A.id[-1 - c.code]);
+        if (b == __builtin_amdgcn_readfirstlane(A.ioff[inst])) {
+          if (lane == 0) st[sp] = (uint32_t)inst;  // the root block, level field 0
+          ++sp;
+        }
+        continue;
+      }
       const float dx = c.x - px, dy = c.y - py;
       acc.mono(c.m, dist2(dx, dy, eps2), dx, dy);
     } else {
@@ -484,6 +507,18 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
     const ChildRec c = root[k];
     if (c.code == 0) continue;
     if (c.code < 0) {
+      if (!THR && A.ref_root) {
+        // the reference's handling of a lone root body (see walk64_group):
+        // body 0 revisits the root's children, any other exerts no force
+        if (A.id[-1 - c.code] == A.ioff[inst]) {
+          if (glane == 0) {
+            stk_node[slot][sp] = inst;
+            stk_dep[slot][sp] = depth0;
+          }
+          ++sp;
+        }
+        continue;
+      }
       const float dx = c.x - px, dy = c.y - py;
       acc.mono(c.m, dist2(dx, dy, eps2), dx, dy);
     } else {
@@ -686,7 +721,8 @@ void launch_forces(stellar_ctx* c, bool zero_counter) {
   WalkArgs A{B.x,       B.y,       B.blk,       B.qblk,   n,
              c->p.eps2, tdiv,      B.dev,       B.ax,     B.ay,
              B.inter,   B.gtab[gi], ngroups,    B.inst_of, B.ibbox,
-             (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? 1 : 0};
+             (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? 1 : 0,
+             (c->p.flags & STELLAR_FLAG_REFERENCE_ROOT) ? 1 : 0, B.id, B.ioff};
   if (zero_counter) launch_zero_counter(c);
   switch (c->p.group) {
     case 32:

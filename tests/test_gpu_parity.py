@@ -669,3 +669,45 @@ def test_deep_plummer_tree_and_forces(S, oracle):
     # ~1700 FP32 terms per body whose pulls largely cancel in the dense core:
     # summation order and rsqrt rounding leave a few parts in 1e6
     assert rms_rel(f["ax"], f["ay"], ax, ay) < 1e-5
+
+
+@pytest.mark.parametrize("group", [64, 32, 16])
+@pytest.mark.parametrize("case", ["disk", "body0", "body1"])
+def test_reference_lone_root_body(S, oracle, case, group):
+    """STELLAR_FLAG_REFERENCE_ROOT: a body alone in a root slot handled as the
+    reference does (body 0 revisits the root's children, any other lone body
+    exerts no force): counts equal the oracle's, forces within the rsqrt
+    tolerance; the plain walk differs."""
+    print(f"reference root case={case} group={group}", flush=True)
+    if case == "disk":
+        st = ic(0, 4096)
+        st = {k: v.copy() for k, v in st.items()}
+        st["x"][0], st["y"][0] = np.float32(-500.0), np.float32(-500.0)
+    else:
+        a, b = (0, 1) if case == "body0" else (1, 0)
+        x = np.zeros(3, np.float32)
+        y = np.zeros(3, np.float32)
+        x[a], y[a] = 1.0, 1.0
+        x[b], y[b] = -1.0, -1.0
+        x[2], y[2] = -1.1, -1.05
+        z = np.zeros(3, np.float32)
+        st = {"x": x, "y": y, "vx": z, "vy": z, "mass": np.array([3.0, 1.0, 2.0], np.float32)}
+    eps2 = 1e-4
+    flags = S.FLAG_REFERENCE_ROOT
+    with make_engine(S, st, eps2, group=group, flags=flags) as e:
+        gpu_tree(e)
+        e.sort()
+        e.forces()
+        f = e.export_forces()
+    obb, ot = oracle_tree(oracle, st)
+    ax, ay, it = oracle.walk(ot, st["x"], st["y"], st["mass"], obb, 0.5, eps2, group, ref_root=True)
+    assert np.array_equal(f["inter"], it)
+    assert rms_rel(f["ax"], f["ay"], ax, ay) < FORCE_RMS_TOL
+    _, _, plain = oracle.walk(ot, st["x"], st["y"], st["mass"], obb, 0.5, eps2, group)
+    assert not np.array_equal(it, plain)
+
+
+def test_reference_root_needs_level_thresholds(S):
+    st = ic(2, 4096)
+    with pytest.raises(S.StellarError):
+        make_engine(S, st, 1e-4, flags=S.FLAG_REFERENCE_ROOT | S.FLAG_QUADRUPOLE)

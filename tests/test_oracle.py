@@ -683,7 +683,8 @@ def test_f16_rounding_against_numpy_and_brute_force(oracle):
     finite = np.unique(finite[np.isfinite(finite)].astype(np.float64))
     for x in xs:
         got = lib.or_f16_round(float(x), 0)
-        want = float(np.float64(x).astype(np.float16))
+        with np.errstate(over="ignore"):
+            want = float(np.float64(x).astype(np.float16))
         assert got == want or (np.isinf(got) and np.isinf(want)), (x, got, want)
         rz = lib.or_f16_round(float(x), 1)
         ax = min(abs(x), 65504.0)
@@ -720,3 +721,80 @@ def test_mixed_far_field_matches_fp32_quadrupoles(oracle):
     zx, zy, _ = oracle.walk(kq, x, y, m, bb, 0.5, eps2, 64, opening=1, quad=True, mixed=True)
     zerr = np.sqrt(np.mean(((zx - dx) ** 2 + (zy - dy) ** 2) / (dx ** 2 + dy ** 2)))
     assert zerr > 5 * err
+
+
+# ---- the reference's handling of a body alone in a root slot ------------------
+def test_reference_lone_root_body_known_answers(oracle):
+    """kernels.cu:373-387,397-398: a lone body in a root slot is pushed like a
+    cell and its child slots read.  Body 0's slots are the root's, so the root's
+    children are walked again one level down (the close pair's cells are
+    opened again and counted twice); any other lone body exerts no force.
+    Bodies: one alone in the north-east quadrant, a close pair in the
+    south-west (a chain of cells down to the level that separates them)."""
+    eps2 = 1e-4
+    x = np.array([1.0, -1.0, -1.1], np.float32)
+    y = np.array([1.0, -1.0, -1.05], np.float32)
+    m = np.array([3.0, 1.0, 2.0], np.float32)
+    st = {"x": x, "y": y, "mass": m}
+    bb, kt = _tree(oracle, st)
+    assert list(kt["root"]) == [0, -1, -1, 3]  # body 0 alone, the pair's cell
+    _, _, plain = oracle.walk(kt, x, y, m, bb, 0.5, eps2, 1)
+    # body 0: itself, the pair's level-2 cell; bodies 1, 2: body 0, the pair
+    assert list(plain) == [2, 3, 3]
+    ax, ay, it = oracle.walk(kt, x, y, m, bb, 0.5, eps2, 1, ref_root=True)
+    # body 0: the pair's level-2 cell, then the root's slots again: itself and
+    # the level-1 cell (accepted at the level-2 threshold); bodies 1, 2: the
+    # pair, then the root's slots again: body 0 and the pair once more
+    assert list(it) == [3, 5, 5]
+
+    def f(i, j_list, xs=x, ys=y):
+        gx = gy = 0.0
+        for (px, py, pm) in j_list:
+            dx, dy = px - float(xs[i]), py - float(ys[i])
+            r = (dx * dx + dy * dy + eps2) ** 1.5
+            gx += pm * dx / r
+            gy += pm * dy / r
+        return gx, gy
+
+    cm = (float(x[1]) + 2.0 * float(x[2])) / 3.0, (float(y[1]) + 2.0 * float(y[2])) / 3.0
+    pair = [(float(x[1]), float(y[1]), 1.0), (float(x[2]), float(y[2]), 2.0)]
+    e0 = f(0, [(cm[0], cm[1], 3.0), (cm[0], cm[1], 3.0)])
+    e1 = f(1, pair + pair + [(1.0, 1.0, 3.0)])
+    assert np.allclose([ax[0], ay[0]], e0, rtol=1e-5)
+    assert np.allclose([ax[1], ay[1]], e1, rtol=1e-5)
+    # a lone body other than 0 exerts no force
+    x2 = np.array([-1.0, 1.0, -1.1], np.float32)
+    y2 = np.array([-1.0, 1.0, -1.05], np.float32)
+    st2 = {"x": x2, "y": y2, "mass": m}
+    bb2, kt2 = _tree(oracle, st2)
+    assert list(kt2["root"]) == [1, -1, -1, 3]
+    bx, by, it2 = oracle.walk(kt2, x2, y2, m, bb2, 0.5, eps2, 1, ref_root=True)
+    # body 1 (lone): the pair's level-2 cell only; the pair: itself, no body 1
+    assert list(it2) == [2, 1, 2]
+    gx, gy = f(0, [(float(x2[0]), float(y2[0]), 3.0), (float(x2[2]), float(y2[2]), 2.0)], x2, y2)
+    assert np.allclose([bx[0], by[0]], [gx, gy], rtol=1e-5)  # no pull from body 1
+    c2 = ((3.0 * float(x2[0]) + 2.0 * float(x2[2])) / 5.0, (3.0 * float(y2[0]) + 2.0 * float(y2[2])) / 5.0)
+    assert np.allclose([bx[1], by[1]], f(1, [(c2[0], c2[1], 5.0)], x2, y2), rtol=1e-5)
+
+
+def _lone_body_disk(n=4096):
+    """Workload 0 (a disk about a central mass) with body 0 moved far to the
+    south-west, alone in its root quadrant."""
+    st = ic(0, n)
+    st = {k: v.copy() for k, v in st.items()}
+    st["x"][0], st["y"][0] = np.float32(-500.0), np.float32(-500.0)
+    return st
+
+
+@pytest.mark.parametrize("group", [1, 32, 64])
+def test_reference_lone_root_body_matches_recursive_statement(oracle, group):
+    st = _lone_body_disk()
+    x, y, m = st["x"], st["y"], st["mass"]
+    bb, kt = _tree(oracle, st)
+    assert sorted(kt["root"].tolist())[:3] == [-1, -1, 0]
+    ax, ay, it = oracle.walk(kt, x, y, m, bb, 0.5, 1e-4, group, ref_root=True)
+    rx, ry, rit = oracle.walk_recursive(kt, x, y, m, bb, 0.5, 1e-4, group, ref_root=True)
+    assert np.array_equal(it, rit)
+    assert np.sqrt(np.mean(((ax - rx) ** 2 + (ay - ry) ** 2) / (rx ** 2 + ry ** 2))) < 5e-6
+    _, _, plain = oracle.walk(kt, x, y, m, bb, 0.5, 1e-4, group)
+    assert (it > plain).all()  # every group walks the root's children twice
