@@ -103,8 +103,8 @@ __global__ __launch_bounds__(256) void update_kernel(
 __global__ __launch_bounds__(256) void output_kernel(
     int32_t n, int64_t cap, int32_t nroot, const float* __restrict__ x,
     const float* __restrict__ y, const int32_t* __restrict__ id,
-    const ChildRec* __restrict__ blk, const int32_t* __restrict__ cparent,
-    const uint8_t* __restrict__ cslot, const DevScalars* __restrict__ dev,
+    const ChildRec* __restrict__ blk, const uint64_t* __restrict__ cmeta,
+    const DevScalars* __restrict__ dev,
     float* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -116,7 +116,8 @@ __global__ __launch_bounds__(256) void output_kernel(
   if (c < dev->ncells && c < cap) {
     // a cell's centre of mass is its record in the parent's block
     const int64_t b = c + nroot;
-    const ChildRec q = blk[4 * (int64_t)cparent[b] + (cslot[b] & 3)];
+    const uint64_t meta = cmeta[b];
+    const ChildRec q = blk[4 * (int64_t)meta_parent(meta) + meta_slot(meta)];
     reinterpret_cast<float2*>(out)[n + c] = make_float2(q.x, q.y);
   }
 }
@@ -310,7 +311,7 @@ void launch_output(stellar_ctx* c) {
   if (n == 0) return;
   Buffers& B = c->b;
   output_kernel<<<grid_for(n + c->cap, 256), 256, 0, c->stream>>>(
-      n, c->cap, c->ninst, B.x, B.y, B.id, B.blk, B.cparent, B.cslot, B.dev, B.out);
+      n, c->cap, c->ninst, B.x, B.y, B.id, B.blk, B.cmeta, B.dev, B.out);
 }
 
 void launch_zero_counter(stellar_ctx* c) {

@@ -444,44 +444,26 @@ struct LinkArgs {
   const float* y;
   const float* m;
   ChildRec* blk;
-  int32_t* cstart;
-  int32_t* cparent;
-  uint8_t* clevel;
-  uint8_t* cslot;
+  uint64_t* cmeta;
   DevScalars* dev;
   const int32_t* inst_of;
   const int32_t* ioff;
   int32_t nroot;  // number of instances = root blocks; cell c is block nroot + c
   MinTree mt;
-  // second block per node (quadrupoles / bmax): a cell child's opening
-  // threshold (its level threshold; the bmax pass replaces it)
-  float4* qblk;
-  int qthr;
   const float4* ibbox;
-  float theta_div;
-  float eps2;
-  int unsoft;
   // x, y and m moved into tree order for the sort stage, which swaps them in
   float* x2;
   float* y2;
   float* m2;
 };
 
-__device__ __forceinline__ void set_cell_threshold(const LinkArgs& A, int64_t slot_index,
-                                                   float t) {
-  reinterpret_cast<float*>(A.qblk + slot_index)[3] = A.unsoft ? t + A.eps2 : t;
-}
-
 // Id of the first cell pair a opens.
 __device__ __forceinline__ int64_t coff_at(const LinkArgs& A, int32_t a) {
   return (int64_t)A.bpre[a / kScanBlock] + A.clocal[a];
 }
 
-__device__ __forceinline__ void put_cell(const LinkArgs& A, int64_t blk, int slot, int64_t child,
-                                         int child_level, float depth0) {
+__device__ __forceinline__ void put_cell(const LinkArgs& A, int64_t blk, int slot, int64_t child) {
   A.blk[4 * blk + slot] = ChildRec{0.f, 0.f, 0.f, (int32_t)child};
-  if (A.qthr)
-    set_cell_threshold(A, 4 * blk + slot, walk_level_threshold(depth0, child_level, 0.f, false));
 }
 
 // The tree's node blocks and the cells' metadata; one thread per tree
@@ -519,7 +501,6 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
     if (cs > A.cap) atomicOr(&A.dev->err, kErrPool);
     A.dev->ncells = (int32_t)(cs < A.cap ? cs : A.cap);
   }
-  const float depth0 = A.qthr ? walk_root_depth(A.ibbox[inst], A.theta_div) : 0.f;
   // block of the node created at s at level L (-1: beyond the pool)
   auto created = [&](int L) -> int64_t {
     if (L == 0) return inst;
@@ -544,20 +525,13 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
     for (int L = first ? 0 : lp + 1; L <= ls; ++L) {
       const int64_t b = created(L);
       if (b < 0) break;
-      if (L == 0) {
-        A.cstart[inst] = i0;
-        A.clevel[inst] = 0;
-        A.cparent[inst] = -1;
-      } else {
-        A.cstart[b] = s;
-        A.clevel[b] = (uint8_t)L;
-        A.cslot[b] = (uint8_t)digit_at(A.K, s, L);
-        A.cparent[b] = (int32_t)(L == lp + 1 && !first ? pblk : created(L - 1));
-      }
+      if (L > 0)
+        A.cmeta[b] = cell_meta(s, (int32_t)(L == lp + 1 && !first ? pblk : created(L - 1)),
+                               digit_at(A.K, s, L), L);
       const int d = L < kMaxLevel ? digit_at(A.K, s, L + 1) : 0;
       if (L < ls) {
         const int64_t cb = created(L + 1);
-        if (cb >= 0) put_cell(A, b, d, cb, L + 1, depth0);
+        if (cb >= 0) put_cell(A, b, d, cb);
       } else {
         A.blk[4 * b + d] = body;
       }
@@ -572,7 +546,7 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
     } else {
       if (ls > lp) {
         const int64_t cb = created(lp + 1);
-        if (cb >= 0) put_cell(A, pblk, dcur, cb, lp + 1, depth0);
+        if (cb >= 0) put_cell(A, pblk, dcur, cb);
       } else {
         A.blk[4 * pblk + dcur] = body;
       }
@@ -673,12 +647,9 @@ void launch_build(stellar_ctx* c) {
   const float theta = c->p.theta;
   const bool textbook = (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) != 0;
   LinkArgs A{n,         c->cap,   K,       B.coff,    B.bsum,   B.ncell,  B.lam,
-             B.perm,    B.x,      B.y,     B.m,       B.blk,    B.cstart,
-             B.cparent, B.clevel, B.cslot, B.dev,     B.inst_of, B.ioff,
-             c->ninst,  mt,
-             B.qblk,
-             (c->p.flags & (STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_BMAX_OPENING)) ? 1 : 0,
-             B.ibbox,   textbook ? theta * theta : theta, c->p.eps2, textbook ? 1 : 0,
+             B.perm,    B.x,      B.y,     B.m,       B.blk,    B.cmeta,
+             B.dev,     B.inst_of, B.ioff,
+             c->ninst,  mt,        B.ibbox,
              B.x2,      B.y2,     B.m2};
   link_kernel<<<nblk, kScanBlock, 0, c->stream>>>(A);
 }

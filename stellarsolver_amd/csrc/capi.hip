@@ -81,7 +81,7 @@ void free_all(stellar_ctx* c) {
                   B.x2,    B.y2,     B.vx2,     B.vy2,    B.m2,     B.id2,
                   B.ax,    B.ay,     B.inter,   B.key,    B.key_alt, B.perm,
                   B.perm_alt, B.key2, B.ncell,  B.coff,   B.tmp,    B.blk, B.bsum,
-                  B.cstart, B.lam, B.cparent, B.clevel, B.cslot,
+                  B.cmeta, B.lam,
                   B.csum,  B.out,    B.dev,
                   B.bbox_part, B.scount, B.choff,
                   B.lmin,  B.qblk,  B.spine,   B.clist, B.clen,
@@ -361,10 +361,7 @@ int create_ctx(const stellar_params* p, int32_t ninst, const int32_t* counts,
   B.tmp_bytes = build_scratch_bytes((int32_t)N);
   chk(hipMalloc(&B.tmp, B.tmp_bytes));
   chk(dmalloc(&B.blk, 4 * NB));
-  chk(dmalloc(&B.cstart, NB));
-  chk(dmalloc(&B.cparent, NB));
-  chk(dmalloc(&B.clevel, NB));
-  chk(dmalloc(&B.cslot, NB));
+  chk(dmalloc(&B.cmeta, NB));
   chk(dmalloc(&B.csum, kSumStride * NB));
   chk(dmalloc(&B.qblk, 4 * NB));
   const size_t clcap = (size_t)com_list_capacity(c->cap);
@@ -780,28 +777,19 @@ int stellar_export_tree_instance(stellar_ctx* c, int32_t inst, int32_t* root4,
       count[q] = t;
     }
   }
-  if (C > 0 && (start || level)) {
-    if (start) {
-      std::vector<int32_t> tmp((size_t)C);
-      HIP_TRY(c, hipMemcpy(tmp.data(), B.cstart + nr + c0, (size_t)C * 4, hipMemcpyDeviceToHost));
-      for (int64_t q = 0; q < C; ++q) start[q] = tmp[(size_t)q] - lo;
-    }
-    if (level) {
-      std::vector<uint8_t> lv((size_t)C);
-      HIP_TRY(c, hipMemcpy(lv.data(), B.clevel + nr + c0, (size_t)C, hipMemcpyDeviceToHost));
-      for (int64_t q = 0; q < C; ++q) level[q] = lv[(size_t)q];
-    }
-  }
+  std::vector<uint64_t> meta((size_t)(C > 0 ? C : 1));
+  if (C > 0 && (start || level || com_x || com_y || com_m))
+    HIP_TRY(c, hipMemcpy(meta.data(), B.cmeta + nr + c0, (size_t)C * 8, hipMemcpyDeviceToHost));
+  if (C > 0 && start)
+    for (int64_t q = 0; q < C; ++q) start[q] = meta_start(meta[(size_t)q]) - lo;
+  if (C > 0 && level)
+    for (int64_t q = 0; q < C; ++q) level[q] = meta_level(meta[(size_t)q]);
   if (C > 0 && (com_x || com_y || com_m)) {
     if (c->stage < kComDone) return fail(c, STELLAR_ERR_STATE, "no centre of mass yet");
     // a cell's centre of mass and mass are its record in the parent's block
-    std::vector<int32_t> par((size_t)C);
-    std::vector<uint8_t> slt((size_t)C);
-    HIP_TRY(c, hipMemcpy(par.data(), B.cparent + nr + c0, (size_t)C * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(slt.data(), B.cslot + nr + c0, (size_t)C, hipMemcpyDeviceToHost));
     for (int64_t q = 0; q < C; ++q) {
-      const int64_t pb = par[(size_t)q];
-      const int k = slt[(size_t)q] & 3;
+      const int64_t pb = meta_parent(meta[(size_t)q]);
+      const int k = meta_slot(meta[(size_t)q]);
       ChildRec r{0.f, 0.f, 0.f, 0};
       if (pb == inst)
         r = rootb[(size_t)k];

@@ -42,9 +42,7 @@ constexpr int kMomL = kMom + 2;
 struct ComArgs {
   ChildRec* blk;
   float4* qblk;
-  const int32_t* cparent;
-  const uint8_t* cslot;
-  const uint8_t* clevel;
+  const uint64_t* cmeta;
   double* csum;  // kSumStride per block
   DevScalars* dev;
   int quad;
@@ -55,9 +53,13 @@ struct ComArgs {
   // mixed precision: qblk .x/.y hold Qh = Q / (m ws^2) as FP16 bits (Qxx | Qxy
   // << 16, Qyy), ws = S / 2^L with S the larger extent of the instance's box
   int mixed;
-  const int32_t* cstart;
   const int32_t* inst_of;
   const float4* ibbox;
+  // the walk's level threshold of a cell, stored beside its quadrupole
+  // moments (walk_level_threshold)
+  float theta_div;
+  float eps2f;
+  int unsoft;
 };
 
 // FP16 bits of v (round to nearest even), clamped to the finite range.
@@ -192,29 +194,29 @@ __device__ void publish_cell(const ComArgs& A, const ChunkLds<EXT>& L, int i, in
                              int32_t p, int s) {
   const double* o = L.mom[i];
   A.blk[4 * (int64_t)p + s] = ChildRec{(float)o[7], (float)o[8], (float)o[0], b};
-  if (A.mixed) {
-    // Qh of a cell at level lev: Q / (m ws^2), ws^2 = S^2 / 4^lev in FP32
-    const float4 bb = A.ibbox[A.inst_of[A.cstart[b]]];
-    const float w = bb.y - bb.x, h = bb.w - bb.z;
-    const float sz = w > h ? w : h;
-    const float ws2 = __builtin_ldexpf(sz * sz, -2 * (int)L.lev[i]);
-    const double den = o[0] * (double)ws2;
-    const float qxx = den != 0.0 ? (float)(o[3] / den) : 0.f;
-    const float qxy = den != 0.0 ? (float)(o[4] / den) : 0.f;
-    const float qyy = den != 0.0 ? (float)(o[5] / den) : 0.f;
-    uint32_t* q = reinterpret_cast<uint32_t*>(A.qblk + 4 * (int64_t)p + s);
-    q[0] = half_bits(qxx) | (half_bits(qxy) << 16);
-    q[1] = half_bits(qyy);
-    if (A.bmax) reinterpret_cast<float*>(q)[3] = (float)((o[6] * o[6]) / A.t2 + A.eps2);
-  } else if (A.bmax) {
-    const float thr = (float)((o[6] * o[6]) / A.t2 + A.eps2);
-    A.qblk[4 * (int64_t)p + s] = make_float4((float)o[3], (float)o[4], (float)o[5], thr);
-  } else if (A.quad) {
-    // .w holds the cell's level threshold, written by the build
-    float* q = reinterpret_cast<float*>(A.qblk + 4 * (int64_t)p + s);
-    q[0] = (float)o[3];
-    q[1] = (float)o[4];
-    q[2] = (float)o[5];
+  if (A.quad || A.bmax) {
+    // second block: the moments the walk needs and the cell's opening
+    // threshold, one 16-byte store
+    const float4 bb = A.ibbox[A.inst_of[meta_start(A.cmeta[b])]];
+    const int lev = L.lev[i];
+    const float thr =
+        A.bmax ? (float)((o[6] * o[6]) / A.t2 + A.eps2)
+               : walk_level_threshold(walk_root_depth(bb, A.theta_div), lev, A.eps2f, A.unsoft != 0);
+    float4 q = make_float4((float)o[3], (float)o[4], (float)o[5], thr);
+    if (A.mixed) {
+      // Qh of a cell at level lev: Q / (m ws^2), ws^2 = S^2 / 4^lev in FP32
+      const float w = bb.y - bb.x, h = bb.w - bb.z;
+      const float sz = w > h ? w : h;
+      const float ws2 = __builtin_ldexpf(sz * sz, -2 * lev);
+      const double den = o[0] * (double)ws2;
+      const float qxx = den != 0.0 ? (float)(o[3] / den) : 0.f;
+      const float qxy = den != 0.0 ? (float)(o[4] / den) : 0.f;
+      const float qyy = den != 0.0 ? (float)(o[5] / den) : 0.f;
+      q.x = __uint_as_float(half_bits(qxx) | (half_bits(qxy) << 16));
+      q.y = __uint_as_float(half_bits(qyy));
+      q.z = 0.f;
+    }
+    A.qblk[4 * (int64_t)p + s] = q;
   }
   const int pi = L.pidx[i];
   if (pi < 0 || !L.local[pi]) {
@@ -250,10 +252,11 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
     if (i < ne) {
       const int32_t b = list ? list[p0 + i] : p0 + i + A.nroot;
       L.ids[i] = b;
-      lev = A.clevel[b];
+      const uint64_t meta = A.cmeta[b];
+      lev = meta_level(meta);
       L.lev[i] = (uint8_t)lev;
-      pb = A.cparent[b];
-      sl = A.cslot[b];
+      pb = meta_parent(meta);
+      sl = meta_slot(meta);
       const ChildRec* r = A.blk + 4 * (int64_t)b;
 #pragma unroll
       for (int k = 0; k < 4; ++k) L.rec[i][k] = r[k];
@@ -269,7 +272,7 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
     if (lane == 63) after = 255;
     __syncthreads();
     for (int w = wv + 1; w < kComThreads / 64; ++w) after = min(after, L.wmin[w]);
-    const int nlev = next == INT32_MAX ? -1 : (int)A.clevel[next];
+    const int nlev = next == INT32_MAX ? -1 : meta_level(A.cmeta[next]);
     after = min(after, nlev);
     if (i < ne) L.local[i] = after <= lev ? 1 : 0;
   }
@@ -467,11 +470,13 @@ int64_t com_list_capacity(int64_t cap) {
 void launch_com(stellar_ctx* c) {
   if (c->n == 0 || c->cap == 0) return;
   Buffers& B = c->b;
-  ComArgs A{B.blk,  B.qblk, B.cparent, B.cslot, B.clevel, B.csum, B.dev,
+  ComArgs A{B.blk,  B.qblk, B.cmeta, B.csum, B.dev,
             (c->p.flags & STELLAR_FLAG_QUADRUPOLE) ? 1 : 0, c->ninst,
             (c->p.flags & STELLAR_FLAG_BMAX_OPENING) ? 1 : 0,
             (double)c->p.theta * (double)c->p.theta, (double)c->p.eps2,
-            c->p.precision == STELLAR_PRECISION_MIXED ? 1 : 0, B.cstart, B.inst_of, B.ibbox};
+            c->p.precision == STELLAR_PRECISION_MIXED ? 1 : 0, B.inst_of, B.ibbox,
+            (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? c->p.theta * c->p.theta : c->p.theta,
+            c->p.eps2, (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? 1 : 0};
   // pass 1 over all cells; each later list holds at most kSlots cells per
   // chunk of the pass before, so grids are sized from that bound and chunks
   // beyond the actual length exit at once

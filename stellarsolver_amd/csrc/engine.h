@@ -111,6 +111,22 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// Metadata of a cell, one 64-bit word per cell block (written with one store
+// by the build): its first tree position (30 bits; n <= 2^30), its parent's
+// block (26 bits; blocks < kMaxBlocks = 2^26), its slot in the parent (2 bits)
+// and its level - 1 (6 bits; levels 1 .. 64).
+__host__ __device__ inline uint64_t cell_meta(int32_t start, int32_t parent, int slot,
+                                              int level) {
+  return (uint64_t)(uint32_t)start | ((uint64_t)(uint32_t)parent << 30) |
+         ((uint64_t)(slot & 3) << 56) | ((uint64_t)(level - 1) << 58);
+}
+__host__ __device__ inline int32_t meta_start(uint64_t m) { return (int32_t)(m & 0x3fffffffu); }
+__host__ __device__ inline int32_t meta_parent(uint64_t m) {
+  return (int32_t)((m >> 30) & 0x3ffffffu);
+}
+__host__ __device__ inline int meta_slot(uint64_t m) { return (int)((m >> 56) & 3u); }
+__host__ __device__ inline int meta_level(uint64_t m) { return (int)(m >> 58) + 1; }
+
 // FP64 moments per cell: sum m, sum m x, sum m y, Qxx, Qxy, Qyy, bmax
 constexpr int kSumStride = 7;
 
@@ -174,8 +190,7 @@ struct Buffers {
   size_t tmp_bytes = 0;
   // cells
   ChildRec* blk = nullptr;    // (1+cap) blocks of 4 records
-  int32_t *cstart = nullptr, *cparent = nullptr;
-  uint8_t *clevel = nullptr, *cslot = nullptr;
+  uint64_t* cmeta = nullptr;  // per cell block: cell_meta (roots hold none)
   double* csum = nullptr;     // kSumStride FP64 moments per cell
   float4* qblk = nullptr;     // per child record: Qxx, Qxy, Qyy, 0
   int32_t* spine = nullptr;   // CoM pass leftovers, kMaxLevel slots per chunk
