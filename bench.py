@@ -45,12 +45,24 @@ OPS_PER_INTERACTION = 13
 # (2), less the monopole's 3 multiplies
 OPS_PER_QUAD_TERM = 11
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per walk launch on the default workload (colliding disks, N = 2^20,
-# theta 0.5, quadrupoles + textbook opening, group 64), from two rocprofv3
-# --pmc runs of this script: FETCH_SIZE 182,078 KiB x 2.0 + WRITE_SIZE
-# 13,312 KiB x 1.0 per dispatch, the factors being what the same counters read
-# for a 1 GiB device copy (profiles/hbm_traffic_colliding_disks_1M.md).
-WALK_TRAFFIC_BYTES = {(2, 1 << 20, 0.5, 64, "quad", "textbook"): (181757 * 2 + 13312) * 1024}
+# Counter figures of the default workload (colliding disks, N = 2^20, theta 0.5,
+# quadrupoles + textbook opening, group 64) from rocprofv3 runs of this script
+# (profiles/hbm_traffic_colliding_disks_1M.md, profiles/walk_counters_colliding_disks_1M.md).
+DEFAULT_KEY = (2, 1 << 20, 0.5, 64, "quad", "textbook")
+# HBM bytes per step of each stage's kernels: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# (KiB per dispatch), the factors being what the same counters read for a 1 GiB
+# device copy; "forces" is the walk launch.
+STAGE_TRAFFIC_BYTES = {DEFAULT_KEY: {
+    "bbox": (4125 * 2 + 8) * 1024,
+    "build": (64613 * 2 + 186595) * 1024,
+    "com": (36474 * 2 + 40367) * 1024,
+    "sort": (21201 * 2 + 12288) * 1024,
+    "forces": (180171 * 2 + 13312) * 1024,
+    "update": (12294 * 2 + 16384) * 1024,
+}}
+# The walk's VALU busy fraction: SQ_ACTIVE_INST_VALU / (256 CUs x GRBM_GUI_ACTIVE
+# per XCD) per dispatch, 432.1e6 / (256 x 18.12e6 / 8).
+WALK_VALU_BUSY = {DEFAULT_KEY: 432.1e6 / (256 * 18.12e6 / 8)}
 
 
 def parse():
@@ -182,16 +194,36 @@ def force_error(eng, n, sample):
     return rms_rel(f["ax"][tg], f["ay"][tg], dx, dy), int(tg.size)
 
 
-def stage_gbs(stages, n, cells):
+def stage_bytes(n, cells):
     """Algorithmic HBM bytes of each memory-bound stage per step (N bodies,
-    C cells: bbox 8N; build 8N + 20C; centre of mass 12N + 12C; sort 4N + 24C
-    + 24N for the bodies moved into tree order; update 40N) over its measured
-    time, in GB/s: what the stage moves at least, not what it fetches."""
-    bytes_ = {"bbox": 8 * n, "build": 8 * n + 20 * cells, "com": 12 * n + 12 * cells,
-              "sort": 4 * n + 24 * cells + 24 * n, "update": 40 * n}
-    # a stage under 1 us ran inside another (the update inside the walk)
-    return {k: b / (stages[k] * 1e-3) / 1e9 for k, b in bytes_.items()
-            if stages.get(k, 0.0) > 1e-3}
+    C cells): bbox 8N (x, y read); build 8N + 20C (x, y read; per cell its
+    four 4-byte child links and a 4-byte first-body index written); centre of
+    mass 12N + 12C (bodies' x, y, m read; a cell's x, y, m written); sort
+    4N + 24C + 24N (the cells' first-body indices and moments read, the six
+    body arrays moved into tree order); update 40N (x, y, vx, vy, ax, ay
+    read, x, y, vx, vy written).  What a stage moves at least, not what it
+    fetches: the walk's second 64-byte block per node (quadrupole moments and
+    opening thresholds) is a term of this layout on top of these."""
+    return {"bbox": 8 * n, "build": 8 * n + 20 * cells, "com": 12 * n + 12 * cells,
+            "sort": 4 * n + 24 * cells + 24 * n, "update": 40 * n}
+
+
+def stage_roofline(stages, n, cells, key):
+    """Per memory-bound stage: algorithmic bytes over the measured time
+    against HBM peak, and, for a workload with committed counter runs, the
+    counter bytes and their ratio to the algorithmic bytes."""
+    traffic = STAGE_TRAFFIC_BYTES.get(key, {})
+    out = {}
+    for k, b in stage_bytes(n, cells).items():
+        ms = stages.get(k, 0.0)
+        if ms <= 1e-3:  # a stage under 1 us ran inside another
+            continue
+        gbs = b / (ms * 1e-3) / 1e9
+        t = traffic.get(k)
+        out[k] = {"bound": "hbm", "ms": ms, "algorithmic_bytes": b, "achieved": gbs,
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                  "traffic": t, "traffic_over_algorithmic": t / b if t else None}
+    return out
 
 
 def synced_diagnostics(eng, dt, forces):
@@ -455,6 +487,9 @@ def main():
     walk_ms = stages["forces"]
     achieved = (inter * OPS_PER_INTERACTION + qinter * OPS_PER_QUAD_TERM) / (walk_ms * 1e-3) / 1e12
     peak = VALU_PEAK_LANE_OPS / 1e12
+    model_key = (args.workload, n, args.theta, args.group, args.multipole, args.opening)
+    if args.precision != "fp32" or args.graph:
+        model_key = None  # counter runs exist for the default FP32 direct launches only
     out = {
         "metric": "BH body-steps/s at N=1M, theta=0.5 + rms rel force err vs FP64 direct sum",
         "value": n * args.steps / wall,
@@ -481,7 +516,7 @@ def main():
         "stage_ms_source": ("HIP events: forces over the timed steps, the other stages over the "
                             "warm-up steps after the first" if not args.graph else
                             "HIP events over two direct steps before the graph replays"),
-        "stage_algorithmic_gbs": stage_gbs(stages, n, ncells),
+        "stage_roofline": stage_roofline(stages, n, ncells, model_key),
         "cells": ncells,
         "interactions_per_step": inter,
         "interactions_per_body": inter / n,
@@ -492,8 +527,8 @@ def main():
             "peak": peak,
             "unit": "Tlane-op/s",
             "frac": achieved / peak,
-            "traffic": WALK_TRAFFIC_BYTES.get(
-                (args.workload, n, args.theta, args.group, args.multipole, args.opening)),
+            "traffic": STAGE_TRAFFIC_BYTES.get(model_key, {}).get("forces"),
+            "valu_busy_counters": WALK_VALU_BUSY.get(model_key),
             "interactions_per_s": inter / (walk_ms * 1e-3),
             "ops_per_interaction": OPS_PER_INTERACTION,
             "ops_per_quadrupole_term": OPS_PER_QUAD_TERM,

@@ -500,6 +500,9 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
   if (s == A.n - 1) {
     if (cs > A.cap) atomicOr(&A.dev->err, kErrPool);
     A.dev->ncells = (int32_t)(cs < A.cap ? cs : A.cap);
+    // the centre-of-mass passes count their leftovers from zero
+#pragma unroll
+    for (int k = 0; k < 4; ++k) A.dev->com_len[k] = 0;
   }
   // block of the node created at s at level L (-1: beyond the pool)
   auto created = [&](int L) -> int64_t {

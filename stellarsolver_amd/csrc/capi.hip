@@ -83,8 +83,8 @@ void free_all(stellar_ctx* c) {
                   B.perm_alt, B.key2, B.ncell,  B.coff,   B.tmp,    B.blk, B.bsum,
                   B.cmeta, B.lam,
                   B.csum,  B.out,    B.dev,
-                  B.bbox_part, B.scount, B.choff,
-                  B.lmin,  B.qblk,  B.spine,   B.clist, B.clen,
+                  B.bbox_part, B.scount,
+                  B.lmin,  B.qblk,  B.spine,
                   B.key3,  B.ioff,  B.inst_of, B.ibbox, B.tstart, B.itile,
                   B.gtab[0], B.gtab[1], B.gtab[2], B.ss_spl, B.ss_tot, B.ss_bstart};
   for (void* p : ptrs)
@@ -364,12 +364,9 @@ int create_ctx(const stellar_params* p, int32_t ninst, const int32_t* counts,
   chk(dmalloc(&B.cmeta, NB));
   chk(dmalloc(&B.csum, kSumStride * NB));
   chk(dmalloc(&B.qblk, 4 * NB));
-  const size_t clcap = (size_t)com_list_capacity(c->cap);
-  chk(dmalloc(&B.spine, clcap));
-  chk(dmalloc(&B.scount, clcap / kMaxLevel + 64));
-  chk(dmalloc(&B.choff, clcap / kMaxLevel + 64));
-  chk(dmalloc(&B.clist, 2 * clcap));
-  chk(dmalloc(&B.clen, 64));
+  const ComSlots cs = com_slots(c->cap);
+  chk(dmalloc(&B.spine, (size_t)cs.slots));
+  chk(dmalloc(&B.scount, (size_t)cs.counts));
   chk(dmalloc(&B.out, 2 * (N + C)));
   chk(dmalloc(&B.dev, 1));
   chk(dmalloc(&B.bbox_part, (size_t)c->ntiles));

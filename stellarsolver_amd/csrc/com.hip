@@ -31,8 +31,18 @@ namespace stellar {
 
 namespace {
 
-constexpr int kChunk = 256;          // cells per workgroup and chunk
+constexpr int kChunk = 256;          // cells per workgroup and chunk (at most)
 constexpr int kComThreads = 256;
+// Chunking.  A pass cuts its list into chunks of at most kChunk entries.
+// Chunk w starts near position kStride w: at the first entry of the lowest
+// level among positions [kStride w, kStride w + kWin), so that few of its
+// cells are ancestors of the entry after it (the chunk's leftovers: at most
+// one per level above that entry's).  Neighbouring chunks choose their
+// shared boundary from the same positions.
+constexpr int kStride = 248;
+constexpr int kWin = 8;
+static_assert(kStride + kWin == 256, "a chunk and its next boundary window fit one window");
+static_assert(kWin <= 32, "one half-wavefront per boundary window");
 constexpr int kSlots = kMaxLevel;    // leftover cells per chunk (one per level)
 constexpr int kMom = kSumStride;     // FP64 moments kept in global memory
 // In LDS a finished cell also keeps its FP64 centre of mass (the quotient its
@@ -77,7 +87,10 @@ struct ChunkLds {
   uint8_t local[kChunk];        // every unfinished descendant is in the chunk
   int16_t cidx[kChunk][4];      // chunk index of each cell child, -1 if none
   ChildRec rec[kChunk][4];
-  double mom[kChunk][kMomL];    // moments of the entry once finished
+  union {
+    double mom[kChunk][kMomL];  // moments of the entry once finished
+    uint64_t wmeta[kChunk];     // metadata of the window's entries (staging)
+  };
   // moments of cell children finished before this chunk (EXT only)
   double ext[EXT ? kChunk : 1][4][kMomL];
   int32_t wcnt[kComThreads / 64];
@@ -87,6 +100,7 @@ struct ChunkLds {
   int32_t lvoff[kMaxLevel + 1];  // their offset in bylev
   uint8_t lvlist[kMaxLevel + 1]; // levels present, deepest first
   int32_t nlev;
+  int32_t blo, bhi;              // the chunk within its window
 };
 static_assert(kChunk == kComThreads, "one chunk entry per thread");
 static_assert(kChunk <= 32767, "chunk indices are int16");
@@ -106,33 +120,47 @@ __device__ __forceinline__ int find_id(const int32_t* ids, int lo, int hi, int32
 }
 
 // Moments of entry i from its child slots (slot order, FP64) into L.mom[i]
-// (publish_cell writes them out).  Every slot is read the same way,
-// without branches: a cell child's moments come from L.mom (finished in this
-// chunk) or L.ext (before it), a body's are (m, m x, m y, 0, 0, 0, 0, x, y)
-// from its record, and an empty slot, whose record is all zeros, gives
-// zeros, which leave every sum unchanged (no sum here is ever -0).
+// (publish_cell writes them out): a cell child's moments come from L.mom
+// (finished in this chunk) or L.ext (before it), a body's are
+// (m, m x, m y, 0, 0, 0, 0, x, y) from its record, and an empty slot, whose
+// record is all zeros, gives zeros, which leave every sum unchanged (no sum
+// here is ever -0).
 template <bool EXT>
-__device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
-  const int32_t b = L.ids[i];
+__device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i, int roff) {
   double cm[4], cmx[4], cmy[4], cqxx[4], cqxy[4], cqyy[4], cbm[4], ccx[4], ccy[4];
   bool used[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const ChildRec r = L.rec[i][k];
-    const int ci = L.cidx[i][k];
-    const double* src = (!EXT || ci >= 0) ? L.mom[ci < 0 ? 0 : ci] : L.ext[EXT ? i : 0][k];
+    const ChildRec r = L.rec[roff + i][k];
     const bool cell = r.code > 0;
     used[k] = r.code != 0;
-    const double m = (double)r.m, x = (double)r.x, y = (double)r.y;
-    cm[k] = cell ? src[0] : m;
-    cmx[k] = cell ? src[1] : m * x;
-    cmy[k] = cell ? src[2] : m * y;
-    cqxx[k] = cell ? src[3] : 0.0;
-    cqxy[k] = cell ? src[4] : 0.0;
-    cqyy[k] = cell ? src[5] : 0.0;
-    cbm[k] = cell ? src[6] : 0.0;
-    ccx[k] = cell ? src[7] : x;
-    ccy[k] = cell ? src[8] : y;
+    if (cell) {
+      // a cell child's moments: finished in this chunk (L.mom) or before it
+      // (L.ext)
+      const int ci = L.cidx[i][k];
+      const double* src = (!EXT || ci >= 0) ? L.mom[ci < 0 ? 0 : ci] : L.ext[EXT ? i : 0][k];
+      cm[k] = src[0];
+      cmx[k] = src[1];
+      cmy[k] = src[2];
+      cqxx[k] = src[3];
+      cqxy[k] = src[4];
+      cqyy[k] = src[5];
+      cbm[k] = src[6];
+      ccx[k] = src[7];
+      ccy[k] = src[8];
+    } else {
+      // a body (an empty slot's record is all zeros and gives zeros)
+      const double m = (double)r.m, x = (double)r.x, y = (double)r.y;
+      cm[k] = m;
+      cmx[k] = m * x;
+      cmy[k] = m * y;
+      cqxx[k] = 0.0;
+      cqxy[k] = 0.0;
+      cqyy[k] = 0.0;
+      cbm[k] = 0.0;
+      ccx[k] = x;
+      ccy[k] = y;
+    }
   }
   double sm = 0.0, sx = 0.0, sy = 0.0;
 #pragma unroll
@@ -185,19 +213,17 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i) {
 
 // Writes finished entry i out, by the thread that staged it (block b, child
 // slot s of parent block p): its child record {CoM, m, b} in the parent's
-// block, {Qxx, Qxy, Qyy} beside it (with the bmax threshold in bmax mode;
-// otherwise the level threshold the build put there stays), and its FP64 moments
+// block, {Qxx, Qxy, Qyy, opening threshold} beside it, and its FP64 moments
 // when a parent finishing in a later pass or chunk will read them (a parent
 // finished in this chunk took the LDS copy).
 template <bool EXT>
 __device__ void publish_cell(const ComArgs& A, const ChunkLds<EXT>& L, int i, int32_t b,
-                             int32_t p, int s) {
+                             int32_t p, int s, float4 bb) {
   const double* o = L.mom[i];
   A.blk[4 * (int64_t)p + s] = ChildRec{(float)o[7], (float)o[8], (float)o[0], b};
   if (A.quad || A.bmax) {
     // second block: the moments the walk needs and the cell's opening
-    // threshold, one 16-byte store
-    const float4 bb = A.ibbox[A.inst_of[meta_start(A.cmeta[b])]];
+    // threshold (bb: the box of the cell's instance), one 16-byte store
     const int lev = L.lev[i];
     const float thr =
         A.bmax ? (float)((o[6] * o[6]) / A.t2 + A.eps2)
@@ -226,40 +252,141 @@ __device__ void publish_cell(const ComArgs& A, const ChunkLds<EXT>& L, int i, in
   }
 }
 
-// One chunk: list positions [p0, p1) of `list` (nullptr: block id = pos + nroot),
-// `next` = block id at p1 (INT32_MAX if none or if every later cell is
-// already finished).  Leftover cells go to out_ids[0..*out_n).
+// Block ids of list positions [W0, W0 + kChunk) below m into win[], from a
+// list held as a pass's per-chunk leftovers (kSlots per chunk, in chunk
+// order; nch chunks): the per-chunk counts are scanned, four per thread per
+// tile, until the window is passed, and each chunk copies the entries that
+// fall in the window.
+__device__ void gather_window(const int32_t* __restrict__ slots,
+                              const int32_t* __restrict__ counts, int32_t nch, int32_t W0,
+                              int32_t* win, int32_t* wsum) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int32_t base = 0;  // entries of the chunks before the tile (uniform)
+  for (int32_t c0 = 0; c0 < nch && base < W0 + kChunk; c0 += 4 * kComThreads) {
+    const int32_t c = c0 + 4 * (int32_t)threadIdx.x;
+    int32_t n[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) n[j] = c + j < nch ? counts[c + j] : 0;
+    const int32_t tot = n[0] + n[1] + n[2] + n[3];
+    int32_t incl = tot;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t o = __shfl_up(incl, off);
+      if (lane >= off) incl += o;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int32_t pos = base + incl - tot, tile = 0;
+    for (int w = 0; w < kComThreads / 64; ++w) {
+      if (w < wv) pos += wsum[w];
+      tile += wsum[w];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int32_t lo = pos > W0 ? pos : W0;
+      const int32_t hi = pos + n[j] < W0 + kChunk ? pos + n[j] : W0 + kChunk;
+      for (int32_t q = lo; q < hi; ++q) win[q - W0] = slots[(int64_t)(c + j) * kSlots + (q - pos)];
+      pos += n[j];
+    }
+    base += tile;
+    __syncthreads();  // wsum is rewritten by the next tile
+  }
+}
+
+// A pass's input: every cell in id order (slots == nullptr), or the
+// leftovers of pass k (len = k, 0-based): held per chunk of that pass, which
+// cut a list of prod entries (the leftovers of pass prod, -1: every cell).
+struct ListIn {
+  const int32_t* slots;
+  const int32_t* counts;
+  int len;
+  int prod;
+};
+
+// The device scalars the passes decide on, read together (one memory round
+// trip at a kernel's start): error bits, cell count, pass list lengths.
+struct ComState {
+  uint32_t err;
+  int32_t ncells;
+  int32_t len[4];
+  // length of list k (-1: every cell); k is uniform, so no indexed access
+  __device__ __forceinline__ int32_t at(int k) const {
+    return k < 0 ? ncells : k == 0 ? len[0] : k == 1 ? len[1] : k == 2 ? len[2] : len[3];
+  }
+  // a later pass is skipped once list a or b (-1: none) fits in one chunk:
+  // the last pass then takes that list (com_final_kernel)
+  __device__ __forceinline__ bool done(int a, int b) const {
+    return (a >= 0 && at(a) <= kChunk) || (b >= 0 && at(b) <= kChunk);
+  }
+};
+static_assert(offsetof(DevScalars, com_len) % 16 == 0, "one 16-byte load");
+
+__device__ __forceinline__ ComState com_state(const DevScalars* d) {
+  const int4 h = *reinterpret_cast<const int4*>(d);  // err, pad, ncells, pad
+  const int4 l = *reinterpret_cast<const int4*>(d->com_len);
+  return ComState{(uint32_t)h.x, h.z, {l.x, l.y, l.z, l.w}};
+}
+
+// Block ids of window positions [W0, W0 + kChunk) below m into L.pidx (used
+// as scratch until the chunk is staged), their metadata into L.wmeta, their
+// child records into L.rec and their levels into L.local (255 past the end).
+template <bool EXT>
+__device__ void load_window(const ComArgs& A, ChunkLds<EXT>& L, const ListIn& in, int32_t m,
+                            int32_t pm, int32_t W0) {
+  const int t = threadIdx.x;
+  // the first pass (EXT false) always runs over every cell
+  if (EXT && in.slots) {
+    gather_window(in.slots, in.counts, (pm + kStride - 1) / kStride, W0, L.pidx, L.wcnt);
+    __syncthreads();
+  }
+  const bool valid = W0 + t < m;
+  const int32_t b = !valid ? 0 : (EXT && in.slots) ? L.pidx[t] : W0 + t + A.nroot;
+  const uint64_t meta = valid ? A.cmeta[b] : 0;
+  if (valid) {
+    const ChildRec* r = A.blk + 4 * (int64_t)b;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) L.rec[t][k] = r[k];
+  }
+  L.pidx[t] = b;
+  L.wmeta[t] = meta;
+  L.local[t] = valid ? (uint8_t)meta_level(meta) : (uint8_t)255;
+  __syncthreads();
+}
+
+// One chunk, staged: thread i < ne holds entry i (block b, metadata meta),
+// whose child records are L.rec[roff + i]; `contiguous` when the entries are consecutive block ids (the first pass);
+// next_lev = level of the list entry after the chunk (-1 if none, or if
+// every later cell is already finished).  Leftover cells go to
+// out_ids[0..*out_n), and their number is added to *out_len.
 //
 // Every list is closed under taking parents (a cell is unfinished while any
 // descendant is), and in such a subsequence of pre-order the first entry
 // after a cell's descendants has a level no deeper than the cell's.  So the
 // unfinished descendants of entry i all lie in the chunk exactly when some
-// later entry of the chunk, or `next`, has level <= level(i).
+// later entry of the chunk, or the next entry, has level <= level(i).
 //
 // Everything the finishing steps need is staged in LDS first, so they issue
 // global stores only and never wait on global memory.
 template <bool EXT>
-__device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t* list,
-                              int32_t p0, int32_t p1, int32_t next,
-                              int32_t* out_ids, int32_t* out_n) {
-  const int ne = p1 - p0;
+__device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, int ne, int roff,
+                              bool contiguous, int32_t b, uint64_t meta, int next_lev,
+                              int32_t* out_ids, int32_t* out_n, int32_t* out_len) {
   const int i = threadIdx.x;  // one entry per thread (kChunk == blockDim)
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int32_t pb = 0;  // parent block and slot of entry i
   int sl = 0;
+  int rank = -1;   // rank among the leftovers, -1 if finished here
+  float4 bb = make_float4(0.f, 0.f, 0.f, 0.f);
   {
     int lev = 255;
     if (i < ne) {
-      const int32_t b = list ? list[p0 + i] : p0 + i + A.nroot;
       L.ids[i] = b;
-      const uint64_t meta = A.cmeta[b];
       lev = meta_level(meta);
       L.lev[i] = (uint8_t)lev;
       pb = meta_parent(meta);
       sl = meta_slot(meta);
-      const ChildRec* r = A.blk + 4 * (int64_t)b;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) L.rec[i][k] = r[k];
+      // the instance's box, for the cell's threshold (loaded here, so that
+      // the first barrier covers its latency)
+      if (A.quad || A.bmax) bb = A.ibbox[A.nroot == 1 ? 0 : A.inst_of[meta_start(meta)]];
     }
     // suffix minimum of the levels: within the wavefront, then across
     int suf = lev;
@@ -272,28 +399,26 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
     if (lane == 63) after = 255;
     __syncthreads();
     for (int w = wv + 1; w < kComThreads / 64; ++w) after = min(after, L.wmin[w]);
-    const int nlev = next == INT32_MAX ? -1 : meta_level(A.cmeta[next]);
-    after = min(after, nlev);
+    after = min(after, next_lev);
     if (i < ne) L.local[i] = after <= lev ? 1 : 0;
   }
   __syncthreads();
   {
     const bool live = i < ne;
-    const int32_t b = live ? L.ids[i] : 0;
     bool left = false;
     if (live) {
-      // a contiguous chunk (the first pass) maps block ids to indices directly
-      const int32_t base = p0 + A.nroot;
+      // consecutive block ids map to chunk indices directly
+      const int32_t base = L.ids[0];
       const int32_t pr = pb - base;
-      L.pidx[i] = list ? find_id(L.ids, 0, i, pb) : (pr >= 0 && pr < i ? pr : -1);
+      L.pidx[i] = contiguous ? (pr >= 0 && pr < i ? pr : -1) : find_id(L.ids, 0, i, pb);
       if (L.local[i]) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const int32_t code = L.rec[i][k].code;
+          const int32_t code = L.rec[roff + i][k].code;
           int ci = -1;
           if (code > 0) {
             const int32_t cr = code - base;
-            ci = list ? find_id(L.ids, i + 1, ne, code) : (cr > i && cr < ne ? cr : -1);
+            ci = contiguous ? (cr > i && cr < ne ? cr : -1) : find_id(L.ids, i + 1, ne, code);
             if (ci < 0 && EXT) {
               // finished by an earlier pass or a later chunk
               const double* g = A.csum + kSumStride * (int64_t)code;
@@ -313,12 +438,14 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
       }
     }
     // leftovers keep ascending id order: rank = leftovers before entry i
+    // (written out at the end: a global store before a barrier would make
+    // every wavefront wait for it)
     const uint64_t bal = __builtin_amdgcn_ballot_w64(left);
     if (lane == 0) L.wcnt[wv] = __popcll(bal);
     __syncthreads();
-    int rank = __popcll(bal & ((1ull << lane) - 1ull));
+    rank = __popcll(bal & ((1ull << lane) - 1ull));
     for (int w = 0; w < wv; ++w) rank += L.wcnt[w];
-    if (left && rank < kSlots) out_ids[rank] = b;
+    rank = left ? rank : -1;
     if (threadIdx.x == 0) {
       int tot = 0;
       for (int w = 0; w < kComThreads / 64; ++w) tot += L.wcnt[w];
@@ -356,115 +483,117 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, const int32_t*
   for (int k = 0; k < nl; ++k) {
     const int l = L.lvlist[k];
     const int cnt = L.lvcnt[l], off = L.lvoff[l];
-    for (int t = threadIdx.x; t < cnt; t += kComThreads) finish_cell<EXT>(A, L, L.bylev[off + t]);
+    for (int t = threadIdx.x; t < cnt; t += kComThreads) finish_cell<EXT>(A, L, L.bylev[off + t], roff);
     __syncthreads();
   }
-  if (fin) publish_cell<EXT>(A, L, i, L.ids[i], pb, sl);
-  if (threadIdx.x == 0) *out_n = L.nout < kSlots ? L.nout : kSlots;
+  if (fin) publish_cell<EXT>(A, L, i, L.ids[i], pb, sl, bb);
+  if (rank >= 0 && rank < kSlots) out_ids[rank] = b;
+  if (threadIdx.x == 0) {
+    const int nout = L.nout < kSlots ? L.nout : kSlots;
+    *out_n = nout;
+    if (out_len && nout > 0) atomicAdd(out_len, nout);
+  }
 }
 
-// A later pass (and its compaction) is skipped once an earlier list fits in
-// one chunk: the last pass then takes that list (com_final_kernel).
-__device__ __forceinline__ bool list_done(const int32_t* a, const int32_t* b) {
-  return (a && *a <= kChunk) || (b && *b <= kChunk);
+// Pass over a list: chunk w (one workgroup at a time, grid-stride) covers
+// the window of positions [kStride w, kStride w + kChunk), cut as described
+// at kStride.  Its leftovers go to out_slots[w kSlots ..], their number to
+// out_counts[w] and the pass's total to *out_len.
+template <bool EXT>
+__device__ void pass_chunk(const ComArgs& A, ChunkLds<EXT>& L, const ListIn& in, int32_t m,
+                           int32_t pm, int32_t w, int32_t* out_slots, int32_t* out_counts,
+                           int32_t* out_len) {
+  const int32_t W0 = w * kStride;
+  load_window<EXT>(A, L, in, m, pm, W0);
+  // chunk boundaries: the first position of the lowest level in each
+  // boundary window (half a wavefront each)
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x, j = lane & 31;
+    const int t = lane < 32 ? j : kStride + j;
+    int key = ((j < kWin ? (int)L.local[t] : 255) << 8) | j;
+    for (int off = 1; off < 32; off <<= 1) key = min(key, __shfl_xor(key, off));
+    if (lane == 0) L.blo = w == 0 ? 0 : (key & 31);
+    if (lane == 32) L.bhi = W0 + kStride >= m ? m - W0 : kStride + (key & 31);
+  }
+  __syncthreads();
+  const int blo = L.blo, bhi = L.bhi;
+  const int ne = bhi - blo;
+  const int next_lev = W0 + bhi < m ? (int)L.local[bhi] : -1;
+  const bool live = (int)threadIdx.x < ne;
+  const int32_t b = live ? L.pidx[blo + threadIdx.x] : 0;
+  const uint64_t meta = live ? L.wmeta[blo + threadIdx.x] : 0;
+  __syncthreads();  // the window's LDS is reused by the chunk
+  process_chunk<EXT>(A, L, ne, blo, !EXT, b, meta, next_lev, out_slots + (int64_t)w * kSlots,
+                     out_counts + w, out_len);
 }
 
-// Pass over list positions: one chunk per workgroup.  list == nullptr: all
-// cells (length dev->ncells); else the list compacted by the previous pass.
+// Pass over a list: chunk w covers the window of positions
+// [kStride w, kStride w + kChunk), cut as described at kStride.  Its
+// leftovers go to out_slots[w kSlots ..], their number to out_counts[w] and
+// the pass's total to *out_len.  The first pass (every cell) runs one chunk
+// per workgroup; the list passes, whose LDS allows one workgroup per CU,
+// loop over chunks with a grid of the resident workgroups.
 template <bool EXT>
 __global__ __launch_bounds__(kComThreads) void com_pass_kernel(
-    ComArgs A, const int32_t* __restrict__ list, const int32_t* __restrict__ len,
-    int32_t* __restrict__ out_slots, int32_t* __restrict__ out_counts,
-    const int32_t* __restrict__ skip_a, const int32_t* __restrict__ skip_b) {
+    ComArgs A, ListIn in, int32_t* __restrict__ out_slots, int32_t* __restrict__ out_counts,
+    int32_t* __restrict__ out_len, int skip_a, int skip_b) {
   __shared__ ChunkLds<EXT> L;
-  if (A.dev->err & kBuildErrors) return;  // no complete tree to sum
-  if (list_done(skip_a, skip_b)) return;
-  const int32_t m = list ? *len : A.dev->ncells;
-  const int32_t p0 = blockIdx.x * kChunk;
-  if (p0 >= m) return;
-  const int32_t p1 = p0 + kChunk < m ? p0 + kChunk : m;
-  const int32_t next = p1 < m ? (list ? list[p1] : p1 + A.nroot) : INT32_MAX;
-  process_chunk<EXT>(A, L, list, p0, p1, next, out_slots + (int64_t)blockIdx.x * kSlots,
-                out_counts + blockIdx.x);
-}
-
-// Compaction of a pass's leftovers into a dense list, in two launches:
-// offsets (one workgroup scans the per-chunk counts) and gather (one
-// wavefront per chunk).  Chunks are in ascending id order and each chunk's
-// leftovers were written in ascending order, so the list is sorted.
-__global__ __launch_bounds__(1024) void com_offsets_kernel(
-    const ComArgs A, const int32_t* __restrict__ len_in, int32_t nchunks_max,
-    const int32_t* __restrict__ counts, int32_t* __restrict__ offs,
-    int32_t* __restrict__ len_out, const int32_t* __restrict__ skip_a,
-    const int32_t* __restrict__ skip_b) {
-  __shared__ int32_t part[1024];
-  if (A.dev->err & kBuildErrors) return;
-  if (list_done(skip_a, skip_b)) return;
-  const int32_t m = len_in ? *len_in : A.dev->ncells;
-  int32_t nchunks = (m + kChunk - 1) / kChunk;
-  if (nchunks > nchunks_max) nchunks = nchunks_max;
-  const int32_t per = (nchunks + blockDim.x - 1) / blockDim.x;
-  const int32_t c0 = threadIdx.x * per;
-  const int32_t c1 = c0 + per < nchunks ? c0 + per : nchunks;
-  int32_t tot = 0;
-  for (int32_t c = c0; c < c1; ++c) tot += counts[c];
-  part[threadIdx.x] = tot;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan
-    const int32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
+  const ComState S = com_state(A.dev);
+  if (S.err & kBuildErrors) return;  // no complete tree to sum
+  if (S.done(skip_a, skip_b)) return;
+  const int32_t m = in.slots ? S.at(in.len) : S.ncells;
+  const int32_t pm = S.at(in.prod);
+  const int32_t nch = (m + kStride - 1) / kStride;
+  if (!EXT) {
+    if ((int32_t)blockIdx.x < nch)
+      pass_chunk<EXT>(A, L, in, m, pm, blockIdx.x, out_slots, out_counts, out_len);
+    return;
+  }
+  for (int32_t w = blockIdx.x; w < nch; w += gridDim.x) {
+    pass_chunk<EXT>(A, L, in, m, pm, w, out_slots, out_counts, out_len);
     __syncthreads();
   }
-  int32_t pos = part[threadIdx.x] - tot;
-  for (int32_t c = c0; c < c1; ++c) {
-    offs[c] = pos;
-    pos += counts[c];
-  }
-  if (threadIdx.x == blockDim.x - 1) *len_out = part[threadIdx.x];
 }
 
-__global__ __launch_bounds__(64) void com_gather_kernel(
-    const ComArgs A, const int32_t* __restrict__ len_in,
-    const int32_t* __restrict__ slots, const int32_t* __restrict__ counts,
-    const int32_t* __restrict__ offs, int32_t* __restrict__ list,
-    const int32_t* __restrict__ skip_a, const int32_t* __restrict__ skip_b) {
-  if (A.dev->err & kBuildErrors) return;
-  if (list_done(skip_a, skip_b)) return;
-  const int32_t m = len_in ? *len_in : A.dev->ncells;
-  const int32_t c = blockIdx.x;
-  if (c * kChunk >= m) return;
-  if ((int)threadIdx.x < counts[c])
-    list[offs[c] + threadIdx.x] = slots[(int64_t)c * kSlots + threadIdx.x];
-}
-
-// Last pass: one workgroup finishes the whole list, chunks from last to
-// first (every descendant of a chunk's cells is then in the chunk or done).
-// The list is the first of the pass-1, pass-2 leftovers that fits in one
-// chunk (the passes after it were skipped), else the pass-3 leftovers.
+// Last pass: one workgroup finishes a whole list, in windows of kChunk from
+// the last to the first (every descendant of a window's cells is then in the
+// window or done).  The list is the first of the pass-1, pass-2 leftovers
+// that fits in one chunk (the passes after it were skipped), else the pass-3
+// leftovers.
 __global__ __launch_bounds__(kComThreads) void com_final_kernel(
-    ComArgs A, const int32_t* __restrict__ l0, const int32_t* __restrict__ l1,
-    const int32_t* __restrict__ lens, int32_t* __restrict__ scratch_slots,
+    ComArgs A, ListIn l1, ListIn l2, ListIn l3, int32_t* __restrict__ scratch_slots,
     int32_t* __restrict__ scratch_count) {
   __shared__ ChunkLds<true> L;
-  if (A.dev->err & kBuildErrors) return;
-  const bool short0 = lens[0] <= kChunk, short1 = lens[1] <= kChunk;
-  const int32_t* list = short0 ? l0 : short1 ? l1 : l0;
-  const int32_t m = short0 ? lens[0] : short1 ? lens[1] : lens[2];
-  const int32_t nch = (m + kChunk - 1) / kChunk;
-  for (int32_t c = nch - 1; c >= 0; --c) {
-    const int32_t p0 = c * kChunk;
-    const int32_t p1 = p0 + kChunk < m ? p0 + kChunk : m;
-    process_chunk<true>(A, L, list, p0, p1, INT32_MAX, scratch_slots, scratch_count);
+  const ComState S = com_state(A.dev);
+  if (S.err & kBuildErrors) return;
+  const ListIn& in = S.len[0] <= kChunk ? l1 : S.len[1] <= kChunk ? l2 : l3;
+  const int32_t m = S.at(in.len), pm = S.at(in.prod);
+  const int32_t nw = (m + kChunk - 1) / kChunk;
+  for (int32_t c = nw - 1; c >= 0; --c) {
+    const int32_t W0 = c * kChunk;
+    load_window<true>(A, L, in, m, pm, W0);
+    const int ne = m - W0 < kChunk ? m - W0 : kChunk;
+    const bool live = (int)threadIdx.x < ne;
+    const int32_t b = live ? L.pidx[threadIdx.x] : 0;
+    const uint64_t meta = live ? L.wmeta[threadIdx.x] : 0;
+    __syncthreads();
+    process_chunk<true>(A, L, ne, 0, false, b, meta, -1, scratch_slots, scratch_count, nullptr);
     __syncthreads();
   }
 }
 
 }  // namespace
 
-// Worst-case list lengths: pass 1 leaves at most kSlots cells per chunk.
-int64_t com_list_capacity(int64_t cap) {
-  return ((cap + kChunk - 1) / kChunk) * (int64_t)kSlots + kChunk;
+// Leftover slots of the three list passes (kSlots per chunk) and their
+// per-chunk counts, for a pool of cap cells.
+ComSlots com_slots(int64_t cap) {
+  ComSlots s;
+  s.ch[0] = (cap + kStride - 1) / kStride;
+  s.ch[1] = (s.ch[0] * kSlots + kStride - 1) / kStride;
+  s.ch[2] = (s.ch[1] * kSlots + kStride - 1) / kStride;
+  s.slots = (s.ch[0] + s.ch[1] + s.ch[2]) * kSlots + kSlots;  // + the final pass's scratch
+  s.counts = s.ch[0] + s.ch[1] + s.ch[2] + 1;
+  return s;
 }
 
 void launch_com(stellar_ctx* c) {
@@ -477,35 +606,30 @@ void launch_com(stellar_ctx* c) {
             c->p.precision == STELLAR_PRECISION_MIXED ? 1 : 0, B.inst_of, B.ibbox,
             (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? c->p.theta * c->p.theta : c->p.theta,
             c->p.eps2, (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? 1 : 0};
-  // pass 1 over all cells; each later list holds at most kSlots cells per
-  // chunk of the pass before, so grids are sized from that bound and chunks
-  // beyond the actual length exit at once
-  const int64_t ch1 = (c->cap + kChunk - 1) / kChunk;
-  const int64_t ch2 = (ch1 * kSlots + kChunk - 1) / kChunk;
-  const int64_t ch3 = (ch2 * kSlots + kChunk - 1) / kChunk;
-  int32_t* l0 = B.clist;
-  int32_t* l1 = B.clist + com_list_capacity(c->cap);
+  const ComSlots cs = com_slots(c->cap);
+  int32_t* sl[3] = {B.spine, B.spine + cs.ch[0] * kSlots,
+                    B.spine + (cs.ch[0] + cs.ch[1]) * kSlots};
+  int32_t* ct[3] = {B.scount, B.scount + cs.ch[0], B.scount + cs.ch[0] + cs.ch[1]};
+  // the passes' list lengths (device scalars, zeroed by the build's link kernel)
+  int32_t* len = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(B.dev) +
+                                            offsetof(DevScalars, com_len));
   hipStream_t s = c->stream;
-  // a pass runs only while every list before it is longer than one chunk
-  auto compact = [&](const int32_t* len_in, int64_t ch, int32_t* list, int32_t* len_out,
-                     const int32_t* ska, const int32_t* skb) {
-    com_offsets_kernel<<<1, 1024, 0, s>>>(A, len_in, (int32_t)ch, B.scount, B.choff, len_out,
-                                          ska, skb);
-    com_gather_kernel<<<(unsigned)ch, 64, 0, s>>>(A, len_in, B.spine, B.scount, B.choff, list,
-                                                  ska, skb);
-  };
-  const int32_t* c0 = B.clen + 0;
-  const int32_t* c1 = B.clen + 1;
-  com_pass_kernel<false><<<(unsigned)ch1, kComThreads, 0, s>>>(A, nullptr, nullptr, B.spine,
-                                                               B.scount, nullptr, nullptr);
-  compact(nullptr, ch1, l0, B.clen + 0, nullptr, nullptr);
-  com_pass_kernel<true><<<(unsigned)ch2, kComThreads, 0, s>>>(A, l0, c0, B.spine, B.scount,
-                                                              c0, nullptr);
-  compact(c0, ch2, l1, B.clen + 1, c0, nullptr);
-  com_pass_kernel<true><<<(unsigned)ch3, kComThreads, 0, s>>>(A, l1, c1, B.spine, B.scount,
-                                                              c0, c1);
-  compact(c1, ch3, l0, B.clen + 2, c0, c1);
-  com_final_kernel<<<1, kComThreads, 0, s>>>(A, l0, l1, B.clen, B.spine, B.scount);
+  // grids: pass 1 one workgroup per chunk of the pool; the list passes the
+  // workgroups resident at once, grid-stride over their chunks
+  const int64_t r2 = resident_blocks(reinterpret_cast<const void*>(&com_pass_kernel<true>),
+                                     kComThreads);
+  const unsigned g1 = (unsigned)cs.ch[0];
+  const unsigned g2 = (unsigned)std::min<int64_t>(cs.ch[1], r2);
+  const unsigned g3 = (unsigned)std::min<int64_t>(cs.ch[2], r2);
+  const ListIn all{nullptr, nullptr, -1, -1};
+  const ListIn p1{sl[0], ct[0], 0, -1};
+  const ListIn p2{sl[1], ct[1], 1, 0};
+  const ListIn p3{sl[2], ct[2], 2, 1};
+  com_pass_kernel<false><<<g1, kComThreads, 0, s>>>(A, all, sl[0], ct[0], len + 0, -1, -1);
+  com_pass_kernel<true><<<g2, kComThreads, 0, s>>>(A, p1, sl[1], ct[1], len + 1, 0, -1);
+  com_pass_kernel<true><<<g3, kComThreads, 0, s>>>(A, p2, sl[2], ct[2], len + 2, 0, 1);
+  com_final_kernel<<<1, kComThreads, 0, s>>>(A, p1, p2, p3, sl[2] + cs.ch[2] * kSlots,
+                                             ct[2] + cs.ch[2]);
 }
 
 }  // namespace stellar

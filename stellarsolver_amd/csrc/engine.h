@@ -151,7 +151,8 @@ struct DevScalars {
   unsigned long long interactions;       // body-node interactions of the last walk
   unsigned long long quad_interactions;  // of which with a cell's quadrupole term
   uint32_t wq[8];  // walk work queues: next unit of each XCD's range (zeroed before each walk)
-  uint32_t pad[44];
+  int32_t com_len[4];  // centre-of-mass pass leftovers (zeroed by each build)
+  uint32_t pad[40];
 };
 static_assert(sizeof(DevScalars) == 256, "one 256-byte line");
 
@@ -195,9 +196,6 @@ struct Buffers {
   float4* qblk = nullptr;     // per child record: Qxx, Qxy, Qyy, 0
   int32_t* spine = nullptr;   // CoM pass leftovers, kMaxLevel slots per chunk
   int32_t* scount = nullptr;  // leftovers per chunk
-  int32_t* choff = nullptr;   // their offsets in the compacted list
-  int32_t* clist = nullptr;   // two compacted CoM lists
-  int32_t* clen = nullptr;    // their lengths
   float* out = nullptr;       // 2*(n+cap) interleaved output buffer
   float4* bbox_part = nullptr;  // per-tile bounding-box partials
   // instances: a context advances ninst independent simulations whose
@@ -294,6 +292,15 @@ void launch_direct_fp64(stellar_ctx* c, int32_t nt, const int32_t* dtargets,
 void launch_energy_fp64(stellar_ctx* c, double eps2, double half_dt, bool sync,
                         double* dpart);
 int energy_parts(stellar_ctx* c);
-int64_t com_list_capacity(int64_t cap);
+// Leftover slots and per-chunk counts of the centre-of-mass passes for a
+// pool of cap cells (chunks of each of the three list passes in ch).
+struct ComSlots {
+  int64_t ch[3];
+  int64_t slots;
+  int64_t counts;
+};
+ComSlots com_slots(int64_t cap);
+// Workgroups of a kernel that are resident at once on the whole device.
+int resident_blocks(const void* kernel, int threads);
 
 }  // namespace stellar

@@ -590,6 +590,8 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
   }
 }
 
+}  // namespace
+
 // Workgroups of a kernel that are resident at once on the whole device.
 int resident_blocks(const void* kernel, int threads) {
   static std::mutex mu;
@@ -609,6 +611,8 @@ int resident_blocks(const void* kernel, int threads) {
   cache[kernel] = nb;
   return nb;
 }
+
+namespace {
 
 template <bool QUAD, bool THR, int KPOP, int WAVES, bool CLAIM_WAVE, bool MIXED>
 void launch64_queue(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
