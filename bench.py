@@ -98,6 +98,11 @@ def parse():
                     help="only copy 1 GiB device-to-device three times (a plain copy of known "
                          "size that scales rocprofv3's FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--graph", action="store_true", help="replay captured two-step graphs")
+    ap.add_argument("--accuracy-sweep", type=str, default="",
+                    help="comma-separated thetas: after --warmup steps, one JSON line per theta "
+                         "with the force error against the FP64 direct sum (rms, p50, p99, max, "
+                         "rms by decile of |a| and of radius from the median position) and the "
+                         "walk's time, all on the same positions (no step line)")
     ap.add_argument("--precision", choices=("fp32", "mixed"), default="fp32",
                     help="force path: FP32 (default) or mixed (accepted cells' terms in packed "
                          "FP16 with Kahan-compensated FP32 sums; needs --multipole quad)")
@@ -383,6 +388,67 @@ def run_ensemble(args, S):
     return 0
 
 
+def decile_rms(q, key):
+    """rms of the squared relative errors q within each decile of key."""
+    edges = np.percentile(key, np.linspace(0, 100, 11))
+    b = np.clip(np.searchsorted(edges, key, side="right") - 1, 0, 9)
+    return [float(np.sqrt(q[b == k].mean())) if np.any(b == k) else None for k in range(10)]
+
+
+def accuracy_sweep(args, S):
+    """Where the force error sits and what a smaller theta buys: the state
+    after --warmup steps, the FP64 direct sum on a seeded subsample, then
+    for each theta one tree and walk (the configured multipoles and opening
+    test) with the per-body relative errors binned by |a| and by radius,
+    and the walk's time per theta."""
+    n = args.n
+    eps2 = float(np.float32(args.eps * args.eps))
+    st = S.generate_ic(S.ic_params(args.workload), n)
+    p = S.default_params(theta=args.theta, eps2=eps2, timestep=args.dt, group=args.group,
+                         precision=precision_of(S, args), flags=model_flags(S, args))
+    eng = S.Engine(n, p)
+    eng.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
+    eng.run(args.warmup)
+    eng.sync()
+    rng = np.random.default_rng(2)
+    tg = np.sort(rng.choice(n, min(n, args.error_sample), replace=False)).astype(np.int32)
+    rx, ry = eng.direct_fp64(tg)
+    xy = eng.get_output_buffer()
+    x, y = xy[0::2][tg].astype(np.float64), xy[1::2][tg].astype(np.float64)
+    radius = np.hypot(x - np.median(x), y - np.median(y))
+    aref = np.hypot(rx, ry)
+    thetas = [float(t) for t in args.accuracy_sweep.split(",")]
+    rows = []
+    for theta in thetas:
+        eng.set_params(theta=theta, flags=model_flags(S, args))
+        eng.bbox(); eng.build(); eng.com(); eng.sort(); eng.forces()
+        f = eng.export_forces()
+        q = ((f["ax"][tg] - rx) ** 2 + (f["ay"][tg] - ry) ** 2) / (rx ** 2 + ry ** 2)
+        dabs = (f["ax"][tg] - rx) ** 2 + (f["ay"][tg] - ry) ** 2
+        rows.append({
+            "workload": WORKLOAD_NAMES.get(args.workload, f"workload {args.workload}"),
+            "n_bodies": n, "eps": args.eps, "steps_before": args.warmup, "theta": theta,
+            "multipoles": multipoles(args), "opening": opening_text(args),
+            "precision": args.precision, "sample": int(tg.size),
+            "rms_rel_force_err": float(np.sqrt(q.mean())),
+            "p50": float(np.sqrt(np.median(q))), "p99": float(np.sqrt(np.percentile(q, 99))),
+            "max": float(np.sqrt(q.max())),
+            "rms_abs_err_over_rms_a": float(np.sqrt(dabs.mean() / (aref ** 2).mean())),
+            "rms_by_abs_a_decile": decile_rms(q, aref),
+            "rms_by_radius_decile": decile_rms(q, radius),
+            "interactions_per_body": eng.last_interactions() / n})
+    # walk time per theta (HIP events around the walk over three steps; the
+    # steps move the bodies, so this comes after every error above)
+    for theta, row in zip(thetas, rows):
+        eng.set_params(theta=theta, flags=model_flags(S, args) | S.FLAG_FORCES_TIMING)
+        eng.run(3)
+        eng.sync()
+        row["walk_ms"] = eng.stage_times()["forces"]
+        print(json.dumps(row))
+    eng.close()
+    return 0
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -405,6 +471,8 @@ def main():
     S.load_library()
     if args.ensemble:
         return run_ensemble(args, S)
+    if args.accuracy_sweep:
+        return accuracy_sweep(args, S)
     n = args.n
     eps2 = float(np.float32(args.eps * args.eps))
     st = S.generate_ic(S.ic_params(args.workload), n)
