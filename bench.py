@@ -54,15 +54,20 @@ DEFAULT_KEY = (2, 1 << 20, 0.5, 64, "quad", "textbook")
 # device copy; "forces" is the walk launch.
 STAGE_TRAFFIC_BYTES = {DEFAULT_KEY: {
     "bbox": (4125 * 2 + 8) * 1024,
-    "build": (64613 * 2 + 186595) * 1024,
-    "com": (36474 * 2 + 40367) * 1024,
-    "sort": (21201 * 2 + 12288) * 1024,
-    "forces": (180171 * 2 + 13312) * 1024,
+    "build": (64555 * 2 + 186633) * 1024,
+    "com": (32047 * 2 + 39591) * 1024,
+    "sort": (21178 * 2 + 12288) * 1024,
+    "forces": (181412 * 2 + 13312) * 1024,
     "update": (12294 * 2 + 16384) * 1024,
 }}
 # The walk's VALU busy fraction: SQ_ACTIVE_INST_VALU / (256 CUs x GRBM_GUI_ACTIVE
-# per XCD) per dispatch, 432.1e6 / (256 x 18.12e6 / 8).
-WALK_VALU_BUSY = {DEFAULT_KEY: 432.1e6 / (256 * 18.12e6 / 8)}
+# per XCD) per dispatch, 432.1e6 / (256 x 18.16e6 / 8).
+WALK_VALU_BUSY = {DEFAULT_KEY: 432.1e6 / (256 * 18.16e6 / 8)}
+
+
+def progress(msg: str) -> None:
+    """One progress line on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -134,6 +139,7 @@ def cpu_baseline(st, args, eps2):
         t1 = time.perf_counter()
         if k > 0:
             times.append(t1 - t0)
+        progress(f"cpu baseline step {k} ({'timed' if k else 'warm-up'}): {(t1 - t0) * 1e3:.0f} ms")
     med = statistics.median(times)
     n = st["x"].size
     return {
@@ -305,6 +311,7 @@ def run_ensemble(args, S):
     eng = S.Engine(params=p, counts=counts)
     eng.upload(cat["mass"], cat["x"], cat["y"], cat["vx"], cat["vy"])
     tuning = eng.autotune(group_mask=0, reps=2)
+    progress(f"ensemble of {len(counts)} instances ({ntot} bodies) tuned: {tuning}")
     eng.run(2)  # eager step timing for the stage split
     stages = eng.stage_times()
     inter = eng.last_interactions()
@@ -317,6 +324,7 @@ def run_ensemble(args, S):
     t1 = time.perf_counter()
     wall = t1 - t0
     batched = ntot * args.steps / wall
+    progress(f"batched: {wall / args.steps * 1e3:.3f} ms/step")
     # accuracy on a sample of instances
     eng.bbox(); eng.build(); eng.com(); eng.sort(); eng.forces()
     f = eng.export_forces()
@@ -346,6 +354,7 @@ def run_ensemble(args, S):
         e.run(plain_steps)
     t1 = time.perf_counter()
     plain = ntot * plain_steps / (t1 - t0)
+    progress(f"plain form: {(t1 - t0) / plain_steps * 1e3:.3f} ms/step")
     for e in engines:
         e.close()
 
@@ -444,7 +453,7 @@ def accuracy_sweep(args, S):
         eng.run(3)
         eng.sync()
         row["walk_ms"] = eng.stage_times()["forces"]
-        print(json.dumps(row))
+        print(json.dumps(row), flush=True)
     eng.close()
     return 0
 
@@ -483,6 +492,7 @@ def main():
                          | (S.FLAG_GRAPH if args.graph else 0))
     eng = S.Engine(n, p)
     eng.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
+    progress(f"N={n} uploaded")
     e0 = None if args.no_energy else synced_diagnostics(eng, args.dt, forces=True)
 
     if args.graph:
@@ -504,9 +514,11 @@ def main():
         # ~35 us a step)
         stages = eng.stage_times() if args.warmup > 0 else {}
         eng.set_params(flags=(p.flags & ~S.FLAG_STAGE_TIMING) | S.FLAG_FORCES_TIMING)
+    progress("warm-up done")
     t0 = time.perf_counter()
     eng.run(args.steps)  # synchronises once at the end
     t1 = time.perf_counter()
+    progress(f"{args.steps} timed steps: {(t1 - t0) / args.steps * 1e3:.3f} ms/step")
     if not args.graph:
         # mean device ms per timed step, HIP events on the engine stream
         stages = dict(stages, forces=eng.stage_times()["forces"])
@@ -519,6 +531,7 @@ def main():
 
     # accuracy of the walk on the final state: one more untimed force pass
     (err_rms, err_p99, err_max), nsample = force_error(eng, n, args.error_sample)
+    progress(f"force error {err_rms:.2e}")
     drift = None
     if e0 is not None:
         # the force pass of the error check left forces at the final positions
@@ -609,6 +622,7 @@ def main():
     }
     if not args.no_reference_opening and args.opening != "reference":
         out["reference_opening"] = reference_opening_run(S, args, st, eps2)
+        progress("reference-opening side run done")
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(st, args, eps2)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
