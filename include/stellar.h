@@ -45,13 +45,15 @@ extern "C" {
  * level-64 cell by rank instead of splitting forever. */
 #define STELLAR_MAX_LEVEL 64
 
-/* Force-path precision.  Only FP32 is implemented: the walk issues one
- * wavefront-wide instruction per (group, node) pair, so packed FP16 would
- * need FP32 differences, conversions and FP32 accumulation around a
- * non-packed rsqrt and saves no issue slots; MIXED is rejected with
- * STELLAR_ERR_ARG.  Moments are summed in FP64 in every mode. */
+/* Force-path precision.  FP32: every term in FP32.  MIXED: an accepted
+ * cell's monopole + quadrupole term is evaluated as a dimensionless bracket
+ * in packed FP16 (two cells per v_pk_fma_f16, moments Q/(m ws^2) stored in
+ * FP16 by the centre-of-mass pass) and added to Kahan-compensated FP32 sums;
+ * bodies (the near field), the acceptance vote and the update stay FP32.
+ * MIXED needs STELLAR_FLAG_QUADRUPOLE and group 64 (else STELLAR_ERR_ARG).
+ * Moments are summed in FP64 in every mode. */
 #define STELLAR_PRECISION_FP32 0  /* FP32 walk and update                    */
-#define STELLAR_PRECISION_MIXED 1 /* reserved: packed-FP16 far field         */
+#define STELLAR_PRECISION_MIXED 1 /* packed-FP16 far field, FP32 near field  */
 
 typedef struct stellar_params {
   /* Opening test: a cell of x-width w is accepted when every body of the

@@ -7,14 +7,26 @@
 // more bodies share their first L slot codes, and a body's slot codes depend
 // only on its own position and the bounding box (the FP32 midpoints of
 // kernels.cu:144-178, recomputed along its own path).  This file builds that
-// same tree without locks:
+// same tree without locks, one launch per step of
 //
-//   1. keys   : each body's 32 first slot codes -> one 64-bit key
-//   2. sort   : LSD radix sort of (key, storage index)
-//   3. ties   : runs of equal keys are ordered by levels 33..64 and body id
-//   4. lambda : common-prefix depth of neighbours -> cells opened per pair
-//   5. scan   : exclusive scan -> every cell gets its DFS pre-order id
-//   6. link   : each cell finds its body range and its four children
+//   1. keys   : each body's first slot codes -> one 64-bit key (instance
+//               bits on top); between steps the same kernel counts the keys
+//               into the sample sort's buckets (sort.hip);
+//   2. sort   : sample sort of (key, storage index) seeded with the previous
+//               sort's quantiles (sort.hip); rocPRIM's radix sort above 2^24
+//               bodies;
+//   3. ties   : runs of equal keys are ordered by the deeper levels and body
+//               id; the kernel also reads off the next sort's splitters;
+//   4. lambda : common-prefix depth of neighbours -> cells opened per pair,
+//               their count per 256 positions, and the first level of a
+//               minimum tree over the depths; clears the tree blocks;
+//   5. mintree: the rest of the minimum tree and the scan of the per-block
+//               counts (every cell's DFS pre-order id is then a block total
+//               plus a block-local prefix);
+//   6. link   : each position creates the cells it opens, links them and its
+//               body into their parents (a previous-smaller search over the
+//               minimum tree), writes the cells' metadata and the bodies'
+//               x, y, m in tree order.
 //
 // The result is directly the canonical form (cells in DFS pre-order, bodies
 // in sort_kernel order), so it is compared bit for bit with the reference
@@ -647,8 +659,6 @@ void launch_build(stellar_ctx* c) {
       nclear);
   mintree_kernel<<<1, 1024, 0, c->stream>>>(mt, B.bsum, nblk, reinterpret_cast<float4*>(B.blk),
                                             nclear, max_blocks, c->ninst);
-  const float theta = c->p.theta;
-  const bool textbook = (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) != 0;
   LinkArgs A{n,         c->cap,   K,       B.coff,    B.bsum,   B.ncell,  B.lam,
              B.perm,    B.x,      B.y,     B.m,       B.blk,    B.cmeta,
              B.dev,     B.inst_of, B.ioff,
