@@ -57,23 +57,24 @@ __device__ __forceinline__ float walk_level_threshold(float depth0, int level, f
 // One level of box subdivision along one axis (kernels.cu:144-157): the
 // FP32 sum lo+hi is halved in double, p < mid is decided in double and the
 // new bound is mid rounded to FP32.  When |lo+hi| >= 2^-125 the halving is
-// exact in FP32, so the FP32 path gives the identical decision and bound.
+// exact in FP32, so the FP32 path gives the identical decision and bound;
+// the double path runs only for the lanes that need it, behind a
+// wavefront-uniform test (a scalar branch, no exec-mask bookkeeping on the
+// common path).
 __device__ __forceinline__ uint32_t split1(float p, float& lo, float& hi) {
   const float s = lo + hi;
-  bool lt;
-  float mid;
-  if (fabsf(s) >= 0x1p-125f) {
-    mid = s * 0.5f;
-    lt = p < mid;
-  } else {
-    const double md = 0.5 * (double)s;
-    lt = (double)p < md;
-    mid = (float)md;
+  float mid = s * 0.5f;
+  bool lt = p < mid;
+  const bool exact = fabsf(s) >= 0x1p-125f;  // false for NaN too
+  if (__builtin_expect(__builtin_amdgcn_ballot_w64(!exact) != 0, 0)) {
+    if (!exact) {
+      const double md = 0.5 * (double)s;
+      lt = (double)p < md;
+      mid = (float)md;
+    }
   }
-  if (lt)
-    hi = mid;
-  else
-    lo = mid;
+  hi = lt ? mid : hi;
+  lo = lt ? lo : mid;
   return lt ? 1u : 0u;
 }
 

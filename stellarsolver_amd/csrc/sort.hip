@@ -49,8 +49,14 @@ constexpr int kSsOver = 4;              // samples per bucket
 constexpr int kSsMaxSamples = 8192;     // at most: beyond 8192 buckets some splitters repeat
 constexpr int kSsThreads = 256;       // threads of the sample, bucket and merge workgroups
 constexpr int kSsTileThreads = 1024;  // threads of the counting / scattering workgroups
-constexpr int kSsTileMin = 4096;  // keys per counting / scattering workgroup (at least)
-constexpr int kSsMaxTiles = 256;  // counting / scattering workgroups (at most)
+constexpr int kSsTileMin = 2048;  // keys per counting / scattering workgroup (at least)
+constexpr int kSsMaxTiles = 1024;  // counting / scattering workgroups (at most)
+// Bucket capacities of the counting and scattering kernels' LDS (splitters
+// and one counter per bucket): the smallest that holds the sort's buckets,
+// so that small sorts keep their LDS, and with it the workgroups per CU, small.
+constexpr int kSsBucketsSmall = 1024;  // n <= 1.5M
+constexpr int kSsBucketsMid = 4096;    // n <= 6.3M
+static_assert(kSsBucketsMid < kSsMaxBuckets, "three capacity classes");
 constexpr int kSsRunItems = kSsCapMax / kSsThreads;         // 8 per thread
 constexpr int kSsSampleItems = kSsMaxSamples / kSsThreads;  // 32 per thread
 template <int IPT>
@@ -133,13 +139,14 @@ __device__ __forceinline__ void add_counts(const uint32_t* cnt, int32_t nb,
 }
 
 // 2. bucket totals of the keys (the first sort after an upload, whose
-// splitters are sampled from keys computed beforehand).
+// splitters are sampled from keys computed beforehand).  MAXB >= nb.
+template <int MAXB>
 __global__ __launch_bounds__(kSsTileThreads) void ss_count_kernel(const uint64_t* __restrict__ key,
                                                               int32_t n, int32_t nb, int32_t tile,
                                                               const uint64_t* __restrict__ spl,
                                                               uint32_t* __restrict__ tot) {
-  __shared__ uint64_t sp[kSsMaxBuckets];
-  __shared__ uint32_t cnt[kSsMaxBuckets];
+  __shared__ uint64_t sp[MAXB];
+  __shared__ uint32_t cnt[MAXB];
   load_splitters(spl, nb, sp, cnt);
   const int64_t t0 = (int64_t)blockIdx.x * tile;
   const int64_t t1 = t0 + tile < n ? t0 + tile : n;
@@ -152,14 +159,15 @@ __global__ __launch_bounds__(kSsTileThreads) void ss_count_kernel(const uint64_t
 
 // 1+2. keys of a tile (as the build's keys kernel computes them) and their
 // bucket totals, when the splitters are already known (every sort after the
-// first).
+// first).  MAXB >= nb.
+template <int MAXB>
 __global__ __launch_bounds__(kSsTileThreads) void ss_keys_count_kernel(
     const float* __restrict__ x, const float* __restrict__ y, int32_t n,
     const int32_t* __restrict__ inst_of, const float4* __restrict__ ibbox, int l1, int ibits,
     int32_t nb, int32_t tile, const uint64_t* __restrict__ spl, uint64_t* __restrict__ key,
     uint32_t* __restrict__ perm, uint32_t* __restrict__ tot) {
-  __shared__ uint64_t sp[kSsMaxBuckets];
-  __shared__ uint32_t cnt[kSsMaxBuckets];
+  __shared__ uint64_t sp[MAXB];
+  __shared__ uint32_t cnt[MAXB];
   load_splitters(spl, nb, sp, cnt);
   const int64_t t0 = (int64_t)blockIdx.x * tile;
   const int64_t t1 = t0 + tile < n ? t0 + tile : n;
@@ -179,13 +187,15 @@ namespace {
 // workgroup scans the totals itself (workgroup 0 also writes the bucket
 // starts for the bucket sort), then reserves its range in each bucket its
 // tile writes to.  cur[] (the buckets' fill levels) starts at zero.
+// MAXB >= nb.
+template <int MAXB>
 __global__ __launch_bounds__(kSsTileThreads) void ss_scatter_kernel(
     const uint64_t* __restrict__ key, const uint32_t* __restrict__ val, int32_t n, int32_t nb,
     int32_t tile, const uint64_t* __restrict__ spl, const uint32_t* __restrict__ tot,
     uint32_t* __restrict__ cur, uint32_t* __restrict__ bstart, uint64_t* __restrict__ key_out,
     uint32_t* __restrict__ val_out) {
-  __shared__ uint64_t sp[kSsMaxBuckets];
-  __shared__ uint32_t pos[kSsMaxBuckets];  // tile counts, then write cursors
+  __shared__ uint64_t sp[MAXB];
+  __shared__ uint32_t pos[MAXB];  // tile counts, then write cursors
   __shared__ uint32_t part[kSsTileThreads / 64];
   load_splitters(spl, nb, sp, pos);
   const int64_t t0 = (int64_t)blockIdx.x * tile;
@@ -464,10 +474,16 @@ void launch_sample_sort(stellar_ctx* c, bool keys_counted) {
   uint32_t* cur = B.ss_tot + kSsMaxBuckets;
   if (!keys_counted) {
     if (!c->ss_have_spl) ss_sample_kernel<<<1, kSsThreads, 0, c->stream>>>(B.key, n, nb, B.ss_spl);
-    ss_count_kernel<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, n, nb, tile, B.ss_spl, tot);
+    auto count = nb <= kSsBucketsSmall ? ss_count_kernel<kSsBucketsSmall>
+                 : nb <= kSsBucketsMid ? ss_count_kernel<kSsBucketsMid>
+                                       : ss_count_kernel<kSsMaxBuckets>;
+    count<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, n, nb, tile, B.ss_spl, tot);
   }
-  ss_scatter_kernel<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, B.perm, n, nb, tile, B.ss_spl, tot,
-                                                           cur, B.ss_bstart, B.key_alt, B.perm_alt);
+  auto scatter = nb <= kSsBucketsSmall ? ss_scatter_kernel<kSsBucketsSmall>
+                 : nb <= kSsBucketsMid ? ss_scatter_kernel<kSsBucketsMid>
+                                       : ss_scatter_kernel<kSsMaxBuckets>;
+  scatter<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, B.perm, n, nb, tile, B.ss_spl, tot, cur,
+                                                 B.ss_bstart, B.key_alt, B.perm_alt);
   // buckets in (key_alt, perm_alt); the sorted order goes back to (key, perm)
   ss_bucket_kernel<<<nb, kSsThreads, 0, c->stream>>>(B.key_alt, B.perm_alt, nb, B.ss_bstart, tot, cur,
                                                      B.ss_spl, c->sort_cap, B.key, B.perm);
@@ -479,9 +495,13 @@ void launch_sample_sort(stellar_ctx* c, bool keys_counted) {
 void launch_keys_count(stellar_ctx* c, int l1, int ibits) {
   Buffers& B = c->b;
   const int32_t n = c->n;
-  ss_keys_count_kernel<<<sample_sort_tiles(n), kSsTileThreads, 0, c->stream>>>(
-      B.x, B.y, n, B.inst_of, B.ibbox, l1, ibits, sample_sort_buckets(n), sample_sort_tile(n),
-      B.ss_spl, B.key, B.perm, B.ss_tot);
+  const int32_t nb = sample_sort_buckets(n);
+  auto k = nb <= kSsBucketsSmall ? ss_keys_count_kernel<kSsBucketsSmall>
+           : nb <= kSsBucketsMid ? ss_keys_count_kernel<kSsBucketsMid>
+                                 : ss_keys_count_kernel<kSsMaxBuckets>;
+  k<<<sample_sort_tiles(n), kSsTileThreads, 0, c->stream>>>(B.x, B.y, n, B.inst_of, B.ibbox, l1,
+                                                            ibits, nb, sample_sort_tile(n),
+                                                            B.ss_spl, B.key, B.perm, B.ss_tot);
 }
 
 }  // namespace stellar
