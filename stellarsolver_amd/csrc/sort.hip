@@ -47,7 +47,8 @@ namespace {
 
 constexpr int kSsOver = 4;              // samples per bucket
 constexpr int kSsMaxSamples = 8192;     // at most: beyond 8192 buckets some splitters repeat
-constexpr int kSsThreads = 256;       // threads of the sample, bucket and merge workgroups
+constexpr int kSsThreads = 256;       // threads of the sample workgroup
+constexpr int kSsBucketThreads = 512;  // threads of a bucket's workgroup
 constexpr int kSsTileThreads = 1024;  // threads of the counting / scattering workgroups
 constexpr int kSsTileMin = 2048;  // keys per counting / scattering workgroup (at least)
 constexpr int kSsMaxTiles = 1024;  // counting / scattering workgroups (at most)
@@ -57,14 +58,14 @@ constexpr int kSsMaxTiles = 1024;  // counting / scattering workgroups (at most)
 constexpr int kSsBucketsSmall = 1024;  // n <= 1.5M
 constexpr int kSsBucketsMid = 4096;    // n <= 6.3M
 static_assert(kSsBucketsMid < kSsMaxBuckets, "three capacity classes");
-constexpr int kSsRunItems = kSsCapMax / kSsThreads;         // 8 per thread
+constexpr int kSsRunItems = kSsCapMax / kSsBucketThreads;   // 4 per thread
 constexpr int kSsSampleItems = kSsMaxSamples / kSsThreads;  // 32 per thread
 template <int IPT>
-using RunSort = rocprim::block_radix_sort<uint64_t, kSsThreads, IPT, uint32_t>;
+using RunSort = rocprim::block_radix_sort<uint64_t, kSsBucketThreads, IPT, uint32_t>;
 template <int IPT>
 using SampleSort = rocprim::block_radix_sort<uint64_t, kSsThreads, IPT>;
 // 32-bit window of the keys with the run position as value (one-run buckets)
-using WinSort = rocprim::block_radix_sort<uint32_t, kSsThreads, kSsRunItems, uint32_t>;
+using WinSort = rocprim::block_radix_sort<uint32_t, kSsBucketThreads, kSsRunItems, uint32_t>;
 constexpr int kSsFixMax = 32;  // longest run of equal windows fixed by one thread
 template <class A, class B>
 constexpr size_t max_size() {
@@ -292,7 +293,7 @@ __device__ void sort_run(const uint64_t* kb, const uint32_t* vb, int64_t r0, int
       k, v, *reinterpret_cast<typename RunSort<IPT>::storage_type*>(storage), 0u, end_bit);
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
-    const int i = j * kSsThreads + threadIdx.x;  // striped arrangement
+    const int i = j * kSsBucketThreads + threadIdx.x;  // striped arrangement
     if (i < rm) {
       ko[r0 + i] = k[j];
       vo[r0 + i] = v[j];
@@ -371,7 +372,7 @@ __device__ bool sort_run_window(const uint64_t* kb, const uint32_t* vb, int64_t 
   }
   __syncthreads();
   if (L.fallback) return false;
-  for (int i = threadIdx.x; i < rm; i += kSsThreads) {
+  for (int i = threadIdx.x; i < rm; i += kSsBucketThreads) {
     const uint32_t src = L.u.post.pos[i];
     ko[r0 + i] = L.key[src];
     vo[r0 + i] = vb[r0 + src];
@@ -381,7 +382,7 @@ __device__ bool sort_run_window(const uint64_t* kb, const uint32_t* vb, int64_t 
 
 // The bucket's totals and fill level are cleared for the next sort here (the
 // scatter that read them has finished).
-__global__ __launch_bounds__(kSsThreads) void ss_bucket_kernel(
+__global__ __launch_bounds__(kSsBucketThreads) void ss_bucket_kernel(
     uint64_t* __restrict__ kb, uint32_t* __restrict__ vb, int32_t nb,
     const uint32_t* __restrict__ bstart, uint32_t* __restrict__ tot, uint32_t* __restrict__ cur,
     const uint64_t* __restrict__ spl, int32_t cap, uint64_t* __restrict__ ko,
@@ -485,7 +486,7 @@ void launch_sample_sort(stellar_ctx* c, bool keys_counted) {
   scatter<<<nwg, kSsTileThreads, 0, c->stream>>>(B.key, B.perm, n, nb, tile, B.ss_spl, tot, cur,
                                                  B.ss_bstart, B.key_alt, B.perm_alt);
   // buckets in (key_alt, perm_alt); the sorted order goes back to (key, perm)
-  ss_bucket_kernel<<<nb, kSsThreads, 0, c->stream>>>(B.key_alt, B.perm_alt, nb, B.ss_bstart, tot, cur,
+  ss_bucket_kernel<<<nb, kSsBucketThreads, 0, c->stream>>>(B.key_alt, B.perm_alt, nb, B.ss_bstart, tot, cur,
                                                      B.ss_spl, c->sort_cap, B.key, B.perm);
   // the next sort's splitters are read off the sorted keys by the build's
   // tie kernel, which runs next
