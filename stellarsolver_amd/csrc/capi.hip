@@ -141,9 +141,11 @@ int run_step(stellar_ctx* c, int32_t step) {
   mark(c, ev, 3);
   launch_sort(c);
   mark(c, ev, 4);
-  launch_forces(c, false);  // the sort stage zeroed the interaction counter
+  // the sort stage zeroed the interaction counter; the update runs inside
+  // the walk when the walk allows it
+  const bool updated = launch_forces(c, false, true);
   mark(c, ev, 5);
-  launch_update(c);
+  if (!updated) launch_update(c);
   mark(c, ev, 6);
   // the output buffer is filled when it is read (stellar_download_positions)
   mark(c, ev, 7);

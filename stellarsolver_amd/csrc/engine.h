@@ -275,7 +275,9 @@ void launch_bbox(stellar_ctx* c);
 void launch_build(stellar_ctx* c);
 void launch_com(stellar_ctx* c);
 void launch_sort(stellar_ctx* c);
-void launch_forces(stellar_ctx* c, bool zero_counter);
+// The walk; with_update: also the step's update, fused into the walk when the
+// walk allows it (returns true then; false: launch_update is still needed).
+bool launch_forces(stellar_ctx* c, bool zero_counter, bool with_update = false);
 // Key sort (sort.hip).
 bool sample_sort_fits(int32_t n);
 int32_t sample_sort_buckets(int32_t n);

@@ -70,7 +70,38 @@ struct WalkArgs {
   int ref_root;
   const int32_t* id;
   const int32_t* ioff;
+  // update != 0 (G = 64 walks that pop one node at a time): each group then
+  // applies the step's update to its own bodies (update_kernel's arithmetic,
+  // v += dt a; x += ddt v with ddt = d dt) right after its forces.  Only the
+  // group reads its bodies' x, y (every other group sees them through the
+  // tree's copies), so the result equals the separate update launch.
+  int update;
+  float dt;
+  float ddt;
+  float* xw;
+  float* yw;
+  float* vx;
+  float* vy;
 };
+
+// The fused update's arguments, copied to LDS (one copy per wavefront) at the
+// kernel's start and read back after the walk: held as kernel arguments they
+// would occupy scalar registers through the walk's loop, which needs all of
+// them (spills otherwise).
+struct WalkUpdate {
+  float* x;
+  float* y;
+  float* vx;
+  float* vy;
+  float dt;
+  float ddt;
+  int on;
+  int pad;
+};
+
+__device__ __forceinline__ void stage_update(const WalkArgs& A, WalkUpdate* u) {
+  if ((threadIdx.x & 63) == 0) *u = WalkUpdate{A.xw, A.yw, A.vx, A.vy, A.dt, A.ddt, A.update, 0};
+}
 
 // r = dx*dx + dy*dy + eps2 as dx*dx + (dy*dy + eps2), two fused
 // multiply-adds (the CPU restatement evaluates the same form).
@@ -227,7 +258,7 @@ template <bool QUAD, bool THR, int KPOP, int STACK = kStack64, bool MIXED = fals
 __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
                                              const float4* __restrict__ qblk,
                                              const WalkArgs& A, int64_t g,
-                                             uint32_t* st) {
+                                             uint32_t* st, const WalkUpdate* up) {
   static_assert(!MIXED || (QUAD && KPOP == 1), "the mixed far field pairs cells in visiting order");
   const float eps2 = A.eps2;
   const int lane = threadIdx.x & 63;
@@ -385,6 +416,18 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
     A.ay[s] = acc.fy;
     A.inter[s] = acc.cnt;
   }
+  // the fused update (incomplete forces, sp > 0 after a stack overflow,
+  // leave the state as it was); only walks that pop one node at a time are
+  // given it
+  if (KPOP == 1 && sp == 0 && __builtin_amdgcn_readfirstlane(up->on) && valid) {
+    const WalkUpdate U = *up;
+    const float u = __builtin_fmaf(U.dt, acc.fx, U.vx[s]);
+    const float v = __builtin_fmaf(U.dt, acc.fy, U.vy[s]);
+    U.vx[s] = u;
+    U.vy[s] = v;
+    U.x[s] = __builtin_fmaf(U.ddt, u, px);
+    U.y[s] = __builtin_fmaf(U.ddt, v, py);
+  }
   if (lane == 0) {
     atomicAdd(&A.dev->interactions, (unsigned long long)acc.cnt * (unsigned long long)cnt);
     if (QUAD)
@@ -403,11 +446,13 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_num_sgpr(kWalkSgp
     const ChildRec* __restrict__ blk, const float4* __restrict__ qblk,
     WalkArgs A) {
   __shared__ uint32_t stk[WAVES][kStack64];
+  __shared__ WalkUpdate upd[WAVES];
   const int wave = threadIdx.x >> 6;
   const int64_t g = (int64_t)xcd_block(blockIdx.x, gridDim.x) * WAVES + wave;
   if (g >= A.ngroups) return;
   if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
-  walk64_group<QUAD, THR, KPOP, kStack64, MIXED>(blk, qblk, A, g, stk[wave]);
+  stage_update(A, &upd[wave]);
+  walk64_group<QUAD, THR, KPOP, kStack64, MIXED>(blk, qblk, A, g, stk[wave], &upd[wave]);
 }
 
 // Persistent form: exactly the resident workgroups, each taking work from
@@ -432,10 +477,12 @@ template <bool QUAD, bool THR, int KPOP, int WAVES, bool CLAIM_WAVE, bool MIXED 
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_num_sgpr(kWalkSgprs))) void walk64_queue_kernel(
     const ChildRec* __restrict__ blk, const float4* __restrict__ qblk, WalkArgs A) {
   __shared__ uint32_t stk[WAVES * kStack64];
+  __shared__ WalkUpdate upd[WAVES];
   uint32_t& claim = stk[WAVES * kStack64 - 1];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   if (A.dev->err & kBuildErrors) return;  // the tree of this step is incomplete
+  stage_update(A, &upd[wave]);
   const int64_t units = CLAIM_WAVE ? A.ngroups : (A.ngroups + WAVES - 1) / WAVES;
   const int x0 = (int)(blockIdx.x % kXcds);
   int k = 0;
@@ -459,7 +506,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_num_sgpr(kWalkSgp
     }
     const int64_t g = CLAIM_WAVE ? u : u * WAVES + wave;
     if (g < A.ngroups)
-      walk64_group<QUAD, THR, KPOP, kStackQ, MIXED>(blk, qblk, A, g, stk + wave * kStack64);
+      walk64_group<QUAD, THR, KPOP, kStackQ, MIXED>(blk, qblk, A, g, stk + wave * kStack64,
+                                                    &upd[wave]);
   }
 }
 
@@ -714,19 +762,29 @@ void launch_g(stellar_ctx* c, int64_t ngroups, const WalkArgs& A) {
 
 }  // namespace
 
-void launch_forces(stellar_ctx* c, bool zero_counter) {
+bool launch_forces(stellar_ctx* c, bool zero_counter, bool with_update) {
   const int32_t n = c->n;
-  if (n == 0) return;
+  if (n == 0) return false;
   Buffers& B = c->b;
   const int gi = c->p.group == 32 ? 1 : c->p.group == 16 ? 2 : 0;
   const int64_t ngroups = c->ngroups[gi];
   const float theta = c->p.theta;
   const float tdiv = (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? theta * theta : theta;
+  // the update is fused into G = 64 walks that pop one node at a time: their
+  // stack cannot overflow (at most 1 + 3 per level), so no group's forces
+  // can be incomplete once another group has updated its bodies
+  const bool quad_or_thr =
+      (c->p.flags & (STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_BMAX_OPENING)) != 0;
+  const int kpop = c->walk_kpop > 0 ? c->walk_kpop : (quad_or_thr ? 1 : 2);
+  const bool fuse = with_update && gi == 0 &&
+                    (c->p.precision == STELLAR_PRECISION_MIXED || kpop == 1);
   WalkArgs A{B.x,       B.y,       B.blk,       B.qblk,   n,
              c->p.eps2, tdiv,      B.dev,       B.ax,     B.ay,
              B.inter,   B.gtab[gi], ngroups,    B.inst_of, B.ibbox,
              (c->p.flags & STELLAR_FLAG_TEXTBOOK_OPENING) ? 1 : 0,
-             (c->p.flags & STELLAR_FLAG_REFERENCE_ROOT) ? 1 : 0, B.id, B.ioff};
+             (c->p.flags & STELLAR_FLAG_REFERENCE_ROOT) ? 1 : 0, B.id, B.ioff,
+             fuse ? 1 : 0, c->p.timestep, c->p.dampening * c->p.timestep,
+             B.x, B.y, B.vx, B.vy};
   if (zero_counter) launch_zero_counter(c);
   switch (c->p.group) {
     case 32:
@@ -739,6 +797,7 @@ void launch_forces(stellar_ctx* c, bool zero_counter) {
       launch_g<64>(c, ngroups, A);
       break;
   }
+  return fuse;
 }
 
 }  // namespace stellar

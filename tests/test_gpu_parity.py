@@ -314,6 +314,43 @@ def test_steps_match_oracle(S, oracle):
             assert np.array_equal(out[0::2], post["x"]) and np.array_equal(out[1::2], post["y"])
 
 
+@pytest.mark.parametrize("model", ["quad", "quad_queue", "bmax", "mixed", "mono_kpop1"])
+def test_fused_update_matches_separate_update(S, oracle, model):
+    """Walks that pop one node at a time apply the step's update to their own
+    bodies right after the forces: a step (stellar_step) equals the stage
+    calls with the separate update kernel bit for bit, and equals the oracle
+    update applied to its forces."""
+    n = 20000
+    st = ic(2, n)
+    eps2 = float(np.float32(1e-4))
+    dt, d = np.float32(1e-3), np.float32(0.999)
+    flags = {"quad": S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING,
+             "quad_queue": S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING,
+             "bmax": S.FLAG_QUADRUPOLE | S.FLAG_BMAX_OPENING,
+             "mixed": S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING,
+             "mono_kpop1": 0}[model]
+    prec = S.PRECISION_MIXED if model == "mixed" else S.PRECISION_FP32
+    kw = dict(timestep=float(dt), dampening=float(d), flags=flags, precision=prec)
+    with make_engine(S, st, eps2, **kw) as a, make_engine(S, st, eps2, **kw) as b:
+        for e in (a, b):
+            if model == "quad_queue":
+                e.set_walk_schedule(S.WALK_QUEUE)
+            if model == "mono_kpop1":
+                e.set_walk_shape(1, 512)
+        for k in range(3):
+            print(f"{model} step {k}", flush=True)
+            pre = a.download()
+            a.update()  # one step: the update inside the walk
+            b.bbox(); b.build(); b.com(); b.sort(); b.forces(); b.update_stage()
+            pa, pb = a.download(), b.download()
+            for key in ("x", "y", "vx", "vy", "ax", "ay"):
+                assert np.array_equal(pa[key], pb[key]), (model, k, key)
+            x, y, vx, vy = oracle.update(pre["x"], pre["y"], pre["vx"], pre["vy"], pa["ax"],
+                                         pa["ay"], dt, d)
+            for u, v in ((x, pa["x"]), (y, pa["y"]), (vx, pa["vx"]), (vy, pa["vy"])):
+                assert np.array_equal(u, v), (model, k)
+
+
 def test_run_is_deterministic(S):
     st = ic(1, 65536)
     eps2 = float(np.float32(0.005 ** 2))
