@@ -10,42 +10,6 @@ inline unsigned grid_for(int64_t items, int threads) {
   return (unsigned)((items + threads - 1) / threads);
 }
 
-// sort_kernel (kernels.cu:292-336) lays bodies out in DFS order through
-// sorted[]; the build here already produced that order (perm), so this stage
-// moves the body state itself into it, making every later access coalesced.
-// The build's link kernel already wrote x, y and m in tree order (it reads
-// them through perm for the tree records); this kernel gathers the rest.
-// After a failed build (sticky error bits) the tree and perm are incomplete:
-// the kernel then copies the state unchanged into the spare buffers, which
-// the host swaps in as after any sort, so a failed step moves no body.
-__global__ __launch_bounds__(256) void permute_kernel(
-    int32_t n, const uint32_t* __restrict__ perm, const float* __restrict__ x,
-    const float* __restrict__ y, const float* __restrict__ m, const float* __restrict__ vx,
-    const float* __restrict__ vy, const int32_t* __restrict__ id, float* __restrict__ x2,
-    float* __restrict__ y2, float* __restrict__ m2, float* __restrict__ vx2,
-    float* __restrict__ vy2, int32_t* __restrict__ id2, DevScalars* __restrict__ dev) {
-  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool failed = (__builtin_amdgcn_readfirstlane(dev->err) & kBuildErrors) != 0;
-  if (s == 0 && !failed) {  // for the walk that follows: counters and queues
-    dev->interactions = 0;
-    dev->quad_interactions = 0;
-    for (int x = 0; x < kXcds; ++x) dev->wq[x] = 0u;
-  }
-  if (s >= n) return;
-  if (failed) {
-    x2[s] = x[s];
-    y2[s] = y[s];
-    m2[s] = m[s];
-    vx2[s] = vx[s];
-    vy2[s] = vy[s];
-    id2[s] = id[s];
-    return;
-  }
-  const uint32_t p = perm[s];
-  vx2[s] = vx[p];
-  vy2[s] = vy[p];
-  id2[s] = id[p];
-}
 
 // update_kernel (kernels.cu:522-537): v += dt*a; x += d*dt*v, both as fused
 // multiply-adds with d*dt rounded first, as nvcc contracts the reference.
@@ -282,13 +246,11 @@ __global__ __launch_bounds__(kTile) void energy_fp64_kernel(
 
 }  // namespace
 
+// The sort stage: the build's link kernel already wrote the body state in
+// tree order into the spare buffers (kernels.cu:292-336 sort the bodies by
+// tree order; here the order is the build's), so the stage swaps them in.
 void launch_sort(stellar_ctx* c) {
-  const int32_t n = c->n;
-  if (n == 0) return;
   Buffers& B = c->b;
-  permute_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(n, B.perm, B.x, B.y, B.m, B.vx, B.vy,
-                                                           B.id, B.x2, B.y2, B.m2, B.vx2, B.vy2,
-                                                           B.id2, B.dev);
   std::swap(B.x, B.x2);
   std::swap(B.y, B.y2);
   std::swap(B.vx, B.vx2);

@@ -463,10 +463,16 @@ struct LinkArgs {
   int32_t nroot;  // number of instances = root blocks; cell c is block nroot + c
   MinTree mt;
   const float4* ibbox;
-  // x, y and m moved into tree order for the sort stage, which swaps them in
+  // the body state moved into tree order for the sort stage, which swaps it in
+  const float* vx;
+  const float* vy;
+  const int32_t* id;
   float* x2;
   float* y2;
   float* m2;
+  float* vx2;
+  float* vy2;
+  int32_t* id2;
 };
 
 // Id of the first cell pair a opens.
@@ -494,13 +500,17 @@ __device__ __forceinline__ void put_cell(const LinkArgs& A, int64_t blk, int slo
 __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= A.n) return;
-  // body s's record values, read once through the permutation and also
-  // written out in tree order: the sort stage then gathers only vx, vy, id
+  // body s's state, read once through the permutation: its record values,
+  // and the whole state written out in tree order for the sort stage (a
+  // permutation even when the build fails, so the state stays whole)
   const uint32_t p = A.perm[s];
   const float bx = A.x[p], by = A.y[p], bm = A.m[p];
   A.x2[s] = bx;
   A.y2[s] = by;
   A.m2[s] = bm;
+  A.vx2[s] = A.vx[p];
+  A.vy2[s] = A.vy[p];
+  A.id2[s] = A.id[p];
   const ChildRec body{bx, by, bm, -1 - s};
   const int32_t inst = A.inst_of[s];
   const int32_t i0 = A.ioff[inst];
@@ -512,9 +522,14 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
   if (s == A.n - 1) {
     if (cs > A.cap) atomicOr(&A.dev->err, kErrPool);
     A.dev->ncells = (int32_t)(cs < A.cap ? cs : A.cap);
-    // the centre-of-mass passes count their leftovers from zero
+    // the centre-of-mass passes count their leftovers from zero, the walk
+    // its interactions and work-queue claims
 #pragma unroll
     for (int k = 0; k < 4; ++k) A.dev->com_len[k] = 0;
+    A.dev->interactions = 0;
+    A.dev->quad_interactions = 0;
+#pragma unroll
+    for (int k = 0; k < kXcds; ++k) A.dev->wq[k] = 0u;
   }
   // block of the node created at s at level L (-1: beyond the pool)
   auto created = [&](int L) -> int64_t {
@@ -663,7 +678,8 @@ void launch_build(stellar_ctx* c) {
              B.perm,    B.x,      B.y,     B.m,       B.blk,    B.cmeta,
              B.dev,     B.inst_of, B.ioff,
              c->ninst,  mt,        B.ibbox,
-             B.x2,      B.y2,     B.m2};
+             B.vx,      B.vy,     B.id,
+             B.x2,      B.y2,     B.m2,      B.vx2,    B.vy2,    B.id2};
   link_kernel<<<nblk, kScanBlock, 0, c->stream>>>(A);
 }
 
