@@ -51,18 +51,16 @@ HBM_PEAK_GBS = 8000.0
 DEFAULT_KEY = (2, 1 << 20, 0.5, 64, "quad", "textbook")
 # HBM bytes per step of each stage's kernels: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (KiB per dispatch), the factors being what the same counters read for a 1 GiB
-# device copy; "forces" is the walk launch.
+# device copy; "forces" is the walk launch (with the update inside it).
 STAGE_TRAFFIC_BYTES = {DEFAULT_KEY: {
     "bbox": (4125 * 2 + 8) * 1024,
-    "build": (64555 * 2 + 186633) * 1024,
-    "com": (32047 * 2 + 39591) * 1024,
-    "sort": (21178 * 2 + 12288) * 1024,
-    "forces": (181412 * 2 + 13312) * 1024,
-    "update": (12294 * 2 + 16384) * 1024,
+    "build": (91796 * 2 + 198884) * 1024,
+    "com": (32032 * 2 + 39590) * 1024,
+    "forces": (184781 * 2 + 27876) * 1024,
 }}
 # The walk's VALU busy fraction: SQ_ACTIVE_INST_VALU / (256 CUs x GRBM_GUI_ACTIVE
-# per XCD) per dispatch, 432.1e6 / (256 x 18.16e6 / 8).
-WALK_VALU_BUSY = {DEFAULT_KEY: 432.1e6 / (256 * 18.16e6 / 8)}
+# per XCD) per dispatch, 432.5e6 / (256 x 18.07e6 / 8).
+WALK_VALU_BUSY = {DEFAULT_KEY: 432.5e6 / (256 * 18.07e6 / 8)}
 
 
 def progress(msg: str) -> None:
@@ -207,16 +205,18 @@ def force_error(eng, n, sample):
 
 def stage_bytes(n, cells):
     """Algorithmic HBM bytes of each memory-bound stage per step (N bodies,
-    C cells): bbox 8N (x, y read); build 8N + 20C (x, y read; per cell its
-    four 4-byte child links and a 4-byte first-body index written); centre of
-    mass 12N + 12C (bodies' x, y, m read; a cell's x, y, m written); sort
-    4N + 24C + 24N (the cells' first-body indices and moments read, the six
-    body arrays moved into tree order); update 40N (x, y, vx, vy, ax, ay
-    read, x, y, vx, vy written).  What a stage moves at least, not what it
-    fetches: the walk's second 64-byte block per node (quadrupole moments and
-    opening thresholds) is a term of this layout on top of these."""
-    return {"bbox": 8 * n, "build": 8 * n + 20 * cells, "com": 12 * n + 12 * cells,
-            "sort": 4 * n + 24 * cells + 24 * n, "update": 40 * n}
+    C cells): bbox 8N (x, y read); build 8N + 20C + 48N (x, y read; per cell
+    its four 4-byte child links and a 4-byte first-body index written; the
+    six body arrays moved into tree order, 24 B read and 24 B written per
+    body, which the sort stage then swaps in); centre of mass 12N + 12C
+    (bodies' x, y, m read; a cell's x, y, m written); update 40N (x, y, vx,
+    vy, ax, ay read, x, y, vx, vy written) when it runs as its own kernel
+    (walks that pop one node at a time do it inside the walk).  What a stage
+    moves at least, not what it fetches: the walk's second 64-byte block per
+    node (quadrupole moments and opening thresholds) is a term of this
+    layout on top of these."""
+    return {"bbox": 8 * n, "build": 8 * n + 20 * cells + 48 * n, "com": 12 * n + 12 * cells,
+            "update": 40 * n}
 
 
 def stage_roofline(stages, n, cells, key):

@@ -230,7 +230,10 @@ int stellar_last_quad_interactions(stellar_ctx* ctx, int64_t* total);
 /* Per-stage device milliseconds of the last step (mean over the steps of
  * the last stellar_run) when STELLAR_FLAG_STAGE_TIMING is set: [bbox,
  * build, com, sort, forces, update, output]; with STELLAR_FLAG_FORCES_TIMING
- * only the forces entry. */
+ * only the forces entry.  A stage that launched no kernel reports 0: sort
+ * (the build already wrote the bodies in tree order; the stage swaps
+ * buffers), output (filled when read) and update when the walk applied it
+ * (group-64 walks that pop one node at a time). */
 int stellar_stage_times(stellar_ctx* ctx, float* ms7);
 
 /* ---- launch-shape autotuner (no counterpart in the reference) ---------- */

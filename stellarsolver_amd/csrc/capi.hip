@@ -141,12 +141,15 @@ int run_step(stellar_ctx* c, int32_t step) {
   mark(c, ev, 3);
   launch_sort(c);
   mark(c, ev, 4);
-  // the sort stage zeroed the interaction counter; the update runs inside
-  // the walk when the walk allows it
+  // the build zeroed the interaction counter; the update runs inside the
+  // walk when the walk allows it
   const bool updated = launch_forces(c, false, true);
   mark(c, ev, 5);
   if (!updated) launch_update(c);
   mark(c, ev, 6);
+  // stages that launched nothing report zero time (the sort stage swaps
+  // buffers, the output buffer is filled when read)
+  c->stage_empty = (uint8_t)((1u << 3) | (updated ? 1u << 5 : 0u) | (1u << 6));
   // the output buffer is filled when it is read (stellar_download_positions)
   mark(c, ev, 7);
   hipError_t e = hipGetLastError();
@@ -233,7 +236,8 @@ int collect_times(stellar_ctx* c, int32_t steps) {
       acc[k] += ms;
     }
   }
-  for (int k = 0; k < 7; ++k) c->stage_ms[k] = (float)(acc[k] / nsets);
+  for (int k = 0; k < 7; ++k)
+    c->stage_ms[k] = (c->stage_empty >> k) & 1u ? 0.f : (float)(acc[k] / nsets);
   return STELLAR_OK;
 }
 

@@ -265,6 +265,7 @@ struct stellar_ctx {
   std::vector<hipEvent_t> evrun;
   int32_t evrun_steps = 0;
   float stage_ms[7] = {};
+  uint8_t stage_empty = 0;  // stages of the last step that launched no kernel (bit k)
   std::string err;
 };
 

@@ -267,17 +267,27 @@ def test_radix_sort_path(S, oracle, graph):
 
 
 def test_stage_timing_modes(S):
-    """Eight HIP events per step time every stage; the forces-only mode times
-    the walk alone and reports zeros for the rest."""
+    """Eight HIP events per step time every stage (zero for a stage that
+    launched nothing: the sort stage only swaps buffers, the output buffer is
+    filled when read, and the update runs inside walks that pop one node at
+    a time); the forces-only mode times the walk alone and reports zeros for
+    the rest."""
     st = ic(2, 65536)
     eps2 = float(np.float32(EPS[2] ** 2))
-    with make_engine(S, st, eps2, flags=S.FLAG_STAGE_TIMING) as e:
+    with make_engine(S, st, eps2, flags=S.FLAG_STAGE_TIMING) as e:  # monopoles: 2 nodes per pop
         e.run(3)
         full = e.stage_times()
+    quad = S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING
+    with make_engine(S, st, eps2, flags=S.FLAG_STAGE_TIMING | quad) as e:
+        e.run(3)
+        fused = e.stage_times()
     with make_engine(S, st, eps2, flags=S.FLAG_FORCES_TIMING) as e:
         e.run(3)
         walk = e.stage_times()
-    assert all(full[k] > 0 for k in ("bbox", "build", "com", "sort", "forces", "update"))
+    assert all(full[k] > 0 for k in ("bbox", "build", "com", "forces", "update"))
+    assert full["sort"] == 0 and full["output"] == 0
+    assert all(fused[k] > 0 for k in ("bbox", "build", "com", "forces"))
+    assert fused["sort"] == 0 and fused["update"] == 0
     assert walk["forces"] > 0
     assert all(walk[k] == 0 for k in walk if k != "forces")
 
