@@ -155,6 +155,22 @@ int stellar_download(stellar_ctx* ctx, float* mass, float* x, float* y,
  * floats of the reference's output buffer (kernels.cu:541-553). */
 int stellar_download_positions(stellar_ctx* ctx, float* out_xy);
 
+/* Headless rendering of the current positions: the reference draws every
+ * body as a one-pixel point of colour (1, 1, 1) and alpha 0.1, blended
+ * additively over black (glBlendFunc(GL_SRC_ALPHA, GL_ONE)) through
+ * glm::ortho(-25, 25, -25, 25) into a 1000 x 1000 window
+ * (StellarSolverVisualizer.cpp:13-19, 208-253).  view = {left, right,
+ * bottom, top}; the body at (x, y) falls in column floor((x - left) * sx)
+ * and row height - 1 - floor((y - bottom) * sy), sx = width / (right - left),
+ * sy = height / (top - bottom), all FP32 as written (row 0 at the top);
+ * bodies outside the image are clipped.  counts (host, width x height, may be
+ * NULL): bodies per pixel; grey (host, width x height bytes, may be NULL):
+ * min(255, round(25.5 count)), the blend's white at alpha 0.1 per body,
+ * clamped and quantised once.  Synchronous; reads the state, changes nothing.
+ * STELLAR_ERR_ARG for an empty view or more than 2^28 pixels. */
+int stellar_render(stellar_ctx* ctx, const float* view, int32_t width, int32_t height,
+                   uint32_t* counts, uint8_t* grey);
+
 /* ---- the six stages (kernels.cu:62-537) -------------------------------- */
 int stellar_bbox(stellar_ctx* ctx);    /* compute_bounding_box_kernel      */
 int stellar_build(stellar_ctx* ctx);   /* build_tree_kernel                */

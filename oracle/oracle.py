@@ -364,6 +364,36 @@ def step(state: dict, theta: float, eps2: float, dt: float, d: float, group: int
     return secs
 
 
+def render_counts(x, y, view, width: int, height: int) -> np.ndarray:
+    """Bodies per pixel of the reference's point drawing, headless
+    (StellarSolverVisualizer.cpp:13-19, 208-253: glm::ortho onto the window,
+    one pixel per point).  The body at (x, y) falls in column
+    floor((x - left) * sx) and row height - 1 - floor((y - bottom) * sy),
+    sx = width / (right - left), sy = height / (top - bottom), FP32 as
+    written; bodies outside the image are clipped.  Returns height x width
+    uint32, row 0 at the top."""
+    x, y = f32(x), f32(y)
+    left, right, bottom, top = (np.float32(v) for v in view)
+    sx = np.float32(width) / (right - left)
+    sy = np.float32(height) / (top - bottom)
+    with np.errstate(invalid="ignore"):
+        fx = (x - left) * sx
+        fy = (y - bottom) * sy
+        ok = (fx >= 0) & (fx < np.float32(width)) & (fy >= 0) & (fy < np.float32(height))
+    col = fx[ok].astype(np.int64)
+    row = height - 1 - fy[ok].astype(np.int64)
+    counts = np.bincount(row * width + col, minlength=width * height)
+    return counts.astype(np.uint32).reshape(height, width)
+
+
+def render_grey(counts) -> np.ndarray:
+    """The reference's blend of white at alpha 0.1 per body over black
+    (glBlendFunc(GL_SRC_ALPHA, GL_ONE)), clamped, quantised once to 8 bits:
+    min(255, round(25.5 count))."""
+    c = np.asarray(counts, dtype=np.uint64)
+    return np.where(c >= 10, 255, (51 * c + 1) // 2).astype(np.uint8)
+
+
 def to_reference_layout(tree: dict, n: int, m: int | None = None) -> dict:
     """Canonical tree -> reference pool arrays (cells at n + c, root in child[0..3])."""
     c = tree["ncells"]

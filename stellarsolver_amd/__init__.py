@@ -118,6 +118,7 @@ SIGNATURES = {
     "stellar_upload": (C.c_int, [_P, _F, _F, _F, _F, _F]),
     "stellar_download": (C.c_int, [_P, _F, _F, _F, _F, _F, _F, _F]),
     "stellar_download_positions": (C.c_int, [_P, _F]),
+    "stellar_render": (C.c_int, [_P, _F, C.c_int32, C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)]),
     "stellar_bbox": (C.c_int, [_P]),
     "stellar_bbox_noalloc": (C.c_int, [_P, _P, C.c_int32, _P, _P]),
     "stellar_build": (C.c_int, [_P]),
@@ -301,6 +302,20 @@ class Engine:
 
     def get_num_particles(self) -> int:
         return self.n
+
+    def render(self, view=(-25.0, 25.0, -25.0, 25.0), width: int = 1000,
+               height: int = 1000) -> tuple[np.ndarray, np.ndarray]:
+        """Headless form of the reference's point drawing
+        (StellarSolverVisualizer.cpp:208-253): (counts, grey), each
+        height x width with row 0 at the top."""
+        v = np.ascontiguousarray(view, dtype=np.float32)
+        counts = np.zeros((height, width), np.uint32)
+        grey = np.zeros((height, width), np.uint8)
+        self._check(self._lib.stellar_render(
+            self._h, _fp(v), int(width), int(height),
+            counts.ctypes.data_as(C.POINTER(C.c_uint32)),
+            grey.ctypes.data_as(C.POINTER(C.c_uint8))), "render")
+        return counts, grey
 
     # -- stages ---------------------------------------------------------
     def bbox(self) -> None:

@@ -86,6 +86,58 @@ __global__ __launch_bounds__(256) void output_kernel(
   }
 }
 
+// Headless form of the reference's point drawing (StellarSolverVisualizer.cpp:
+// 208-253): every body adds one to the count of the pixel it falls in,
+// column floor((x - left) * sx), row height-1-floor((y - bottom) * sy) (row 0
+// at the top, as an image is stored), FP32 as written; bodies outside the
+// view are clipped.  Bodies are in tree order, so neighbouring lanes mostly
+// share a pixel: each run of equal pixels within a wavefront adds its length
+// with one atomic from its first lane.
+__global__ __launch_bounds__(256) void render_count_kernel(
+    int32_t n, const float* __restrict__ x, const float* __restrict__ y, float left,
+    float bottom, float sx, float sy, int32_t width, int32_t height,
+    uint32_t* __restrict__ counts) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int32_t p = -1;
+  if (i < n) {
+    const float fx = (x[i] - left) * sx;
+    const float fy = (y[i] - bottom) * sy;
+    // also false for NaN
+    if (fx >= 0.f && fx < (float)width && fy >= 0.f && fy < (float)height) {
+      const int32_t col = min((int32_t)fx, width - 1);
+      const int32_t row = height - 1 - min((int32_t)fy, height - 1);
+      p = row * width + col;
+    }
+  }
+  const int32_t prev = __shfl_up(p, 1);
+  const uint64_t starts = __builtin_amdgcn_ballot_w64(lane == 0 || prev != p);
+  if (p >= 0 && (lane == 0 || prev != p)) {
+    // the run ends at the next start (or the wavefront's end)
+    const uint64_t later = lane == 63 ? 0ull : starts >> (lane + 1);
+    const int len = later ? __builtin_ctzll(later) + 1 : 64 - lane;
+    atomicAdd(&counts[p], (uint32_t)len);
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_u32_kernel(int64_t len, uint32_t* __restrict__ a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) a[i] = 0u;
+}
+
+// The reference's blend, white at alpha 0.1 added per body over black and
+// clamped (glBlendFunc(GL_SRC_ALPHA, GL_ONE), colour (1, 1, 1, 0.1)),
+// quantised once: min(255, round(25.5 count)) = min(255, (51 count + 1) / 2).
+__global__ __launch_bounds__(256) void render_grey_kernel(int64_t len,
+                                                          const uint32_t* __restrict__ counts,
+                                                          uint8_t* __restrict__ grey) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) {
+    const uint32_t c = counts[i];
+    grey[i] = c >= 10u ? (uint8_t)255 : (uint8_t)((51u * c + 1u) / 2u);
+  }
+}
+
 __global__ void zero_counter_kernel(DevScalars* dev) {
   dev->interactions = 0;
   dev->quad_interactions = 0;
@@ -278,6 +330,19 @@ void launch_output(stellar_ctx* c) {
 
 void launch_zero_counter(stellar_ctx* c) {
   zero_counter_kernel<<<1, 1, 0, c->stream>>>(c->b.dev);
+}
+
+void launch_render(stellar_ctx* c, const float view[4], int32_t width, int32_t height,
+                   uint32_t* dcounts, uint8_t* dgrey) {
+  Buffers& B = c->b;
+  const int64_t px = (int64_t)width * height;
+  fill_u32_kernel<<<grid_for(px, 256), 256, 0, c->stream>>>(px, dcounts);
+  const float sx = (float)width / (view[1] - view[0]);
+  const float sy = (float)height / (view[3] - view[2]);
+  if (c->n > 0)
+    render_count_kernel<<<grid_for(c->n, 256), 256, 0, c->stream>>>(
+        c->n, B.x, B.y, view[0], view[2], sx, sy, width, height, dcounts);
+  if (dgrey) render_grey_kernel<<<grid_for(px, 256), 256, 0, c->stream>>>(px, dcounts, dgrey);
 }
 
 void launch_direct_fp64(stellar_ctx* c, int32_t nt, const int32_t* dtargets,

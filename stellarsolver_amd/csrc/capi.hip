@@ -453,6 +453,7 @@ int stellar_destroy(stellar_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graphs(c);
   free_all(c);
+  if (c->render_buf) (void)hipFree(c->render_buf);
   for (int k = 0; k < 8; ++k)
     if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
   for (hipEvent_t e : c->evrun) (void)hipEventDestroy(e);
@@ -546,6 +547,35 @@ int stellar_download_positions(stellar_ctx* c, float* out_xy) {
   HIP_TRY(c, hipGetLastError());
   HIP_TRY(c, hipMemcpyAsync(out_xy, c->b.out, 2 * (size_t)c->n * sizeof(float),
                             hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return STELLAR_OK;
+}
+
+int stellar_render(stellar_ctx* c, const float* view, int32_t width, int32_t height,
+                   uint32_t* counts, uint8_t* grey) {
+  if (!c || !view) return STELLAR_ERR_ARG;
+  if (width <= 0 || height <= 0 || (int64_t)width * height > ((int64_t)1 << 28) ||
+      !(view[1] > view[0]) || !(view[3] > view[2]))
+    return fail(c, STELLAR_ERR_ARG, "render view or size");
+  if (!c->uploaded) return fail(c, STELLAR_ERR_STATE, "no state uploaded");
+  const int64_t px = (int64_t)width * height;
+  const int64_t need = 4 * px + ((px + 255) / 256) * 256;
+  if (need > c->render_bytes) {
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->render_buf) (void)hipFree(c->render_buf);
+    c->render_buf = nullptr;
+    c->render_bytes = 0;
+    HIP_TRY(c, hipMalloc(&c->render_buf, (size_t)need));
+    c->render_bytes = need;
+  }
+  uint32_t* dcounts = static_cast<uint32_t*>(c->render_buf);
+  uint8_t* dgrey = reinterpret_cast<uint8_t*>(dcounts + px);
+  launch_render(c, view, width, height, dcounts, grey ? dgrey : nullptr);
+  HIP_TRY(c, hipGetLastError());
+  if (counts)
+    HIP_TRY(c, hipMemcpyAsync(counts, dcounts, 4 * (size_t)px, hipMemcpyDeviceToHost, c->stream));
+  if (grey)
+    HIP_TRY(c, hipMemcpyAsync(grey, dgrey, (size_t)px, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return STELLAR_OK;
 }

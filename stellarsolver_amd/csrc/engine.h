@@ -265,6 +265,9 @@ struct stellar_ctx {
   std::vector<hipEvent_t> evrun;
   int32_t evrun_steps = 0;
   float stage_ms[7] = {};
+  // stellar_render's device image (counts, then greys), grown on demand
+  void* render_buf = nullptr;
+  int64_t render_bytes = 0;
   uint8_t stage_empty = 0;  // stages of the last step that launched no kernel (bit k)
   std::string err;
 };
@@ -288,6 +291,10 @@ void launch_keys_count(stellar_ctx* c, int l1, int ibits);
 void launch_update(stellar_ctx* c);
 void launch_output(stellar_ctx* c);
 void launch_zero_counter(stellar_ctx* c);
+// Per-pixel body counts of the current positions into dcounts (width x height,
+// cleared first) and, when dgrey is given, their 8-bit greys.
+void launch_render(stellar_ctx* c, const float view[4], int32_t width, int32_t height,
+                   uint32_t* dcounts, uint8_t* dgrey);
 void launch_init_scalars(stellar_ctx* c);
 size_t build_scratch_bytes(int32_t n);
 size_t mintree_bytes(int32_t n);

@@ -122,6 +122,13 @@ void BarnesHutParticleSystem::reset() {
   step_ = 0;
 }
 
+const std::vector<uint8_t>& BarnesHutParticleSystem::renderFrame(const float view[4], int width,
+                                                                int height) {
+  frame_.resize((size_t)width * (size_t)height);
+  check(stellar_render(ctx_, view, width, height, nullptr, frame_.data()), "stellar_render");
+  return frame_;
+}
+
 float* BarnesHutParticleSystem::getOutputBuffer() {
   output_.resize(2 * (size_t)numParticles_);
   check(stellar_download_positions(ctx_, output_.data()), "stellar_download_positions");

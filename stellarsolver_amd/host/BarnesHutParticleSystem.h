@@ -36,6 +36,13 @@ class BarnesHutParticleSystem : public ParticleSystem {
   // order; valid until the next update() or destruction.
   float* getOutputBuffer() override;
 
+  // Headless form of the reference's OpenGL drawing of the output buffer
+  // (StellarSolverVisualizer.cpp:208-253): an 8-bit grey image, row 0 at the
+  // top, of the current positions through view = {left, right, bottom, top}
+  // (the reference: -25, 25, -25, 25 into 1000 x 1000), each body adding
+  // white at alpha 0.1 (stellar_render).
+  const std::vector<uint8_t>& renderFrame(const float view[4], int width, int height);
+
   // Milliseconds of the last update() (the value -benchmark prints).
   float lastStepMilliseconds() const { return last_ms_; }
   stellar_ctx* context() { return ctx_; }
@@ -54,6 +61,7 @@ class BarnesHutParticleSystem : public ParticleSystem {
   int step_;
   stellar_ctx* ctx_;
   std::vector<float> output_;
+  std::vector<uint8_t> frame_;
   float last_ms_;
 };
 

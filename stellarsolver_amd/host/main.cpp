@@ -7,13 +7,25 @@
 // -dampening= (and -barnes-hut, which the README shows and which selects
 // nothing).  Added: -numbodies= (reference: 512*256), -theta=, -eps=,
 // -group=, -workload= (seeded workload 0, 1, 2 or 4), -seed=, -quad,
-// -textbook.  As in the reference, a flag's first character is skipped
+// -textbook; -frames=DIR, -frame-every=K, -view=S, -resolution=R.
+//
+// -opengl: the reference opens a window and draws every body after every
+// step (StellarSolverVisualizer.cpp:208-253).  Headless here: every step is
+// rendered on the GPU into the reference's view (glm::ortho(-S, S, -S, S),
+// S = 25, into R x R pixels, R = 1000) as an 8-bit grey image, and with
+// -frames=DIR every K-th image is written as DIR/frame_NNNNN.pgm (-frames
+// alone implies -opengl).  As in the reference, a flag's first character is skipped
 // when matching, so "-x" and "--x"-style prefixes of one character both work.
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
 
 #include "BarnesHutParticleSystem.h"
 #include "SimulationParameters.h"
@@ -49,6 +61,16 @@ float getCmdLineParameterFloat(const int argc, char** argv, const char* s, const
   return v ? (float)std::atof(v) : def;
 }
 
+// Binary PGM (P5) of a grey image, row 0 at the top.
+void writePgm(const char* path, const std::vector<uint8_t>& img, int w, int h) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) throw std::runtime_error(std::string("cannot write ") + path);
+  std::fprintf(f, "P5\n%d %d\n255\n", w, h);
+  const size_t wrote = std::fwrite(img.data(), 1, img.size(), f);
+  std::fclose(f);
+  if (wrote != img.size()) throw std::runtime_error(std::string("short write to ") + path);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -79,9 +101,12 @@ int main(int argc, char** argv) {
   if (seed) options.seed = (uint64_t)std::strtoull(seed, nullptr, 10);
   options.quadrupole = checkCmdLineFlag(argc, argv, "quad");
   options.textbook = checkCmdLineFlag(argc, argv, "textbook");
-  if (parameters.opengl)
-    std::cerr << "note: no renderer in this build; -opengl only fetches the output buffer "
-                 "every step" << std::endl;
+  const char* frames = cmdLineValue(argc, argv, "frames");
+  if (frames) parameters.opengl = true;
+  const int frameEvery = std::max(1, getCmdLineParameterInt(argc, argv, "frame-every", 1));
+  const float viewHalf = getCmdLineParameterFloat(argc, argv, "view", 25.0f);
+  const int resolution = getCmdLineParameterInt(argc, argv, "resolution", 1000);
+  const float view[4] = {-viewHalf, viewHalf, -viewHalf, viewHalf};
   try {
     BarnesHutParticleSystem particles(parameters, numbodies, options);
     particles.reset();
@@ -89,6 +114,14 @@ int main(int argc, char** argv) {
     for (int i = 0; i < (int)parameters.iterations; i++) {
       particles.update();
       total_ms += particles.lastStepMilliseconds();
+      if (parameters.opengl) {
+        const std::vector<uint8_t>& img = particles.renderFrame(view, resolution, resolution);
+        if (frames && i % frameEvery == 0) {
+          char path[4096];
+          std::snprintf(path, sizeof path, "%s/frame_%05d.pgm", frames, i);
+          writePgm(path, img, resolution, resolution);
+        }
+      }
     }
     if (parameters.benchmark && parameters.iterations > 0)
       std::cout << "Bodies: " << numbodies << "    Steps: " << (int)parameters.iterations

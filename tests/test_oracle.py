@@ -798,3 +798,43 @@ def test_reference_lone_root_body_matches_recursive_statement(oracle, group):
     assert np.sqrt(np.mean(((ax - rx) ** 2 + (ay - ry) ** 2) / (rx ** 2 + ry ** 2))) < 5e-6
     _, _, plain = oracle.walk(kt, x, y, m, bb, 0.5, 1e-4, group)
     assert (it > plain).all()  # every group walks the root's children twice
+
+
+def test_render_known_answers(oracle):
+    """Headless point drawing (StellarSolverVisualizer.cpp:208-253): the
+    reference's glm::ortho(-25, 25, -25, 25) view of a 1000 x 1000 window,
+    20 pixels per unit; pixels are counted from the top-left corner."""
+    view = (-25.0, 25.0, -25.0, 25.0)
+    x = np.array([0.0, -25.0, 24.99, 25.0, -25.001, 0.0, 0.0, 0.0, np.nan, 0.025], np.float32)
+    y = np.array([0.0, -25.0, 24.99, 0.0, 0.0, 30.0, 0.0, 0.0, 0.0, -0.025], np.float32)
+    c = oracle.render_counts(x, y, view, 1000, 1000)
+    assert c.shape == (1000, 1000) and c.dtype == np.uint32
+    # the origin: window (500, 500) -> column 500, row 999 - 500; three bodies
+    assert c[499, 500] == 3
+    assert c[999, 0] == 1          # bottom-left corner
+    assert c[0, 999] == 1          # just inside the top-right corner
+    assert c[500, 500] == 1        # (0.025, -0.025): half a pixel right of and below the origin
+    # right edge, left of the view, above it, NaN: clipped
+    assert int(c.sum()) == 6
+    # the blend: white at alpha 0.1 per body, clamped, quantised once
+    g = oracle.render_grey(np.array([0, 1, 2, 3, 9, 10, 11, 1000], np.uint32))
+    assert g.tolist() == [0, 26, 51, 77, 230, 255, 255, 255]
+
+
+def test_render_invariants(oracle):
+    """Sum of counts = bodies inside the view; a view shifted by exactly one
+    pixel shifts the image by one column; the mirrored view of mirrored
+    bodies is the mirrored image (pixel-exact coordinates only)."""
+    rng = np.random.default_rng(7)
+    # coordinates on a 1/64 grid, so every product below is exact
+    x = (rng.integers(-2000, 2000, 20000) / 64.0).astype(np.float32)
+    y = (rng.integers(-2000, 2000, 20000) / 64.0).astype(np.float32)
+    view = (-16.0, 16.0, -8.0, 8.0)
+    c = oracle.render_counts(x, y, view, 256, 128)  # 8 pixels per unit
+    inside = (x >= -16) & (x < 16) & (y >= -8) & (y < 8)
+    assert int(c.sum()) == int(inside.sum())
+    c1 = oracle.render_counts(x, y, (-16.125, 15.875, -8.0, 8.0), 256, 128)
+    assert np.array_equal(c1[:, 1:], c[:, :-1])
+    # on the 1/64 grid, x -> -x - 1/64 maps pixel column k to 255 - k
+    cm = oracle.render_counts((-x - np.float32(1 / 64)).astype(np.float32), y, view, 256, 128)
+    assert np.array_equal(cm, c[:, ::-1])
