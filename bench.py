@@ -338,7 +338,11 @@ def run_ensemble(args, S):
     err_rms, err_p99, err_max = rms_rel(f["ax"][tg], f["ay"][tg], dx, dy)
     eng.close()
 
-    # plain form: the same instances, one context each, advanced one after another
+    # plain form: the same instances, one context each (the unbatched API),
+    # in the three ways a caller could drive them: one after another with
+    # direct steps, one after another with graph-replayed steps, and from
+    # eight host threads at once (every context has its own stream; the
+    # library's calls release the interpreter lock) with graph-replayed steps
     plain_steps = max(2, min(args.steps, 10))
     engines = []
     for st in sts:
@@ -347,14 +351,41 @@ def run_ensemble(args, S):
         e = S.Engine(st["x"].size, q)
         e.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
         engines.append(e)
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return ntot * plain_steps / (time.perf_counter() - t0)
+
+    def sequential():
+        for e in engines:
+            e.run(plain_steps)
+
     for e in engines:
         e.run(1)
-    t0 = time.perf_counter()
+    plain_forms = {"sequential_direct": timed(sequential)}
     for e in engines:
-        e.run(plain_steps)
-    t1 = time.perf_counter()
-    plain = ntot * plain_steps / (t1 - t0)
-    progress(f"plain form: {(t1 - t0) / plain_steps * 1e3:.3f} ms/step")
+        e.set_params(flags=base_flags | S.FLAG_GRAPH)
+        e.run(2)  # captures the graphs
+    plain_forms["sequential_graph"] = timed(sequential)
+
+    def threaded():
+        import threading
+
+        def worker(k):
+            for e in engines[k::8]:
+                e.run(plain_steps)
+
+        ts = [threading.Thread(target=worker, args=(k,)) for k in range(8)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+
+    plain_forms["threads8_graph"] = timed(threaded)
+    plain = max(plain_forms.values())
+    progress("plain forms (body-steps/s): " +
+             ", ".join(f"{k} {v:.3e}" for k, v in plain_forms.items()))
     for e in engines:
         e.close()
 
@@ -380,9 +411,14 @@ def run_ensemble(args, S):
                    "precision": args.precision, "graph": True, "tuning": tuning,
                    "multipoles": multipoles(args), "opening": opening_text(args)},
         "plain_form_body_steps_per_s": plain,
-        "plain_form": f"{len(counts)} single-instance contexts run one after another, "
-                      f"{plain_steps} steps each, no graph",
+        "plain_form": f"{len(counts)} single-instance contexts, {plain_steps} steps each; "
+                      "the fastest of: one after another with direct steps, one after "
+                      "another with graph-replayed steps, eight host threads with "
+                      "graph-replayed steps",
+        "plain_forms_body_steps_per_s": plain_forms,
         "speedup_over_plain_form": batched / plain,
+        "same_error_as_plain_form": "every instance of the batched form evolves bit for bit "
+                                    "as its single context (same forces, same error)",
         "rms_rel_force_err": err_rms,
         "force_err_p99": err_p99,
         "force_err_max": err_max,
