@@ -119,6 +119,34 @@ __device__ __forceinline__ int find_id(const int32_t* ids, int lo, int hi, int32
   return -1;
 }
 
+// Indices of four block ids among ids[lo, hi) (ascending, distinct), -1 for
+// an absent one: four branch-free binary searches side by side, so that each
+// step issues four LDS reads and waits once.
+__device__ __forceinline__ void find_ids4(const int32_t* ids, int lo, int hi, const int32_t key[4],
+                                          int out[4]) {
+  int base[4] = {lo, lo, lo, lo};
+  int n = hi - lo;
+  if (n <= 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = -1;
+    return;
+  }
+  while (n > 1) {
+    const int half = n >> 1;
+    int32_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = ids[base[k] + half];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) base[k] = v[k] < key[k] ? base[k] + half : base[k];
+    n -= half;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int idx = ids[base[k]] < key[k] ? base[k] + 1 : base[k];
+    out[k] = idx < hi && ids[idx] == key[k] ? idx : -1;
+  }
+}
+
 // Moments of entry i from its child slots (slot order, FP64) into L.mom[i]
 // (publish_cell writes them out): a cell child's moments come from L.mom
 // (finished in this chunk) or L.ext (before it), a body's are
@@ -129,25 +157,41 @@ template <bool EXT>
 __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i, int roff) {
   double cm[4], cmx[4], cmy[4], cqxx[4], cqxy[4], cqyy[4], cbm[4], ccx[4], ccy[4];
   bool used[4];
+  // Every LDS read of the four children is issued before any is used (one
+  // wait instead of one per child): the records and child indices first,
+  // then nine moments per slot from a row chosen without branching (a cell
+  // child's row, finished in this chunk (L.mom) or before it (L.ext); row 0,
+  // unused, for bodies and empty slots).
+  ChildRec rr[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) rr[k] = L.rec[roff + i][k];
+  const int2 cw = *reinterpret_cast<const int2*>(L.cidx[i]);
+  const int cis[4] = {(int)(int16_t)(cw.x & 0xffff), cw.x >> 16, (int)(int16_t)(cw.y & 0xffff),
+                      cw.y >> 16};
+  double mv[4][kMomL];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const ChildRec r = L.rec[roff + i][k];
+    const int ci = cis[k];
+    const double* src = (!EXT || ci >= 0 || rr[k].code <= 0) ? L.mom[ci < 0 ? 0 : ci]
+                                                             : L.ext[EXT ? i : 0][k];
+#pragma unroll
+    for (int t = 0; t < kMomL; ++t) mv[k][t] = src[t];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const ChildRec r = rr[k];
     const bool cell = r.code > 0;
     used[k] = r.code != 0;
     if (cell) {
-      // a cell child's moments: finished in this chunk (L.mom) or before it
-      // (L.ext)
-      const int ci = L.cidx[i][k];
-      const double* src = (!EXT || ci >= 0) ? L.mom[ci < 0 ? 0 : ci] : L.ext[EXT ? i : 0][k];
-      cm[k] = src[0];
-      cmx[k] = src[1];
-      cmy[k] = src[2];
-      cqxx[k] = src[3];
-      cqxy[k] = src[4];
-      cqyy[k] = src[5];
-      cbm[k] = src[6];
-      ccx[k] = src[7];
-      ccy[k] = src[8];
+      cm[k] = mv[k][0];
+      cmx[k] = mv[k][1];
+      cmy[k] = mv[k][2];
+      cqxx[k] = mv[k][3];
+      cqxy[k] = mv[k][4];
+      cqyy[k] = mv[k][5];
+      cbm[k] = mv[k][6];
+      ccx[k] = mv[k][7];
+      ccy[k] = mv[k][8];
     } else {
       // a body (an empty slot's record is all zeros and gives zeros)
       const double m = (double)r.m, x = (double)r.x, y = (double)r.y;
@@ -412,26 +456,52 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, int ne, int ro
       const int32_t pr = pb - base;
       L.pidx[i] = contiguous ? (pr >= 0 && pr < i ? pr : -1) : find_id(L.ids, 0, i, pb);
       if (L.local[i]) {
+        bool ext[4];
+        int32_t codes[4];
+        int cis[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) codes[k] = L.rec[roff + i][k].code;
+        if (contiguous) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int32_t cr = codes[k] - base;
+            cis[k] = codes[k] > 0 && cr > i && cr < ne ? cr : -1;
+          }
+        } else {
+          // bodies and empty slots (codes <= 0) match no block id
+          find_ids4(L.ids, i + 1, ne, codes, cis);
+        }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const int32_t code = L.rec[roff + i][k].code;
-          int ci = -1;
-          if (code > 0) {
-            const int32_t cr = code - base;
-            ci = contiguous ? (cr > i && cr < ne ? cr : -1) : find_id(L.ids, i + 1, ne, code);
-            if (ci < 0 && EXT) {
-              // finished by an earlier pass or a later chunk
-              const double* g = A.csum + kSumStride * (int64_t)code;
-              double* e = L.ext[EXT ? i : 0][k];
+          const int ci = codes[k] > 0 ? cis[k] : -1;
+          // finished by an earlier pass or a later chunk
+          ext[k] = EXT && codes[k] > 0 && ci < 0;
+          L.cidx[i][k] = (int16_t)ci;
+        }
+        if (EXT && (ext[0] || ext[1] || ext[2] || ext[3])) {
+          // the moments of such children, all four loads issued before any
+          // is used
+          double gv[4][kMom];
 #pragma unroll
-              for (int t = 0; t < kMom; ++t) e[t] = g[t];
-              // its centre of mass, as it computed it: sums times 1/m
-              const double inv = g[0] != 0.0 ? 1.0 / g[0] : 0.0;
-              e[7] = g[1] * inv;
-              e[8] = g[2] * inv;
+          for (int k = 0; k < 4; ++k) {
+            if (ext[k]) {
+              const double* g = A.csum + kSumStride * (int64_t)codes[k];
+#pragma unroll
+              for (int t = 0; t < kMom; ++t) gv[k][t] = g[t];
             }
           }
-          L.cidx[i][k] = (int16_t)ci;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            if (ext[k]) {
+              double* e = L.ext[EXT ? i : 0][k];
+#pragma unroll
+              for (int t = 0; t < kMom; ++t) e[t] = gv[k][t];
+              // its centre of mass, as it computed it: sums times 1/m
+              const double inv = gv[k][0] != 0.0 ? 1.0 / gv[k][0] : 0.0;
+              e[7] = gv[k][1] * inv;
+              e[8] = gv[k][2] * inv;
+            }
+          }
         }
       } else {
         left = true;
