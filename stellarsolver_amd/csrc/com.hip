@@ -82,9 +82,9 @@ template <bool EXT>
 struct ChunkLds {
   int32_t ids[kChunk];          // block ids, ascending
   int32_t pidx[kChunk];         // chunk index of the parent, -1 if none
-  int16_t bylev[kChunk];        // local entries grouped by level, deepest first
   uint8_t lev[kChunk];          // level of each entry
   uint8_t local[kChunk];        // every unfinished descendant is in the chunk
+  int16_t bylev[kChunk];        // local entries grouped by level, deepest first
   int16_t cidx[kChunk][4];      // chunk index of each cell child, -1 if none
   ChildRec rec[kChunk][4];
   union {
@@ -96,14 +96,12 @@ struct ChunkLds {
   int32_t wcnt[kComThreads / 64];
   int32_t wmin[kComThreads / 64];
   int32_t nout;
+  uint32_t levmask[2];           // levels of the local entries (bit l-1 for level l)
   int32_t lvcnt[kMaxLevel + 1];  // local entries per level (levels 1..kMaxLevel)
   int32_t lvoff[kMaxLevel + 1];  // their offset in bylev
-  uint8_t lvlist[kMaxLevel + 1]; // levels present, deepest first
-  int32_t nlev;
   int32_t blo, bhi;              // the chunk within its window
 };
 static_assert(kChunk == kComThreads, "one chunk entry per thread");
-static_assert(kChunk <= 32767, "chunk indices are int16");
 
 // Index of block id b among ids[lo, hi), or -1.
 __device__ __forceinline__ int find_id(const int32_t* ids, int lo, int hi, int32_t b) {
@@ -154,20 +152,15 @@ __device__ __forceinline__ void find_ids4(const int32_t* ids, int lo, int hi, co
 // record is all zeros, gives zeros, which leave every sum unchanged (no sum
 // here is ever -0).
 template <bool EXT>
-__device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i, int roff) {
+__device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i, const ChildRec rr[4],
+                            const int cis[4]) {
   double cm[4], cmx[4], cmy[4], cqxx[4], cqxy[4], cqyy[4], cbm[4], ccx[4], ccy[4];
   bool used[4];
-  // Every LDS read of the four children is issued before any is used (one
-  // wait instead of one per child): the records and child indices first,
-  // then nine moments per slot from a row chosen without branching (a cell
-  // child's row, finished in this chunk (L.mom) or before it (L.ext); row 0,
-  // unused, for bodies and empty slots).
-  ChildRec rr[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) rr[k] = L.rec[roff + i][k];
-  const int2 cw = *reinterpret_cast<const int2*>(L.cidx[i]);
-  const int cis[4] = {(int)(int16_t)(cw.x & 0xffff), cw.x >> 16, (int)(int16_t)(cw.y & 0xffff),
-                      cw.y >> 16};
+  // The entry's child records and chunk indices come in registers; the nine
+  // moments of every slot are read before any is used (one wait), each from
+  // a row chosen without branching (a cell child's row, finished in this
+  // chunk (L.mom) or before it (L.ext); row 0, unused, for bodies and empty
+  // slots).
   double mv[4][kMomL];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -420,6 +413,7 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, int ne, int ro
   int sl = 0;
   int rank = -1;   // rank among the leftovers, -1 if finished here
   float4 bb = make_float4(0.f, 0.f, 0.f, 0.f);
+
   {
     int lev = 255;
     if (i < ne) {
@@ -521,18 +515,24 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, int ne, int ro
       for (int w = 0; w < kComThreads / 64; ++w) tot += L.wcnt[w];
       L.nout = tot;
     }
+    if (threadIdx.x < 2) L.levmask[threadIdx.x] = 0u;
     for (int l = threadIdx.x; l <= kMaxLevel; l += kComThreads) L.lvcnt[l] = 0;
   }
   __syncthreads();
-  // group the local entries by level (counting sort in LDS)
-  int rk = 0;
+  // group the local entries by level, deepest first (counting sort in LDS),
+  // and note the levels present
   const bool fin = i < ne && L.local[i];
-  if (fin) rk = atomicAdd(&L.lvcnt[L.lev[i]], 1);
+  const int mylev = fin ? (int)L.lev[i] : 0;
+  int rk = 0;
+  if (fin) {
+    rk = atomicAdd(&L.lvcnt[mylev], 1);
+    atomicOr(&L.levmask[(mylev - 1) >> 5], 1u << ((mylev - 1) & 31));
+  }
   __syncthreads();
   static_assert(kMaxLevel == 64, "one lane per cell level 1..64");
   if (threadIdx.x < 64) {
     // lane j holds level 64 - j (cells have levels 1..64): offsets are an
-    // exclusive scan in lane order, the present levels a ballot
+    // exclusive scan in lane order
     const int l = kMaxLevel - lane;
     const int c = L.lvcnt[l];
     int incl = c;
@@ -541,19 +541,32 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, int ne, int ro
       if (lane >= off) incl += o;
     }
     L.lvoff[l] = incl - c;
-    const uint64_t present = __builtin_amdgcn_ballot_w64(c > 0);
-    if (c > 0) L.lvlist[__popcll(present & ((1ull << lane) - 1ull))] = (uint8_t)l;
-    if (lane == 0) L.nlev = __popcll(present);
+    if (lane == 63) L.lvcnt[0] = incl;  // every local entry
   }
   __syncthreads();
-  if (fin) L.bylev[L.lvoff[L.lev[i]] + rk] = (int16_t)i;
+  if (fin) L.bylev[L.lvoff[mylev] + rk] = (int16_t)i;
   __syncthreads();
+  // Thread t finishes the t-th entry of that order at its level's step, so
+  // that the cells of one level sit on consecutive threads (few wavefronts
+  // per step); its child records and child indices are read once, here.
+  int e = -1, le = 0;
+  ChildRec rr[4] = {};
+  int cs[4] = {-1, -1, -1, -1};
+  if (i < L.lvcnt[0]) {
+    e = L.bylev[i];
+    le = L.lev[e];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      rr[k] = L.rec[roff + e][k];
+      cs[k] = L.cidx[e][k];
+    }
+  }
+  uint64_t levs = (uint64_t)L.levmask[0] | ((uint64_t)L.levmask[1] << 32);
   // deepest level first: every child of a cell is then finished
-  const int nl = L.nlev;
-  for (int k = 0; k < nl; ++k) {
-    const int l = L.lvlist[k];
-    const int cnt = L.lvcnt[l], off = L.lvoff[l];
-    for (int t = threadIdx.x; t < cnt; t += kComThreads) finish_cell<EXT>(A, L, L.bylev[off + t], roff);
+  while (levs) {
+    const int l = 64 - __clzll(levs);
+    levs &= ~(1ull << (l - 1));
+    if (e >= 0 && le == l) finish_cell<EXT>(A, L, e, rr, cs);
     __syncthreads();
   }
   if (fin) publish_cell<EXT>(A, L, i, L.ids[i], pb, sl, bb);
