@@ -379,21 +379,23 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
         const float dy = __int_as_float(q[jj][k].y) - py;
         const float cmv = __int_as_float(q[jj][k].z);
         const float r = dist2(dx, dy, eps2);
-        // one vote for bodies and cells: a body always passes (threshold -1
-        // against r >= eps2 > 0), so acceptance is a single scalar test
-        // (a body's slot in the second block is not written: the code decides)
-        const int32_t ti = code < 0 ? __float_as_int(-1.0f)
-                                    : (kQ ? __float_as_int(qm[kQ ? jj : 0][k].w)
-                                          : __float_as_int(dp[jj]));
+        // a body is always accepted (a scalar branch on its code, no vote;
+        // its slot in the second block is not written)
+        if (code < 0) {
+          acc.mono(cmv, r, dx, dy);
+          continue;
+        }
+        // a cell: the group's vote against its threshold
+        const int32_t ti = kQ ? __float_as_int(qm[kQ ? jj : 0][k].w) : __float_as_int(dp[jj]);
         const float t = __int_as_float(__builtin_amdgcn_readfirstlane(ti));
         if (__builtin_expect(__builtin_amdgcn_ballot_w64(!(t <= r)) == 0, 1)) {
-          if (MIXED && code > 0) {
+          if (MIXED) {
             ++acc.cnt;
             ++acc.qcnt;
             const float4 q = qm[kQ ? jj : 0][k];
             mix.cell(cmv, r, dx, dy, ws2, __float_as_uint(q.x), __float_as_uint(q.y));
           } else {
-            acc.term(QUAD && code > 0, cmv, r, dx, dy, qm[kQ ? jj : 0][k]);
+            acc.term(QUAD, cmv, r, dx, dy, qm[kQ ? jj : 0][k]);
           }
         } else {
           if (lane == 0 && sp < STACK) st[sp] = (uint32_t)code | child_lvl;
