@@ -443,8 +443,11 @@ void or_com_exact(int32_t n, int64_t ncells, const int32_t* child,
  *     than about theta*eps passes -- including the cell that holds the body
  *     itself, whose centre of mass then stands in for the body's neighbours.)
  *   cthr != NULL: a cell's own threshold (or_bmax_exact) replaces dp.
- *   cq != NULL: accepted cells add their quadrupole term
- *     a += -Q.d/r^5 + (5/2)(d.Q.d) d/r^7, r^2 = |d|^2 + eps2. */
+ *   cq != NULL: accepted cells add their quadrupole term, the second-order
+ *     term of the softened potential's expansion about the cell's centre of
+ *     mass: a += -Q.d/r^5 + (5/2)(d.Q.d - eps2 tr Q) d/r^7, r^2 = |d|^2 +
+ *     eps2 (the eps2 tr Q part is what softening adds to the unsoftened
+ *     form). */
 #define OR_STACK 256
 
 static inline float depth0_of(const float* bbox, float theta, int opening) {
@@ -464,11 +467,15 @@ static inline float dist2(float dx, float dy, float eps2) {
 }
 
 /* FP32 accumulation of one accepted node for one body.  With q (a cell's
- * quadrupole) the three terms m d/r^3 - Q.d/r^5 + (5/2)(d.Q.d) d/r^7 are
- * evaluated as rs^5 [ (m r + 2.5 (d.Q.d) rs^2) d - Q.d ], r = |d|^2 + eps2,
- * rs = 1/sqrt(r): the form the device walk uses. */
+ * quadrupole, tr Q = q[0] + q[2]) the terms
+ *   m d/r^3 - Q.d/r^5 + (5/2)(d.Q.d - eps2 tr Q) d/r^7
+ * -- the expansion of the softened sum m_i / sqrt(|d + s_i|^2 + eps2) to
+ * second order in the offsets s_i -- are evaluated as
+ *   rs^5 [ (m r + (2.5 (d.Q.d) + T) rs^2) d - Q.d ],
+ * r = |d|^2 + eps2, rs = 1/sqrt(r), T = (q[0] + q[2]) kq with kq = -2.5 eps2
+ * (FP32): the form the device walk uses. */
 static inline void acc_node(float* fx, float* fy, float dx, float dy, float r,
-                            float cmv, const float* q) {
+                            float cmv, const float* q, float kq) {
   const float rs = rsqrt_ref(r);
   if (!q) {
     const float f = cmv * rs * rs * rs;
@@ -481,7 +488,8 @@ static inline void acc_node(float* fx, float* fy, float dx, float dy, float r,
   const float px = fmaf(q[0], dx, q[1] * dy);
   const float py = fmaf(q[1], dx, q[2] * dy);
   const float s = fmaf(dx, px, dy * py);
-  const float c = fmaf(2.5f * s, u, cmv * r);
+  const float tq = (q[0] + q[2]) * kq; /* -2.5 eps2 tr Q */
+  const float c = fmaf(fmaf(s, 2.5f, tq), u, cmv * r);
   *fx = fmaf(rs5, fmaf(c, dx, -px), *fx);
   *fy = fmaf(rs5, fmaf(c, dy, -py), *fy);
 }
@@ -512,6 +520,7 @@ static inline double hfma(double a, double b, double c) { return h16(fma(a, b, c
 typedef struct mixacc {
   float sx, sy, cx, cy;     /* Kahan sums */
   float pex, pey, pk, pt;   /* the held cell */
+  float peu;
 } mixacc;
 
 static inline void kahan(float* sum, float* comp, float v) {
@@ -523,19 +532,22 @@ static inline void kahan(float* sum, float* comp, float v) {
 
 /* Two accepted cells (a: held, b: current) for one body. */
 static void mix_pair(mixacc* M, const float* qa, float ex, float ey, float k, float t,
-                     const float* qb) {
+                     float eu, const float* qb) {
   const double EXa = f16_round(M->pex, 1), EXb = f16_round(ex, 1);
   const double EYa = f16_round(M->pey, 1), EYb = f16_round(ey, 1);
   const double Ta = f16_round(M->pt, 1), Tb = f16_round(t, 1);
+  const double EUa = f16_round(M->peu, 1), EUb = f16_round(eu, 1);
   double BX[2], BY[2];
-  const double EX[2] = {EXa, EXb}, EY[2] = {EYa, EYb}, T[2] = {Ta, Tb};
+  const double EX[2] = {EXa, EXb}, EY[2] = {EYa, EYb}, T[2] = {Ta, Tb}, EU[2] = {EUa, EUb};
   const float* Q[2] = {qa, qb};
   for (int h = 0; h < 2; ++h) {
     const double qxx = Q[h][0], qxy = Q[h][1], qyy = Q[h][2];
     const double qex = hfma(qxx, EX[h], hmul(qxy, EY[h]));
     const double qey = hfma(qxy, EX[h], hmul(qyy, EY[h]));
     const double s = hfma(EX[h], qex, hmul(EY[h], qey));
-    const double cc = hmul(s, 2.5);
+    /* softening: e.Qh.e - tr(Qh) eps2 rs^2 */
+    const double trh = h16(qxx + qyy);
+    const double cc = hmul(hfma(-trh, EU[h], s), 2.5);
     BX[h] = hfma(T[h], hfma(cc, EX[h], -qex), EX[h]);
     BY[h] = hfma(T[h], hfma(cc, EY[h], -qey), EY[h]);
   }
@@ -607,7 +619,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
           for (int32_t j = 0; j < gs; ++j) {
             const float dx = x[ch] - px[j], dy = y[ch] - py[j];
             const float r = dist2(dx, dy, eps2);
-            acc_node(&fx[j], &fy[j], dx, dy, r, mass[ch], NULL);
+            acc_node(&fx[j], &fy[j], dx, dy, r, mass[ch], NULL, 0.f);
           }
           ++cnt;
         } else {
@@ -650,13 +662,15 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
               const float rs = rsqrt_ref(rr[j]);
               const float u = rs * rs;
               const float ex = dx * rs, ey = dy * rs, k = cmv * u, t = u * ws2;
+              const float eu = eps2 * u;
               if (pend) {
-                mix_pair(&mx[j], pq, ex, ey, k, t, qc);
+                mix_pair(&mx[j], pq, ex, ey, k, t, eu, qc);
               } else {
                 mx[j].pex = ex;
                 mx[j].pey = ey;
                 mx[j].pk = k;
                 mx[j].pt = t;
+                mx[j].peu = eu;
               }
             }
             if (pend) {
@@ -669,7 +683,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
           } else if (ch < n || all) {
             for (int32_t j = 0; j < gs; ++j) {
               const float dx = cxv - px[j], dy = cyv - py[j];
-              acc_node(&fx[j], &fy[j], dx, dy, rr[j], cmv, q);
+              acc_node(&fx[j], &fy[j], dx, dy, rr[j], cmv, q, -2.5f * eps2);
             }
             ++cnt;
           } else {
@@ -686,7 +700,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
       }
       if (mixed) {
         for (int32_t j = 0; j < gs; ++j) {
-          if (pend) mix_pair(&mx[j], pq, 0.f, 0.f, 0.f, 0.f, qzero);
+          if (pend) mix_pair(&mx[j], pq, 0.f, 0.f, 0.f, 0.f, 0.f, qzero);
           fx[j] = fx[j] + mx[j].sx;
           fy[j] = fy[j] + mx[j].sy;
         }
@@ -729,7 +743,7 @@ static void rec_acc(recctx* R, float cxv, float cyv, float cmv, const float* q) 
       const double rs2 = rs * rs, rs5 = rs2 * rs2 * rs, rs7 = rs5 * rs2;
       const double qdx = (double)q[0] * X + (double)q[1] * Y;
       const double qdy = (double)q[1] * X + (double)q[2] * Y;
-      const double dqd = X * qdx + Y * qdy;
+      const double dqd = X * qdx + Y * qdy - ((double)q[0] + (double)q[2]) * (double)R->eps2;
       gx += -rs5 * qdx + 2.5 * dqd * rs7 * X;
       gy += -rs5 * qdy + 2.5 * dqd * rs7 * Y;
     }

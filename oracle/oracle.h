@@ -102,7 +102,9 @@ typedef struct or_walk_params {
                         1: w^2/theta^2 + eps2 <= r (textbook, unsoftened d) */
   int32_t mixed;     /* 1: accepted cells' terms in FP16 (see or_quad_mixed),
                         needs cqh and clevel */
-  const float* cq;   /* NULL: monopole; else 3 quadrupole terms per cell */
+  const float* cq;   /* NULL: monopole; else 3 quadrupole terms per cell
+                        (Qxx, Qxy, Qyy), applied with the softened form
+                        -Q.d/r^5 + (5/2)(d.Q.d - eps2 tr Q) d/r^7 */
   const float* cthr; /* NULL: per-level thresholds; else one per cell
                         (or_bmax_exact), compared with r like dp */
   const float* cqh;  /* mixed: Qh per cell (3 FP16 values, or_quad_mixed) */
@@ -121,8 +123,11 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
             float* ax, float* ay, int32_t* inter);
 
 /* Mixed precision (not in the reference).  An accepted cell's term
- * m d/r^3 - Q.d/r^5 + (5/2)(d.Q.d) d/r^7 is k B with k = m rs^2 (FP32) and
- *   B = e + t (-Qh.e + (5/2)(e.Qh.e) e),  e = d rs, t = rs^2 ws^2,
+ * m d/r^3 - Q.d/r^5 + (5/2)(d.Q.d - eps2 tr Q) d/r^7 is k B with k = m rs^2
+ * (FP32) and
+ *   B = e + t (-Qh.e + (5/2)(e.Qh.e - tr(Qh) eu) e),  e = d rs, t = rs^2 ws^2,
+ * eu = eps2 rs^2 (FP32 product, converted toward zero like e and t),
+ * tr(Qh) = Qxx + Qyy in FP16,
  * rs = 1/sqrt(|d|^2 + eps2), ws^2 = ldexpf(S*S, -2 L) for a cell at level L
  * (S the larger extent of the box, FP32), Qh = Q / (m ws^2) rounded to FP32
  * and then to FP16 (nearest even, clamped to +-65504).  B is evaluated in

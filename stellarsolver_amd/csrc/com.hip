@@ -259,13 +259,19 @@ __device__ void publish_cell(const ComArgs& A, const ChunkLds<EXT>& L, int i, in
   const double* o = L.mom[i];
   A.blk[4 * (int64_t)p + s] = ChildRec{(float)o[7], (float)o[8], (float)o[0], b};
   if (A.quad || A.bmax) {
-    // second block: the moments the walk needs and the cell's opening
-    // threshold (bb: the box of the cell's instance), one 16-byte store
+    // second block: the moments the walk needs and, with per-cell opening
+    // thresholds (bmax) or mixed precision, the cell's threshold (bb: the box
+    // of the cell's instance), one 16-byte store; an FP32 quadrupole walk
+    // with level thresholds (it reads those from a per-level table) finds
+    // the quadrupole's softening term -2.5 eps2 (Qxx + Qyy) there instead
     const int lev = L.lev[i];
-    const float thr =
-        A.bmax ? (float)((o[6] * o[6]) / A.t2 + A.eps2)
-               : walk_level_threshold(walk_root_depth(bb, A.theta_div), lev, A.eps2f, A.unsoft != 0);
-    float4 q = make_float4((float)o[3], (float)o[4], (float)o[5], thr);
+    float4 q = make_float4((float)o[3], (float)o[4], (float)o[5], 0.f);
+    if (A.bmax || A.mixed)
+      q.w = A.bmax ? (float)((o[6] * o[6]) / A.t2 + A.eps2)
+                   : walk_level_threshold(walk_root_depth(bb, A.theta_div), lev, A.eps2f,
+                                          A.unsoft != 0);
+    else
+      q.w = (q.x + q.z) * (-2.5f * A.eps2f);
     if (A.mixed) {
       // Qh of a cell at level lev: Q / (m ws^2), ws^2 = S^2 / 4^lev in FP32
       const float w = bb.y - bb.x, h = bb.w - bb.z;

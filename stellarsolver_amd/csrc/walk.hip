@@ -35,7 +35,12 @@
 // root's children one level down.
 //
 // QUAD adds each accepted cell's quadrupole term (a second 64-byte block of
-// {Qxx, Qxy, Qyy, threshold} per node); bodies stay monopoles.
+// {Qxx, Qxy, Qyy, w} per node); bodies stay monopoles.  The term is the
+// second-order term of the softened potential sum m / sqrt(|d + s|^2 + eps2)
+// (s a body's offset from the cell's centre of mass), which carries, beside
+// the unsoftened form's two terms, -(5/2) tr(Q) eps2 d / r^7: w holds
+// T = -2.5 eps2 (Qxx + Qyy) for it.  With per-cell opening thresholds (bmax)
+// w holds the threshold instead and T is computed from Q in the walk.
 #include <map>
 #include <mutex>
 
@@ -109,6 +114,10 @@ __device__ __forceinline__ float dist2(float dx, float dy, float eps2) {
   return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, eps2));
 }
 
+// The softening term T = (Qxx + Qyy) kq, kq = -2.5 eps2 (FP32; the
+// centre-of-mass pass stores the same product).
+__device__ __forceinline__ float quad_soft_term(float4 q, float kq) { return (q.x + q.z) * kq; }
+
 struct Acc {
   float fx = 0.f, fy = 0.f;
   int32_t cnt = 0, qcnt = 0;
@@ -119,19 +128,23 @@ struct Acc {
     fy = __builtin_fmaf(f, dy, fy);
     ++cnt;
   }
-  // Monopole plus quadrupole of an accepted cell:
-  //   a = m d/r^3 - Q.d/r^5 + (5/2)(d.Q.d) d/r^7
-  //     = rs^5 [ (m r^2 + 2.5 (d.Q.d) rs^2) d - Q.d ],  rs = 1/r, r^2 = |d|^2+eps2
-  // (20 issue slots with the rsqrt instead of 27 for the three terms apart).
+  // Monopole plus quadrupole of an accepted cell (Q the traceless in-plane
+  // moments, tr Q = Qxx + Qyy = sum m |s|^2):
+  //   a = m d/r^3 - Q.d/r^5 + (5/2)(d.Q.d - tr Q eps2) d/r^7
+  //     = rs^5 [ (m r^2 + (2.5 (d.Q.d) + T) rs^2) d - Q.d ],
+  //   rs = 1/r, r^2 = |d|^2 + eps2, T = -2.5 eps2 tr Q
+  // (20 issue slots with the rsqrt instead of 27 for the terms apart).
+  // c25 = 2.5 held in a VGPR (see walk64_group), so that the multiply-add
+  // 2.5 s + T takes T from a scalar register without a copy
   __device__ __forceinline__ void quad(float cm, float r, float dx, float dy,
-                                       float4 q) {
+                                       float4 q, float tq, float c25) {
     const float rs = __builtin_amdgcn_rsqf(r);
     const float u = rs * rs;
     const float rs5 = (u * u) * rs;
     const float px = __builtin_fmaf(q.x, dx, q.y * dy);  // Q.d
     const float py = __builtin_fmaf(q.y, dx, q.z * dy);
     const float s = __builtin_fmaf(dx, px, dy * py);      // d.Q.d
-    const float c = __builtin_fmaf(2.5f * s, u, cm * r);
+    const float c = __builtin_fmaf(__builtin_fmaf(s, c25, tq), u, cm * r);
     fx = __builtin_fmaf(rs5, __builtin_fmaf(c, dx, -px), fx);
     fy = __builtin_fmaf(rs5, __builtin_fmaf(c, dy, -py), fy);
     ++cnt;
@@ -142,7 +155,7 @@ struct Acc {
   // (a, bx, by) and add a*bx, a*by, so the results stay in place instead of
   // being copied out of one side (the same arithmetic as mono / quad).
   __device__ __forceinline__ void term(bool with_quad, float cm, float r,
-                                       float dx, float dy, float4 q) {
+                                       float dx, float dy, float4 q, float tq, float c25) {
     const float rs = __builtin_amdgcn_rsqf(r);
     float a, bx, by;
     if (with_quad) {
@@ -151,7 +164,7 @@ struct Acc {
       const float px = __builtin_fmaf(q.x, dx, q.y * dy);
       const float py = __builtin_fmaf(q.y, dx, q.z * dy);
       const float s = __builtin_fmaf(dx, px, dy * py);
-      const float c = __builtin_fmaf(2.5f * s, u, cm * r);
+      const float c = __builtin_fmaf(__builtin_fmaf(s, c25, tq), u, cm * r);
       bx = __builtin_fmaf(c, dx, -px);
       by = __builtin_fmaf(c, dy, -py);
       ++qcnt;
@@ -168,9 +181,10 @@ struct Acc {
 
 // ------------------------------------------------ mixed-precision far field --
 // STELLAR_PRECISION_MIXED: an accepted cell's term m d/r^3 - Q.d/r^5 +
-// (5/2)(d.Q.d) d/r^7 is written as k B with k = m rs^2 (FP32) and the
-// dimensionless bracket
-//   B = e + t (-Qh.e + (5/2)(e.Qh.e) e),  e = d rs, t = rs^2 ws^2,
+// (5/2)(d.Q.d - eps2 tr Q) d/r^7 is written as k B with k = m rs^2 (FP32)
+// and the dimensionless bracket
+//   B = e + t (-Qh.e + (5/2)(e.Qh.e - tr(Qh) eu) e),  e = d rs, t = rs^2 ws^2,
+// eu = eps2 rs^2 (the softening term, tr(Qh) = Qxx + Qyy in FP16),
 // where rs = 1/sqrt(|d|^2 + eps2), ws = S / 2^L is the cell's size at level L
 // (S the larger extent of the instance's box) and Qh = Q / (m ws^2) is stored
 // by the centre-of-mass pass as three FP16 values.  |e| <= 1 and t, |Qh| are
@@ -185,7 +199,7 @@ typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 struct MixedAcc {
   float sx = 0.f, sy = 0.f, cx = 0.f, cy = 0.f;  // compensated far-field sums
   bool pend = false;
-  float pex = 0.f, pey = 0.f, pk = 0.f, pt = 0.f;  // the held cell (per lane)
+  float pex = 0.f, pey = 0.f, pk = 0.f, pt = 0.f, peu = 0.f;  // the held cell (per lane)
   uint32_t pq0 = 0u, pq1 = 0u;                     // its Qh halves (uniform)
 
   __device__ __forceinline__ static void kahan(float& sum, float& comp, float v) {
@@ -195,18 +209,20 @@ struct MixedAcc {
     sum = t;
   }
   // q0 = Qxx | Qxy << 16, q1 = Qyy (FP16 bits)
-  __device__ __forceinline__ void pair(float ex, float ey, float k, float t, uint32_t q0,
-                                       uint32_t q1) {
+  __device__ __forceinline__ void pair(float ex, float ey, float k, float t, float eu,
+                                       uint32_t q0, uint32_t q1) {
     const half2_t EX = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(pex, ex));
     const half2_t EY = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(pey, ey));
     const half2_t T = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(pt, t));
+    const half2_t EU = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(peu, eu));
     const half2_t QXX = __builtin_bit_cast(half2_t, (pq0 & 0xffffu) | (q0 << 16));
     const half2_t QXY = __builtin_bit_cast(half2_t, (pq0 >> 16) | (q0 & 0xffff0000u));
     const half2_t QYY = __builtin_bit_cast(half2_t, (pq1 & 0xffffu) | (q1 << 16));
     const half2_t QEX = __builtin_elementwise_fma(QXX, EX, QXY * EY);
     const half2_t QEY = __builtin_elementwise_fma(QXY, EX, QYY * EY);
     const half2_t S = __builtin_elementwise_fma(EX, QEX, EY * QEY);
-    const half2_t C = S * (half2_t)(_Float16)2.5f;
+    const half2_t TRH = QXX + QYY;
+    const half2_t C = __builtin_elementwise_fma(-TRH, EU, S) * (half2_t)(_Float16)2.5f;
     const half2_t BX = __builtin_elementwise_fma(T, __builtin_elementwise_fma(C, EX, -QEX), EX);
     const half2_t BY = __builtin_elementwise_fma(T, __builtin_elementwise_fma(C, EY, -QEY), EY);
     kahan(sx, cx, __builtin_fmaf(pk, (float)BX.x, k * (float)BX.y));
@@ -214,25 +230,26 @@ struct MixedAcc {
   }
   // One accepted cell (r = |d|^2 + eps2, ws2 = ws^2 of its level, q0/q1 its Qh).
   __device__ __forceinline__ void cell(float cm, float r, float dx, float dy, float ws2,
-                                       uint32_t q0, uint32_t q1) {
+                                       float eps2, uint32_t q0, uint32_t q1) {
     const float rs = __builtin_amdgcn_rsqf(r);
     const float u = rs * rs;
-    const float ex = dx * rs, ey = dy * rs, k = cm * u, t = u * ws2;
+    const float ex = dx * rs, ey = dy * rs, k = cm * u, t = u * ws2, eu = eps2 * u;
     if (pend) {
-      pair(ex, ey, k, t, q0, q1);
+      pair(ex, ey, k, t, eu, q0, q1);
       pend = false;
     } else {
       pex = ex;
       pey = ey;
       pk = k;
       pt = t;
+      peu = eu;
       pq0 = q0;
       pq1 = q1;
       pend = true;
     }
   }
   __device__ __forceinline__ void flush() {
-    if (pend) pair(0.f, 0.f, 0.f, 0.f, 0u, 0u);
+    if (pend) pair(0.f, 0.f, 0.f, 0.f, 0.f, 0u, 0u);
     pend = false;
   }
 };
@@ -271,14 +288,22 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
   const float px = A.x[sl];
   const float py = A.y[sl];
 
-  // A node's second block (quadrupoles, bmax) carries each child's threshold
-  // (a cell's level threshold or bmax threshold, -1 for a body); otherwise
+  // With per-cell thresholds (bmax) or mixed precision, a node's second block
+  // carries each cell child's threshold; otherwise
   // thr(level) = threshold for the children of a node at `level`:
   // 0.25*depth(level), depth(1) = depth0, depth(l+1) = 0.25*depth(l)
-  // (kernels.cu:394-395), one level per lane.
+  // (kernels.cu:394-395), one level per lane (read with v_readlane once per
+  // visited node: every child of a node has the same level).
   constexpr bool kQ = QUAD || THR;  // the node's second block is needed
+  constexpr bool kTable = !THR && !MIXED;  // thresholds from the per-lane table
+  const float kq = -2.5f * eps2;            // quadrupole softening factor
+  // 2.5 as an opaque vector value: a literal would make the compiler use
+  // v_fmamk, which cannot read T from a scalar register (one extra copy per
+  // accepted cell)
+  float c25 = 2.5f;
+  asm volatile("" : "+v"(c25));
   float thr = 0.f;
-  if (!kQ) {
+  if (kTable) {
     thr = walk_root_depth(A.ibbox[inst], A.theta_div);
     for (int i = 0; i <= lane; ++i) thr = 0.25f * thr;
     // textbook test on the unsoftened distance: dp + eps2 <= d^2 + eps2
@@ -339,8 +364,8 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
           __builtin_amdgcn_readfirstlane(st[sp + (j < take ? j : 0)]);
       node[j] = w & kIdxMask;
       lvl[j] = w >> kIdxBits;  // level - 1
-      dp[j] = kQ ? 0.f
-                 : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr), (int)lvl[j]));
+      dp[j] = !kTable ? 0.f
+                      : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr), (int)lvl[j]));
     }
     int4 q[KPOP][4];
     float4 qm[kQ ? KPOP : 1][4];
@@ -374,28 +399,28 @@ __device__ __forceinline__ void walk64_group(const ChildRec* __restrict__ blk,
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int32_t code = q[jj][k].w;
-        if (code == 0) continue;
         const float dx = __int_as_float(q[jj][k].x) - px;
         const float dy = __int_as_float(q[jj][k].y) - py;
         const float cmv = __int_as_float(q[jj][k].z);
-        const float r = dist2(dx, dy, eps2);
         // a body is always accepted (a scalar branch on its code, no vote;
-        // its slot in the second block is not written)
-        if (code < 0) {
-          acc.mono(cmv, r, dx, dy);
+        // its slot in the second block is not written); code 0 is empty
+        if (code <= 0) {
+          if (code < 0) acc.mono(cmv, dist2(dx, dy, eps2), dx, dy);
           continue;
         }
+        const float r = dist2(dx, dy, eps2);
         // a cell: the group's vote against its threshold
-        const int32_t ti = kQ ? __float_as_int(qm[kQ ? jj : 0][k].w) : __float_as_int(dp[jj]);
+        const int32_t ti = kTable ? __float_as_int(dp[jj]) : __float_as_int(qm[kQ ? jj : 0][k].w);
         const float t = __int_as_float(__builtin_amdgcn_readfirstlane(ti));
         if (__builtin_expect(__builtin_amdgcn_ballot_w64(!(t <= r)) == 0, 1)) {
           if (MIXED) {
             ++acc.cnt;
             ++acc.qcnt;
             const float4 q = qm[kQ ? jj : 0][k];
-            mix.cell(cmv, r, dx, dy, ws2, __float_as_uint(q.x), __float_as_uint(q.y));
+            mix.cell(cmv, r, dx, dy, ws2, eps2, __float_as_uint(q.x), __float_as_uint(q.y));
           } else {
-            acc.term(QUAD, cmv, r, dx, dy, qm[kQ ? jj : 0][k]);
+            const float4 qc = qm[kQ ? jj : 0][k];
+            acc.term(QUAD, cmv, r, dx, dy, qc, THR ? quad_soft_term(qc, kq) : qc.w, c25);
           }
         } else {
           if (lane == 0 && sp < STACK) st[sp] = (uint32_t)code | child_lvl;
@@ -609,7 +634,8 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(
       const uint64_t bad = __ballot(valid && !(th <= r)) & gmask;
       if (bad == 0) {
         if (QUAD)
-          acc.quad(cmv, r, dx, dy, qm[k]);
+          acc.quad(cmv, r, dx, dy, qm[k], THR ? quad_soft_term(qm[k], -2.5f * eps2) : qm[k].w,
+                   2.5f);
         else
           acc.mono(cmv, r, dx, dy);
       } else {

@@ -535,6 +535,42 @@ def test_quadrupole_improves_far_field(oracle):
     assert errs[1] < 0.2 * errs[0]
 
 
+def test_softened_quadrupole_is_the_second_order_term(oracle):
+    """A symmetric pair (no octupole) seen by a lone body at a distance
+    comparable to the softening length: the walk's monopole + quadrupole term
+    equals the softened direct sum m_i / sqrt(|d + s_i|^2 + eps2) up to the
+    fourth-order residual, in FP32 and in the FP64 recursive statement.  The
+    unsoftened quadrupole form (-Q.d/r^5 + 2.5 (d.Q.d) d/r^7 with softened r,
+    from the closed-form Q of the pair) misses by the eps2 tr Q term, 400
+    times more here, so a dropped, mis-signed or mis-scaled softening term
+    fails."""
+    half, ang = 0.5, 0.7
+    c = np.array([8.0, 6.0])
+    u = np.array([np.cos(ang), np.sin(ang)])
+    pts = np.array([[0.0, 0.0], c + half * u, c - half * u], np.float32)
+    x, y = pts[:, 0].copy(), pts[:, 1].copy()
+    m = np.ones(3, np.float32)
+    eps2 = 25.0
+    bb, kt = _tree(oracle, {"x": x, "y": y, "mass": m})
+    oracle.quad_exact(kt, x, y, m)
+    ex, ey = oracle.direct_fp64(x, y, m, eps2, targets=np.array([0], np.int32))
+    ref = np.array([ex[0], ey[0]])
+    # closed form of the pair's expansion about its centre of mass
+    s2 = 2 * half * half  # sum m |s|^2 = tr Q
+    q = 3 * s2 * np.outer(u, u) - s2 * np.eye(2)
+    r2 = c @ c + eps2
+    unsoft = 2 * c / r2 ** 1.5 - q @ c / r2 ** 2.5 + 2.5 * (c @ q @ c) * c / r2 ** 3.5
+    soft = unsoft - 2.5 * s2 * eps2 * c / r2 ** 3.5
+    rel = lambda a: float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
+    assert rel(unsoft) > 5e-4 and rel(soft) < 5e-6
+    args = (kt, x, y, m, bb, 1.0, eps2, 1)
+    ax, ay, it = oracle.walk(*args, opening=1, quad=True)
+    assert it[0] == 2  # itself (zero force) and the pair's cell
+    assert rel(np.array([ax[0], ay[0]])) < 1e-5
+    rx, ry, rit = oracle.walk_recursive(*args, opening=1, quad=True)
+    assert rit[0] == 2 and rel(np.array([rx[0], ry[0]])) < 5e-6
+
+
 @pytest.mark.parametrize("opening", [0, 1])
 def test_quad_walk_matches_recursive_statement(oracle, opening):
     st = ic(2, 20000)
