@@ -39,10 +39,11 @@ VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 # d^2+eps^2, 1 rsqrt at quarter rate (4 slots), 3 mul, 2 FMA accumulate
 OPS_PER_INTERACTION = 13
 # a cell's quadrupole term, in the combined form the walk evaluates
-# (rs^5 [(m r + 2.5 (d.Q.d) rs^2) d - Q.d]), issues 11 more than the monopole's
-# 3 multiplies (already in the 13): rs^2, rs^5 (2), Q.d (2 mul + 2 FMA),
-# d.Q.d (mul + FMA), the radial factor (2 mul + FMA), one FMA per component
-# (2), less the monopole's 3 multiplies
+# (rs^5 [(m r + (2.5 (d.Q.d) + T) rs^2) d - Q.d], T the softening term read
+# with the moments), issues 11 more than the monopole's 3 multiplies (already
+# in the 13): rs^2, rs^5 (2), Q.d (2 mul + 2 FMA), d.Q.d (mul + FMA), the
+# radial factor (mul + 2 FMA), one FMA per component (2), less the
+# monopole's 3 multiplies
 OPS_PER_QUAD_TERM = 11
 HBM_PEAK_GBS = 8000.0
 # Counter figures of the default workload (colliding disks, N = 2^20, theta 0.5,
@@ -54,13 +55,13 @@ DEFAULT_KEY = (2, 1 << 20, 0.5, 64, "quad", "textbook")
 # device copy; "forces" is the walk launch (with the update inside it).
 STAGE_TRAFFIC_BYTES = {DEFAULT_KEY: {
     "bbox": (4125 * 2 + 8) * 1024,
-    "build": (91796 * 2 + 198884) * 1024,
-    "com": (32032 * 2 + 39590) * 1024,
-    "forces": (184781 * 2 + 27876) * 1024,
+    "build": (91785 * 2 + 198985) * 1024,
+    "com": (32022 * 2 + 39583) * 1024,
+    "forces": (181246 * 2 + 27876) * 1024,
 }}
 # The walk's VALU busy fraction: SQ_ACTIVE_INST_VALU / (256 CUs x GRBM_GUI_ACTIVE
-# per XCD) per dispatch, 432.5e6 / (256 x 18.07e6 / 8).
-WALK_VALU_BUSY = {DEFAULT_KEY: 432.5e6 / (256 * 18.07e6 / 8)}
+# per XCD) per dispatch, 457.8e6 / (256 x 17.75e6 / 8).
+WALK_VALU_BUSY = {DEFAULT_KEY: 457.8e6 / (256 * 17.75e6 / 8)}
 
 
 def progress(msg: str) -> None:

@@ -78,8 +78,11 @@ typedef struct stellar_params {
 
 /* Record per-stage HIP event timings on every step (small overhead). */
 #define STELLAR_FLAG_STAGE_TIMING 1
-/* Accepted cells add their quadrupole term (not in the reference):
- * a += -Q.d/r^5 + (5/2)(d.Q.d) d/r^7 with r^2 = |d|^2 + eps2, moments
+/* Accepted cells add their quadrupole term (not in the reference), the
+ * second-order term of the softened potential sum m / sqrt(|d + s|^2 + eps2)
+ * expanded about the cell's centre of mass:
+ * a += -Q.d/r^5 + (5/2)(d.Q.d - eps2 tr Q) d/r^7 with r^2 = |d|^2 + eps2,
+ * Q = sum m (3 s s^T - |s|^2 I) over the cell's bodies (in-plane), moments
  * accumulated bottom-up in FP64. */
 #define STELLAR_FLAG_QUADRUPOLE 2
 /* Opening test w/d <= theta on the unsoftened distance d (the textbook test
