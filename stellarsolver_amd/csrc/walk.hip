@@ -14,9 +14,13 @@
 //   * the vote is one 64-lane ballot; lanes past the last body repeat the
 //     last body, so their votes change nothing and need no mask;
 //   * the stack (in LDS, one region per wavefront, written by lane 0) holds
-//     node index and level in one word; with a second block per node
-//     (quadrupoles, bmax) each cell child's threshold comes with it, otherwise
-//     the threshold dp of a level comes from a per-lane table (v_readlane);
+//     node index and level in one word; the opening threshold of a node's
+//     children (they share a level) comes from a per-lane table of level
+//     thresholds, one v_readlane per visited node, except with per-cell
+//     thresholds (bmax) or mixed precision, where each cell child's
+//     threshold comes with the node's second block;
+//   * a child's code decides empty, body or cell with one compare; a body is
+//     accepted without a vote;
 //   * the eight wavefronts of a workgroup take consecutive groups and each
 //     XCD one contiguous range of workgroups (scalar-cache and L2 reuse);
 //   * bodies are stored in tree order: the group's positions and outputs are
@@ -40,7 +44,8 @@
 // (s a body's offset from the cell's centre of mass), which carries, beside
 // the unsoftened form's two terms, -(5/2) tr(Q) eps2 d / r^7: w holds
 // T = -2.5 eps2 (Qxx + Qyy) for it.  With per-cell opening thresholds (bmax)
-// w holds the threshold instead and T is computed from Q in the walk.
+// or mixed precision w holds the threshold instead; bmax walks compute T from
+// Q, and the mixed bracket carries the term in FP16 (MixedAcc).
 #include <map>
 #include <mutex>
 
