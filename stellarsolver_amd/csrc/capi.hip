@@ -465,9 +465,10 @@ int stellar_destroy(stellar_ctx* c) {
 int stellar_set_params(stellar_ctx* c, const stellar_params* p) {
   if (!c) return STELLAR_ERR_ARG;
   if (!params_ok(p)) return fail(c, STELLAR_ERR_ARG, "parameters");
-  // the build and centre-of-mass stages store per-child opening thresholds
-  // and moments that depend on these: a tree built under the old values must
-  // not be walked under the new ones, so the stages start again from bbox
+  // the build and centre-of-mass stages store per-child opening thresholds,
+  // the quadrupole softening term and moments that depend on these: a tree
+  // built under the old values must not be walked under the new ones, so the
+  // stages start again from bbox
   constexpr int32_t kTreeFlags = STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_TEXTBOOK_OPENING |
                                  STELLAR_FLAG_BMAX_OPENING | STELLAR_FLAG_REFERENCE_ROOT;
   if (((p->flags ^ c->p.flags) & kTreeFlags) || p->theta != c->p.theta ||

@@ -65,8 +65,8 @@ struct ComArgs {
   int mixed;
   const int32_t* inst_of;
   const float4* ibbox;
-  // the walk's level threshold of a cell, stored beside its quadrupole
-  // moments (walk_level_threshold)
+  // the walk's level threshold of a cell (walk_level_threshold), stored
+  // beside its moments in mixed precision
   float theta_div;
   float eps2f;
   int unsoft;
@@ -250,7 +250,8 @@ __device__ void finish_cell(const ComArgs& A, ChunkLds<EXT>& L, int i, const Chi
 
 // Writes finished entry i out, by the thread that staged it (block b, child
 // slot s of parent block p): its child record {CoM, m, b} in the parent's
-// block, {Qxx, Qxy, Qyy, opening threshold} beside it, and its FP64 moments
+// block, {Qxx, Qxy, Qyy, w} beside it (w: the quadrupole's softening term, or
+// the cell's opening threshold with bmax or mixed precision), and its FP64 moments
 // when a parent finishing in a later pass or chunk will read them (a parent
 // finished in this chunk took the LDS copy).
 template <bool EXT>
@@ -428,9 +429,9 @@ __device__ void process_chunk(const ComArgs& A, ChunkLds<EXT>& L, int ne, int ro
       L.lev[i] = (uint8_t)lev;
       pb = meta_parent(meta);
       sl = meta_slot(meta);
-      // the instance's box, for the cell's threshold (loaded here, so that
-      // the first barrier covers its latency)
-      if (A.quad || A.bmax) bb = A.ibbox[A.nroot == 1 ? 0 : A.inst_of[meta_start(meta)]];
+      // the instance's box, for the mixed-precision cell's level threshold
+      // and size (loaded here, so that the first barrier covers its latency)
+      if (A.mixed) bb = A.ibbox[A.nroot == 1 ? 0 : A.inst_of[meta_start(meta)]];
     }
     // suffix minimum of the levels: within the wavefront, then across
     int suf = lev;

@@ -579,6 +579,29 @@ def test_quadrupole_parity(S, oracle, workload, n, textbook):
     assert rms_rel(f["ax"], f["ay"], ax, ay) < FORCE_RMS_TOL
 
 
+@pytest.mark.parametrize("group", [32, 16])
+@pytest.mark.parametrize("workload", [2, 4])
+def test_quadrupole_parity_small_groups(S, oracle, workload, group):
+    """Quadrupole walks of 32- and 16-body groups (several groups per
+    wavefront, the softening term read from the node's second block):
+    counts exact, forces within the rsqrt tolerance."""
+    print(f"quadrupole parity small groups workload={workload} group={group}", flush=True)
+    st = ic(workload, 65536)
+    eps2 = float(np.float32(EPS[workload] ** 2))
+    flags = S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING
+    with make_engine(S, st, eps2, group=group, flags=flags) as e:
+        gbb, gt = gpu_tree(e)
+        e.sort()
+        e.forces()
+        f = e.export_forces()
+    obb, ot = oracle_tree(oracle, st)
+    oracle.quad_exact(ot, st["x"], st["y"], st["mass"])
+    ax, ay, it = oracle.walk(ot, st["x"], st["y"], st["mass"], obb, 0.5, eps2, group,
+                             opening=1, quad=True)
+    assert np.array_equal(f["inter"], it)
+    assert rms_rel(f["ax"], f["ay"], ax, ay) < FORCE_RMS_TOL
+
+
 @pytest.mark.parametrize("kpop,threads,schedule", [(1, 512, 0), (2, 512, 0), (4, 256, 0), (1, 1024, 0),
                                                    (2, 64, 0), (1, 512, 1), (2, 256, 1),
                                                    (1, 512, 2), (2, 256, 2)])
