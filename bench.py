@@ -47,9 +47,9 @@ OPS_PER_INTERACTION = 13
 OPS_PER_QUAD_TERM = 11
 HBM_PEAK_GBS = 8000.0
 # Counter figures of the default workload (colliding disks, N = 2^20, theta 0.5,
-# quadrupoles + textbook opening, group 64) from rocprofv3 runs of this script
+# quadrupoles + textbook opening, group 64, Hilbert order) from rocprofv3 runs of this script
 # (profiles/hbm_traffic_colliding_disks_1M.md, profiles/walk_counters_colliding_disks_1M.md).
-DEFAULT_KEY = (2, 1 << 20, 0.5, 64, "quad", "textbook")
+DEFAULT_KEY = (2, 1 << 20, 0.5, 64, "quad", "textbook", "hilbert")
 # HBM bytes per step of each stage's kernels: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (KiB per dispatch), the factors being what the same counters read for a 1 GiB
 # device copy; "forces" is the walk launch (with the update inside it).
@@ -85,6 +85,9 @@ def parse():
     ap.add_argument("--opening", choices=("textbook", "reference", "bmax"), default="textbook",
                     help="opening test: textbook w/d <= theta (default), the reference's "
                          "w^2/theta <= d^2+eps^2, or bmax/d <= theta with a per-cell bound")
+    ap.add_argument("--order", choices=("hilbert", "slot"), default="hilbert",
+                    help="storage order of the bodies, and so the walk groups: along a Hilbert "
+                         "curve of the tree (default) or the reference's depth-first slot order")
     ap.add_argument("--no-reference-opening", action="store_true",
                     help="skip the side run with the reference's criterion")
     ap.add_argument("--error-sample", type=int, default=8192)
@@ -176,6 +179,8 @@ def model_flags(S, args):
         f |= S.FLAG_TEXTBOOK_OPENING
     elif args.opening == "bmax":
         f |= S.FLAG_BMAX_OPENING
+    if args.order == "hilbert":
+        f |= S.FLAG_HILBERT_ORDER
     return f
 
 
@@ -251,7 +256,8 @@ def synced_diagnostics(eng, dt, forces):
 
 def reference_opening_run(S, args, st, eps2):
     """The reference's force model (monopoles, w^2/theta <= d^2+eps^2, walk
-    group 64) on the same workload: speed and error."""
+    groups of 64 consecutive bodies of its depth-first slot order) on the same
+    workload: speed and error."""
     n = st["x"].size
     p = S.default_params(theta=args.theta, eps2=eps2, timestep=args.dt, group=args.group,
                          flags=S.FLAG_STAGE_TIMING)
@@ -271,7 +277,7 @@ def reference_opening_run(S, args, st, eps2):
             "ms_per_step": (t1 - t0) / k * 1e3, "rms_rel_force_err": err,
             "force_err_p99": p99, "force_err_max": emax, "stage_ms": stages,
             "opening": "reference (w^2/theta <= d^2+eps^2, group vote)",
-            "multipoles": "monopole"}
+            "multipoles": "monopole", "order": "slot"}
 
 
 def ensemble_members(S, ninst, seed=3):
@@ -410,7 +416,8 @@ def run_ensemble(args, S):
                                "[1024, 65536] (x64), disk/Plummer/colliding disks",
                    "n_bodies": ntot, "theta": args.theta, "eps": args.eps, "dt": args.dt,
                    "precision": args.precision, "graph": True, "tuning": tuning,
-                   "multipoles": multipoles(args), "opening": opening_text(args)},
+                   "multipoles": multipoles(args), "opening": opening_text(args),
+                   "order": args.order},
         "plain_form_body_steps_per_s": plain,
         "plain_form": f"{len(counts)} single-instance contexts, {plain_steps} steps each; "
                       "the fastest of: one after another with direct steps, one after "
@@ -474,7 +481,7 @@ def accuracy_sweep(args, S):
         rows.append({
             "workload": WORKLOAD_NAMES.get(args.workload, f"workload {args.workload}"),
             "n_bodies": n, "eps": args.eps, "steps_before": args.warmup, "theta": theta,
-            "multipoles": multipoles(args), "opening": opening_text(args),
+            "multipoles": multipoles(args), "opening": opening_text(args), "order": args.order,
             "precision": args.precision, "sample": int(tg.size),
             "rms_rel_force_err": float(np.sqrt(q.mean())),
             "p50": float(np.sqrt(np.median(q))), "p99": float(np.sqrt(np.percentile(q, 99))),
@@ -605,7 +612,8 @@ def main():
     walk_ms = stages["forces"]
     achieved = (inter * OPS_PER_INTERACTION + qinter * OPS_PER_QUAD_TERM) / (walk_ms * 1e-3) / 1e12
     peak = VALU_PEAK_LANE_OPS / 1e12
-    model_key = (args.workload, n, args.theta, args.group, args.multipole, args.opening)
+    model_key = (args.workload, n, args.theta, args.group, args.multipole, args.opening,
+                 args.order)
     if args.precision != "fp32" or args.graph:
         model_key = None  # counter runs exist for the default FP32 direct launches only
     out = {
@@ -625,7 +633,8 @@ def main():
                    "n_bodies": n, "theta": args.theta, "eps": args.eps, "dt": args.dt,
                    "group": args.group, "precision": args.precision, "seed": args.workload,
                    "graph": bool(args.graph),
-                   "opening": opening_text(args), "multipoles": multipoles(args)},
+                   "opening": opening_text(args), "multipoles": multipoles(args),
+                   "order": args.order},
         "rms_rel_force_err": err_rms,
         "force_err_p99": err_p99,
         "force_err_max": err_max,

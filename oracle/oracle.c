@@ -555,6 +555,60 @@ static void mix_pair(mixacc* M, const float* qa, float ex, float ey, float k, fl
   kahan(&M->sy, &M->cy, fmaf(M->pk, (float)BY[0], k * (float)BY[1]));
 }
 
+/* Hilbert frames: base order of the four quadrants (x: 1 east, y: 1 north)
+ * SW, NW, NE, SE; frame bit 0 transposes, bit 1 turns by 180 degrees (they
+ * commute); the child at curve position h gets frame ^ kHilbertChildFrame[h]. */
+static const int kHilbertChildFrame[4] = {1, 0, 0, 3};
+
+/* Slot of the child at curve position h in frame f. */
+static int hilbert_slot(int f, int h) {
+  static const int bx[4] = {0, 0, 1, 1}, by[4] = {0, 1, 1, 0};
+  int x = bx[h], y = by[h];
+  if (f & 1) {
+    const int t = x;
+    x = y;
+    y = t;
+  }
+  if (f & 2) {
+    x = 1 - x;
+    y = 1 - y;
+  }
+  return (1 - x) | ((1 - y) << 1);
+}
+
+int or_hilbert_order(int32_t n, const int32_t* root, const int32_t* child, int32_t l1,
+                     int32_t* order) {
+  /* explicit stack of (node code, level, frame); node -1 is the root */
+  enum { kStack = 4 * (OR_MAX_LEVEL + 2) };
+  int32_t sn[kStack], sl[kStack], sf[kStack];
+  int sp = 0, k = 0;
+  sn[sp] = -1;
+  sl[sp] = 0;
+  sf[sp++] = 0;
+  while (sp > 0) {
+    --sp;
+    const int32_t node = sn[sp], lev = sl[sp], f = sf[sp];
+    if (node >= 0 && node < n) {
+      if (k >= n) return OR_ERR_ARG;
+      order[k++] = node;
+      continue;
+    }
+    const int32_t* kids = node < 0 ? root : child + 4 * (int64_t)(node - n);
+    /* children pushed in reverse visiting order */
+    for (int h = 3; h >= 0; --h) {
+      const int hil = lev + 1 <= l1;
+      const int slot = hil ? hilbert_slot(f, h) : h;
+      const int32_t ch = kids[slot];
+      if (ch < 0) continue;
+      if (sp >= kStack) return OR_ERR_DEPTH;
+      sn[sp] = ch;
+      sl[sp] = lev + 1;
+      sf[sp++] = hil ? (f ^ kHilbertChildFrame[h]) : f;
+    }
+  }
+  return k == n ? OR_OK : OR_ERR_ARG;
+}
+
 int or_walk(int32_t n, const float* x, const float* y, const float* mass,
             const float* bbox, const int32_t* sorted, const int32_t* root,
             const int32_t* child, int64_t ncells, const float* ccx,
@@ -566,6 +620,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
   const float eps2 = p->eps2;
   const float depth0 = depth0_of(bbox, p->theta, p->opening);
   const float* cq = p->cq;
+  const int32_t* grp = p->order ? p->order : sorted; /* body order of the walk groups */
   const int32_t ngroups = (n + G - 1) / G;
   const int mixed = p->mixed && p->cqh && p->clevel;
   float s2 = 0.f;
@@ -594,7 +649,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
       const int32_t gs = (n - s0) < G ? (n - s0) : G;
       int32_t cnt = 0;
       for (int32_t j = 0; j < gs; ++j) {
-        const int32_t b = sorted[s0 + j];
+        const int32_t b = grp[s0 + j];
         px[j] = x[b];
         py[j] = y[b];
         fx[j] = 0.f;
@@ -706,7 +761,7 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
         }
       }
       for (int32_t j = 0; j < gs; ++j) {
-        const int32_t b = sorted[s0 + j];
+        const int32_t b = grp[s0 + j];
         ax[b] = fx[j];
         ay[b] = fy[j];
         if (inter) inter[b] = cnt;
@@ -790,6 +845,7 @@ int or_walk_recursive(int32_t n, const float* x, const float* y,
   if (G <= 0 || G > 1024) return OR_ERR_ARG;
   const float depth0 = depth0_of(bbox, p->theta, p->opening);
   const int32_t ngroups = (n + G - 1) / G;
+  const int32_t* grp = p->order ? p->order : sorted; /* body order of the walk groups */
 #pragma omp parallel num_threads(nthreads(p->threads))
   {
     float* px = (float*)malloc(sizeof(float) * (size_t)G * 2);
@@ -799,8 +855,8 @@ int or_walk_recursive(int32_t n, const float* x, const float* y,
       const int32_t s0 = g * G;
       const int32_t gs = (n - s0) < G ? (n - s0) : G;
       for (int32_t j = 0; j < gs; ++j) {
-        px[j] = x[sorted[s0 + j]];
-        px[G + j] = y[sorted[s0 + j]];
+        px[j] = x[grp[s0 + j]];
+        px[G + j] = y[grp[s0 + j]];
         f[j] = f[G + j] = 0.0;
       }
       recctx R = {n, gs, p->opening, root, x, y, mass, ccx, ccy, ccm, p->cq, p->cthr, child,
@@ -817,7 +873,7 @@ int or_walk_recursive(int32_t n, const float* x, const float* y,
         }
       }
       for (int32_t j = 0; j < gs; ++j) {
-        const int32_t b = sorted[s0 + j];
+        const int32_t b = grp[s0 + j];
         ax[b] = f[j];
         ay[b] = f[G + j];
         if (inter) inter[b] = R.cnt;
@@ -1139,7 +1195,7 @@ int or_step(int32_t n, float* x, float* y, float* vx, float* vy,
   t[4] = now_s();
   {
     /* the walk reads the pool layout: cells at n.. with CoM in cx/cy/cm */
-    or_walk_params wp = {p->theta, p->eps2, p->group, p->threads, 0, 0, NULL, NULL, NULL, NULL, 0, 0};
+    or_walk_params wp = {p->theta, p->eps2, p->group, p->threads, 0, 0, NULL, NULL, NULL, NULL, 0, 0, NULL};
     rc = or_walk(n, x, y, mass, bbox, sorted, child, child + 4 * (size_t)n,
                  ncells, cx + n, cy + n, cm + n, &wp, ax, ay, NULL);
     if (rc) goto out;

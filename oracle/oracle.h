@@ -114,6 +114,9 @@ typedef struct or_walk_params {
                         root's (the root's children are visited again one
                         level down), any other body's are empty */
   int32_t pad2;
+  const int32_t* order; /* NULL: walk groups are G consecutive bodies of
+                           sorted[]; else of this body order (a permutation of
+                           0..n-1, e.g. or_hilbert_order's) */
 } or_walk_params;
 
 int or_walk(int32_t n, const float* x, const float* y, const float* mass,
@@ -121,6 +124,15 @@ int or_walk(int32_t n, const float* x, const float* y, const float* mass,
             const int32_t* child, int64_t ncells, const float* ccx,
             const float* ccy, const float* ccm, const or_walk_params* p,
             float* ax, float* ay, int32_t* inter);
+
+/* Hilbert order of the bodies of a canonical tree (not in the reference):
+ * the depth-first order that visits the children of a node at level L < l1
+ * (the root is level 0) along a Hilbert curve -- base order SW, NW, NE, SE;
+ * the first child's frame transposed, the last child's transposed and
+ * turned by 180 degrees -- and the children of deeper nodes in slot order.
+ * Slot code c = west | south << 1.  order[k] = the k-th body id. */
+int or_hilbert_order(int32_t n, const int32_t* root, const int32_t* child, int32_t l1,
+                     int32_t* order);
 
 /* Mixed precision (not in the reference).  An accepted cell's term
  * m d/r^3 - Q.d/r^5 + (5/2)(d.Q.d - eps2 tr Q) d/r^7 is k B with k = m rs^2

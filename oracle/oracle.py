@@ -35,7 +35,7 @@ _I64 = C.POINTER(C.c_int64)
 class WalkParams(C.Structure):
     _fields_ = [("theta", C.c_float), ("eps2", C.c_float), ("group", C.c_int32), ("threads", C.c_int32),
                 ("opening", C.c_int32), ("mixed", C.c_int32), ("cq", _F), ("cthr", _F), ("cqh", _F),
-                ("clevel", _I), ("ref_root", C.c_int32), ("pad2", C.c_int32)]
+                ("clevel", _I), ("ref_root", C.c_int32), ("pad2", C.c_int32), ("order", _I)]
 
 
 class StepParams(C.Structure):
@@ -53,6 +53,7 @@ _SIG = {
     "or_build_keys": (C.c_int, [C.c_int32, _F, _F, _F, C.c_int64, _I, _I, _I, _I, _I, _I, _I64]),
     "or_com_exact": (None, [C.c_int32, C.c_int64, _I, _F, _F, _F, _F, _F, _F]),
     "or_bmax_exact": (None, [C.c_int32, C.c_int64, _I, _F, _F, _F, C.c_float, C.c_float, _F]),
+    "or_hilbert_order": (C.c_int, [C.c_int32, _I, _I, C.c_int32, _I]),
     "or_quad_exact": (None, [C.c_int32, C.c_int64, _I, _F, _F, _F, _F]),
     "or_quad_mixed": (None, [C.c_int32, C.c_int64, _I, _I, _F, _F, _F, _F, _F]),
     "or_f16_round": (C.c_double, [C.c_double, C.c_int32]),
@@ -219,7 +220,8 @@ def quad_exact(tree: dict, x, y, mass) -> None:
     tree["quad"] = q[: 3 * c]
 
 
-def _wp(tree, theta, eps2, group, threads, opening, quad, bmax=False, mixed=False, ref_root=False):
+def _wp(tree, theta, eps2, group, threads, opening, quad, bmax=False, mixed=False, ref_root=False,
+        order=None):
     cq = None
     if quad:
         cq = tree["quad"] if tree["quad"].size else np.zeros(3, np.float32)
@@ -231,9 +233,10 @@ def _wp(tree, theta, eps2, group, threads, opening, quad, bmax=False, mixed=Fals
         qh = tree["qh"] if tree["qh"].size else np.zeros(3, np.float32)
         lv = np.ascontiguousarray(tree["level"], dtype=np.int32)
         lv = lv if lv.size else np.zeros(1, np.int32)
+    od = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
     p = WalkParams(theta, eps2, group, threads, opening, 1 if mixed else 0, _f(cq), _f(th), _f(qh),
-                   _i(lv), 1 if ref_root else 0, 0)
-    return p, (cq, th, qh, lv)
+                   _i(lv), 1 if ref_root else 0, 0, _i(od))
+    return p, (cq, th, qh, lv, od)
 
 
 def quad_mixed(tree: dict, x, y, mass, bb) -> None:
@@ -259,13 +262,14 @@ def bmax_exact(tree: dict, x, y, mass, theta: float, eps2: float) -> None:
 
 def walk(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0,
          opening: int = 0, quad: bool = False, bmax: bool = False, mixed: bool = False,
-         ref_root: bool = False):
+         ref_root: bool = False, order=None):
     """Group-vote walk (kernels.cu:340-435) over a canonical tree; by body id.
     bmax: per-cell thresholds tree["thr"] (bmax_exact) instead of levels.
-    mixed: accepted cells' terms in FP16 from tree["qh"] (quad_mixed)."""
+    mixed: accepted cells' terms in FP16 from tree["qh"] (quad_mixed).
+    order: body order of the walk groups (default tree["sorted"])."""
     x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
     n = x.size
-    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad, bmax, mixed, ref_root)
+    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad, bmax, mixed, ref_root, order)
     ax = np.zeros(n, np.float32)
     ay = np.zeros(n, np.float32)
     it = np.zeros(n, np.int32)
@@ -282,10 +286,12 @@ def walk(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, thre
 
 
 def walk_recursive(tree: dict, x, y, mass, bb, theta: float, eps2: float, group: int, threads: int = 0,
-                   opening: int = 0, quad: bool = False, bmax: bool = False, ref_root: bool = False):
+                   opening: int = 0, quad: bool = False, bmax: bool = False, ref_root: bool = False,
+                   order=None):
     x, y, mass, bb = f32(x), f32(y), f32(mass), f32(bb)
     n = x.size
-    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad, bmax, ref_root=ref_root)
+    p, _keep = _wp(tree, theta, eps2, group, threads, opening, quad, bmax, ref_root=ref_root,
+                   order=order)
     ax = np.zeros(n, np.float64)
     ay = np.zeros(n, np.float64)
     it = np.zeros(n, np.int32)
@@ -299,6 +305,17 @@ def walk_recursive(tree: dict, x, y, mass, bb, theta: float, eps2: float, group:
     if rc:
         raise OracleError(rc, "walk_recursive")
     return ax, ay, it
+
+
+def hilbert_order(tree: dict, n: int, l1: int = 32) -> np.ndarray:
+    """Body ids in Hilbert order of a canonical tree (Hilbert children above
+    level l1, slot order below; or_hilbert_order)."""
+    out = np.zeros(max(n, 1), np.int32)
+    ch = tree["child"] if tree["child"].size else np.zeros(4, np.int32)
+    rc = lib().or_hilbert_order(n, _i(tree["root"]), _i(ch), int(l1), _i(out))
+    if rc:
+        raise OracleError(rc, "hilbert_order")
+    return out[:n]
 
 
 def direct_fp64(x, y, mass, eps2: float, targets=None, threads: int = 0):

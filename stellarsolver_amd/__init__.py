@@ -43,6 +43,7 @@ FLAG_GRAPH = 8
 FLAG_BMAX_OPENING = 16
 FLAG_FORCES_TIMING = 32
 FLAG_REFERENCE_ROOT = 64
+FLAG_HILBERT_ORDER = 128
 
 IC_DISK = 0
 IC_PLUMMER = 1

@@ -146,16 +146,19 @@ struct KeyView {
   const uint32_t* key3;
   int l1;
   int ibits;
+  // key holds Hilbert curve positions instead of slot codes (levels 1..l1;
+  // key2 and key3 hold slot codes either way)
+  int hilbert;
 };
 
 __global__ __launch_bounds__(256) void keys_kernel(
     const float* __restrict__ x, const float* __restrict__ y, int32_t n,
     const int32_t* __restrict__ inst_of, const float4* __restrict__ ibbox,
-    int l1, int ibits, uint64_t* __restrict__ key, uint32_t* __restrict__ perm) {
+    int l1, int ibits, int hilbert, uint64_t* __restrict__ key, uint32_t* __restrict__ perm) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t inst = inst_of[i];
-  key[i] = path_key(x[i], y[i], ibbox[inst], inst, l1, ibits);
+  key[i] = path_key(x[i], y[i], ibbox[inst], inst, l1, ibits, hilbert != 0);
   perm[i] = (uint32_t)i;
 }
 
@@ -232,8 +235,38 @@ __device__ __forceinline__ int common_levels(const KeyView& K, int32_t s, int32_
   return kMaxLevel;
 }
 
-__device__ __forceinline__ int digit_at(const KeyView& K, int32_t s, int lev) {
-  if (lev <= K.l1) return (int)((K.key[s] >> (2 * (K.l1 - lev))) & 3u);
+// Slot code of position s at level lev.  With Hilbert keys the code of a
+// level <= l1 is its curve position mapped through the frame its ancestors
+// give it; `cur` carries that frame from one call to the next (calls at
+// non-decreasing levels cost one step per level; a lower level starts over).
+struct SlotCursor {
+  int lev = 0;          // levels the frame has taken in
+  uint32_t frame = 0;   // frame after `lev` levels
+  int last_lev = 0;     // the last answer, for a repeated level
+  int last_code = 0;
+};
+
+__device__ __forceinline__ int digit_at(const KeyView& K, int32_t s, int lev, SlotCursor& cur) {
+  if (lev <= K.l1) {
+    const uint64_t k = K.key[s];
+    if (!K.hilbert) return (int)((k >> (2 * (K.l1 - lev))) & 3u);
+    if (lev == cur.last_lev) return cur.last_code;
+    if (lev <= cur.lev) {
+      cur.lev = 0;
+      cur.frame = 0;
+    }
+    uint32_t h = 0;
+    while (cur.lev < lev) {
+      ++cur.lev;
+      h = (uint32_t)(k >> (2 * (K.l1 - cur.lev))) & 3u;
+      if (cur.lev < lev) cur.frame = hilbert_child_frame(cur.frame, h);
+    }
+    const int code = (int)hilbert_slot(cur.frame, h);
+    cur.frame = hilbert_child_frame(cur.frame, h);
+    cur.last_lev = lev;
+    cur.last_code = code;
+    return code;
+  }
   if (lev <= K.l1 + 32) return (int)((K.key2[s] >> (2 * (K.l1 + 32 - lev))) & 3u);
   return (int)((K.key3[s] >> (30 - 2 * (lev - K.l1 - 33))) & 3u);
 }
@@ -550,6 +583,11 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
     pblk = pid < A.cap ? pid + A.nroot : -1;
   }
 
+  // slot codes of s are asked for at non-decreasing levels (B's first: it is
+  // also the first created node's slot)
+  SlotCursor cur;
+  const int dcur = first || pblk < 0 ? 0 : lp < kMaxLevel ? digit_at(A.K, s, lp + 1, cur) : s - a;
+
   // A: nodes created at s
   if (first || ls > lp) {
     for (int L = first ? 0 : lp + 1; L <= ls; ++L) {
@@ -557,8 +595,8 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
       if (b < 0) break;
       if (L > 0)
         A.cmeta[b] = cell_meta(s, (int32_t)(L == lp + 1 && !first ? pblk : created(L - 1)),
-                               digit_at(A.K, s, L), L);
-      const int d = L < kMaxLevel ? digit_at(A.K, s, L + 1) : 0;
+                               digit_at(A.K, s, L, cur), L);
+      const int d = L < kMaxLevel ? digit_at(A.K, s, L + 1, cur) : 0;
       if (L < ls) {
         const int64_t cb = created(L + 1);
         if (cb >= 0) put_cell(A, b, d, cb);
@@ -570,7 +608,6 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
 
   // B: s starts a new child of the node at level lp
   if (!first && pblk >= 0) {
-    const int dcur = lp < kMaxLevel ? digit_at(A.K, s, lp + 1) : s - a;
     if (dcur > 3) {
       atomicOr(&A.dev->err, kErrBucket);
     } else {
@@ -639,11 +676,13 @@ void launch_build(stellar_ctx* c) {
   // with the previous sort's splitters at hand, the keys kernel of the key
   // sort also counts the keys per bucket
   const bool fused = ss && c->ss_have_spl;
+  const bool hilbert = (c->p.flags & STELLAR_FLAG_HILBERT_ORDER) != 0;
   if (fused)
-    launch_keys_count(c, l1, ibits);
+    launch_keys_count(c, l1, ibits, hilbert);
   else
     keys_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, n, B.inst_of, B.ibbox,
-                                                          l1, ibits, B.key, B.perm);
+                                                          l1, ibits, hilbert ? 1 : 0, B.key,
+                                                          B.perm);
   if (ss) {
     launch_sample_sort(c, fused);  // sorted data back in (key, perm)
   } else {
@@ -662,7 +701,7 @@ void launch_build(stellar_ctx* c) {
                                                         B.ibbox, l1, B.perm, B.key2, B.key3,
                                                         B.dev, ss ? sample_sort_buckets(n) : 0,
                                                         ss ? B.ss_spl : nullptr);
-  const KeyView K{B.key, B.key2, B.key3, l1, ibits};
+  const KeyView K{B.key, B.key2, B.key3, l1, ibits, hilbert ? 1 : 0};
   int64_t mtb = 0;
   const MinTree mt = mintree_layout(n, B.lam, B.lmin, &mtb);
   const int32_t nblk = (int32_t)grid_for(n, kScanBlock);

@@ -470,7 +470,10 @@ int stellar_set_params(stellar_ctx* c, const stellar_params* p) {
   // built under the old values must not be walked under the new ones, so the
   // stages start again from bbox
   constexpr int32_t kTreeFlags = STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_TEXTBOOK_OPENING |
-                                 STELLAR_FLAG_BMAX_OPENING | STELLAR_FLAG_REFERENCE_ROOT;
+                                 STELLAR_FLAG_BMAX_OPENING | STELLAR_FLAG_REFERENCE_ROOT |
+                                 STELLAR_FLAG_HILBERT_ORDER;
+  // the sample sort's splitters are keys of the previous order
+  if ((p->flags ^ c->p.flags) & STELLAR_FLAG_HILBERT_ORDER) c->ss_have_spl = false;
   if (((p->flags ^ c->p.flags) & kTreeFlags) || p->theta != c->p.theta ||
       p->eps2 != c->p.eps2 || p->precision != c->p.precision)
     c->stage = kNone;
@@ -733,6 +736,111 @@ int stellar_stage_times(stellar_ctx* c, float* ms7) {
   return STELLAR_OK;
 }
 
+// The reference's numbering of a tree stored in another depth-first order
+// (STELLAR_FLAG_HILBERT_ORDER): a depth-first pass in slot order numbers the
+// cells and lists the bodies as the reference's sort does; the tree itself
+// (slots, levels, moments) is read as stored.  srt[s]: body id at storage
+// position s; blk: the C cell blocks of the instance (device cell q at
+// blk[4q]).
+static int export_canonical(stellar_ctx* c, int32_t inst, int32_t n, int32_t nr, int64_t c0,
+                            int64_t C, const std::vector<int32_t>& srt,
+                            const std::vector<ChildRec>& rootb, const std::vector<ChildRec>& blk,
+                            int32_t* root4, int32_t* child, int32_t* count, int32_t* start,
+                            int32_t* level, int32_t* sorted, float* com_x, float* com_y,
+                            float* com_m) {
+  Buffers& B = c->b;
+  std::vector<int64_t> q2k((size_t)(C > 0 ? C : 1), -1);
+  std::vector<int32_t> body_order;
+  body_order.reserve((size_t)(n > 0 ? n : 1));
+  std::vector<int32_t> kstart((size_t)(C > 0 ? C : 1), 0), kcount((size_t)(C > 0 ? C : 1), 0);
+  std::vector<int64_t> k2q((size_t)(C > 0 ? C : 1), 0);
+  // device cell q of a code, -1 for a body or an empty slot, -2 out of range
+  auto cell_of = [&](int32_t code) -> int64_t {
+    if (code <= 0) return -1;
+    const int64_t q = (int64_t)code - nr - c0;
+    return q >= 0 && q < C ? q : -2;
+  };
+  int64_t next = 0;
+  // explicit stack: (device cell, or -1 for the root; child slot to visit next)
+  std::vector<std::pair<int64_t, int>> stk;
+  stk.push_back({-1, 0});
+  while (!stk.empty()) {
+    auto& top = stk.back();
+    if (top.second == 4) {
+      if (top.first >= 0) {
+        const int64_t k = q2k[(size_t)top.first];
+        kcount[(size_t)k] = (int32_t)body_order.size() - kstart[(size_t)k];
+      }
+      stk.pop_back();
+      continue;
+    }
+    const int slot = top.second++;
+    const ChildRec r = top.first < 0 ? rootb[(size_t)slot] : blk[(size_t)(4 * top.first + slot)];
+    if (r.code == 0) continue;
+    if (r.code < 0) {
+      const int64_t sp = -1 - (int64_t)r.code - (int64_t)c->ioff[(size_t)inst];
+      if (sp < 0 || sp >= n) return fail(c, STELLAR_ERR_DEVICE, "tree body out of range");
+      body_order.push_back(srt[(size_t)sp]);
+      continue;
+    }
+    const int64_t q = cell_of(r.code);
+    if (q < 0 || q2k[(size_t)q] >= 0 || next >= C)
+      return fail(c, STELLAR_ERR_DEVICE, "tree cell out of range");
+    q2k[(size_t)q] = next;
+    k2q[(size_t)next] = q;
+    kstart[(size_t)next] = (int32_t)body_order.size();
+    ++next;
+    if (stk.size() > (size_t)(4 * kMaxLevel)) return fail(c, STELLAR_ERR_DEVICE, "tree depth");
+    stk.push_back({q, 0});
+  }
+  if (next != C || (int64_t)body_order.size() != n)
+    return fail(c, STELLAR_ERR_DEVICE, "tree does not span the instance");
+  // canonical codes: -1 empty, body id, n + canonical cell
+  auto conv = [&](int32_t code) -> int32_t {
+    if (code == 0) return -1;
+    if (code < 0) {
+      const int64_t sp = -1 - (int64_t)code - (int64_t)c->ioff[(size_t)inst];
+      return sp >= 0 && sp < n ? srt[(size_t)sp] : -2;
+    }
+    const int64_t q = cell_of(code);
+    return q >= 0 ? (int32_t)(n + q2k[(size_t)q]) : -2;
+  };
+  if (root4)
+    for (int k = 0; k < 4; ++k) root4[k] = conv(rootb[(size_t)k].code);
+  if (child)
+    for (int64_t k = 0; k < C; ++k)
+      for (int sl = 0; sl < 4; ++sl)
+        child[4 * k + sl] = conv(blk[(size_t)(4 * k2q[(size_t)k] + sl)].code);
+  if (count)
+    for (int64_t k = 0; k < C; ++k) count[k] = kcount[(size_t)k];
+  if (start)
+    for (int64_t k = 0; k < C; ++k) start[k] = kstart[(size_t)k];
+  if (sorted && n > 0) std::memcpy(sorted, body_order.data(), (size_t)n * 4);
+  std::vector<uint64_t> meta((size_t)(C > 0 ? C : 1));
+  if (C > 0 && (level || com_x || com_y || com_m))
+    HIP_TRY(c, hipMemcpy(meta.data(), B.cmeta + nr + c0, (size_t)C * 8, hipMemcpyDeviceToHost));
+  if (C > 0 && level)
+    for (int64_t k = 0; k < C; ++k) level[k] = meta_level(meta[(size_t)k2q[(size_t)k]]);
+  if (C > 0 && (com_x || com_y || com_m)) {
+    if (c->stage < kComDone) return fail(c, STELLAR_ERR_STATE, "no centre of mass yet");
+    // a cell's centre of mass and mass are its record in the parent's block
+    for (int64_t k = 0; k < C; ++k) {
+      const uint64_t m = meta[(size_t)k2q[(size_t)k]];
+      const int64_t pb = meta_parent(m);
+      const int sl = meta_slot(m);
+      ChildRec r{0.f, 0.f, 0.f, 0};
+      if (pb == inst)
+        r = rootb[(size_t)sl];
+      else if (pb - nr - c0 >= 0 && pb - nr - c0 < C)
+        r = blk[(size_t)(4 * (pb - nr - c0) + sl)];
+      if (com_x) com_x[k] = r.x;
+      if (com_y) com_y[k] = r.y;
+      if (com_m) com_m[k] = r.m;
+    }
+  }
+  return STELLAR_OK;
+}
+
 int stellar_export_tree_instance(stellar_ctx* c, int32_t inst, int32_t* root4,
                                  int32_t* child, int32_t* count, int32_t* start,
                                  int32_t* level, int32_t* sorted, float* com_x,
@@ -793,6 +901,9 @@ int stellar_export_tree_instance(stellar_ctx* c, int32_t inst, int32_t* root4,
     const int64_t q = (int64_t)code - nr - c0;
     return q >= 0 && q < C ? (int32_t)(n + q) : -2;
   };
+  if (c->p.flags & STELLAR_FLAG_HILBERT_ORDER)
+    return export_canonical(c, inst, n, nr, c0, C, srt, rootb, blk, root4, child, count, start,
+                            level, sorted, com_x, com_y, com_m);
   if (root4)
     for (int k = 0; k < 4; ++k) root4[k] = conv(rootb[(size_t)k].code);
   if (child)
@@ -1035,7 +1146,8 @@ int stellar_autotune(stellar_ctx* c, int32_t group_mask, int32_t reps, stellar_t
   for (int64_t per = c->n / c->ninst; per > 1; per >>= 1) ++lg;
   int li = 0;
   for (int32_t k = c->ninst; k > 1; k >>= 2) ++li;
-  const int fl = c->p.flags & (STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_TEXTBOOK_OPENING);
+  const int fl = c->p.flags & (STELLAR_FLAG_QUADRUPOLE | STELLAR_FLAG_TEXTBOOK_OPENING |
+                               STELLAR_FLAG_HILBERT_ORDER);
   const auto key = std::make_tuple(lg, li, fl, group_mask, (int)c->p.precision);
   auto apply = [&](const stellar_tuning& t) {
     c->p.group = t.group;

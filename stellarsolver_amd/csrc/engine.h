@@ -86,15 +86,45 @@ __device__ __forceinline__ uint32_t split2(float px, float py, float& l,
   return cx | (cy << 1);
 }
 
+// Hilbert order of a node's children (STELLAR_FLAG_HILBERT_ORDER).  Frames:
+// base curve SW, NW, NE, SE; frame bit 0 transposes, bit 1 turns by 180
+// degrees (the two commute, so frames compose by XOR); the child at curve
+// position h gets frame ^ child(h), child = {1, 0, 0, 3}.
+//   kHilbertNext: 4 bits per (frame f, slot code c) at 4 (4f + c): the
+//     curve position h (bits 0-1) and the child's frame (bits 2-3);
+//   kHilbertSlot: 2 bits per (frame f, curve position h) at 2 (4f + h): the
+//     slot code;
+//   kHilbertChild: 2 bits per h: the frame change.
+constexpr uint64_t kHilbertNext = 0xe3d8a97c05b64f12ull;
+constexpr uint32_t kHilbertSlot = 0xb4784b87u;
+constexpr uint32_t kHilbertChild = 0xc1u;
+__host__ __device__ inline uint32_t hilbert_step(uint32_t& frame, uint32_t slot) {
+  const uint32_t v = (uint32_t)(kHilbertNext >> (4 * ((frame << 2) | slot))) & 15u;
+  frame = v >> 2;
+  return v & 3u;
+}
+__host__ __device__ inline uint32_t hilbert_slot(uint32_t frame, uint32_t h) {
+  return (kHilbertSlot >> (2 * ((frame << 2) | h))) & 3u;
+}
+__host__ __device__ inline uint32_t hilbert_child_frame(uint32_t frame, uint32_t h) {
+  return frame ^ ((kHilbertChild >> (2 * h)) & 3u);
+}
+
 // Sort key of one body: the slot codes of levels 1..l1 below the instance's
-// box, with the instance index in the top ibits bits (ibits = 0: one
-// instance, l1 = 32).
+// box (hilbert: their positions along each node's Hilbert curve instead, so
+// that sorting orders the bodies along a Hilbert curve of the same tree), with
+// the instance index in the top ibits bits (ibits = 0: one instance, l1 = 32).
 __device__ __forceinline__ uint64_t path_key(float px, float py, float4 bb, int32_t inst, int l1,
-                                             int ibits) {
+                                             int ibits, bool hilbert) {
   float l = bb.x, r = bb.y, b = bb.z, t = bb.w;
   uint64_t k = 0;
+  uint32_t frame = 0;
 #pragma unroll 4
-  for (int lev = 0; lev < l1; ++lev) k = (k << 2) | split2(px, py, l, r, b, t);
+  for (int lev = 0; lev < l1; ++lev) {
+    uint32_t c = split2(px, py, l, r, b, t);
+    if (hilbert) c = hilbert_step(frame, c);
+    k = (k << 2) | c;
+  }
   if (ibits > 0) k |= (uint64_t)inst << (64 - ibits);
   return k;
 }
@@ -287,7 +317,7 @@ bool sample_sort_fits(int32_t n);
 int32_t sample_sort_buckets(int32_t n);
 int32_t sample_sort_tiles(int32_t n);
 void launch_sample_sort(stellar_ctx* c, bool keys_counted);
-void launch_keys_count(stellar_ctx* c, int l1, int ibits);
+void launch_keys_count(stellar_ctx* c, int l1, int ibits, bool hilbert);
 void launch_update(stellar_ctx* c);
 void launch_output(stellar_ctx* c);
 void launch_zero_counter(stellar_ctx* c);

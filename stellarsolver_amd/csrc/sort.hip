@@ -165,7 +165,7 @@ template <int MAXB>
 __global__ __launch_bounds__(kSsTileThreads) void ss_keys_count_kernel(
     const float* __restrict__ x, const float* __restrict__ y, int32_t n,
     const int32_t* __restrict__ inst_of, const float4* __restrict__ ibbox, int l1, int ibits,
-    int32_t nb, int32_t tile, const uint64_t* __restrict__ spl, uint64_t* __restrict__ key,
+    int hilbert, int32_t nb, int32_t tile, const uint64_t* __restrict__ spl, uint64_t* __restrict__ key,
     uint32_t* __restrict__ perm, uint32_t* __restrict__ tot) {
   __shared__ uint64_t sp[MAXB];
   __shared__ uint32_t cnt[MAXB];
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kSsTileThreads) void ss_keys_count_kernel(
   const int64_t t1 = t0 + tile < n ? t0 + tile : n;
   for (int64_t e = t0 + threadIdx.x; e < t1; e += blockDim.x) {
     const int32_t inst = inst_of[e];
-    const uint64_t k = path_key(x[e], y[e], ibbox[inst], inst, l1, ibits);
+    const uint64_t k = path_key(x[e], y[e], ibbox[inst], inst, l1, ibits, hilbert != 0);
     key[e] = k;
     perm[e] = (uint32_t)e;
     atomicAdd(&cnt[bucket_of(sp, nb, k)], 1u);
@@ -493,7 +493,7 @@ void launch_sample_sort(stellar_ctx* c, bool keys_counted) {
   c->ss_have_spl = true;
 }
 
-void launch_keys_count(stellar_ctx* c, int l1, int ibits) {
+void launch_keys_count(stellar_ctx* c, int l1, int ibits, bool hilbert) {
   Buffers& B = c->b;
   const int32_t n = c->n;
   const int32_t nb = sample_sort_buckets(n);
@@ -501,7 +501,8 @@ void launch_keys_count(stellar_ctx* c, int l1, int ibits) {
            : nb <= kSsBucketsMid ? ss_keys_count_kernel<kSsBucketsMid>
                                  : ss_keys_count_kernel<kSsMaxBuckets>;
   k<<<sample_sort_tiles(n), kSsTileThreads, 0, c->stream>>>(B.x, B.y, n, B.inst_of, B.ibbox, l1,
-                                                            ibits, nb, sample_sort_tile(n),
+                                                            ibits, hilbert ? 1 : 0, nb,
+                                                            sample_sort_tile(n),
                                                             B.ss_spl, B.key, B.perm, B.ss_tot);
 }
 

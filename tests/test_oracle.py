@@ -491,6 +491,74 @@ def test_synchronised_energy_of_kick_drift_orbit(oracle):
     assert np.ptp(lz) / abs(lz[0]) < 1e-5
 
 
+# -------------------------------------------------------- Hilbert order ---
+def test_hilbert_order_known_answer(oracle):
+    """One body per quadrant (ids NE, NW, SE, SW): the root's curve visits
+    SW, NW, NE, SE."""
+    x = np.array([1, -1, 1, -1], np.float32)
+    y = np.array([1, 1, -1, -1], np.float32)
+    bb = oracle.bbox(x, y)
+    kt = oracle.build_keys(x, y, bb)
+    assert list(oracle.hilbert_order(kt, 4)) == [3, 1, 0, 2]
+
+
+def test_hilbert_order_is_a_continuous_curve(oracle):
+    """Bodies at the centres of a 16 x 16 grid (the tree's midpoints fall
+    between grid lines): consecutive bodies of the order are grid neighbours
+    (a Morton order jumps), the order starts in the south-west corner and ends
+    in the south-east one, and every cell's bodies are one contiguous run."""
+    k = 16
+    i, j = np.meshgrid(np.arange(k), np.arange(k), indexing="ij")
+    perm = np.random.default_rng(1).permutation(k * k)
+    gi, gj = i.ravel()[perm], j.ravel()[perm]
+    x = (gi + 0.5).astype(np.float32)
+    y = (gj + 0.5).astype(np.float32)
+    bb = oracle.bbox(x, y)
+    kt = oracle.build_keys(x, y, bb)
+    h = oracle.hilbert_order(kt, k * k)
+    assert np.array_equal(np.sort(h), np.arange(k * k))
+    step = np.abs(np.diff(gi[h])) + np.abs(np.diff(gj[h]))
+    assert np.all(step == 1)
+    assert (gi[h[0]], gj[h[0]]) == (0, 0) and (gi[h[-1]], gj[h[-1]]) == (k - 1, 0)
+    ms = kt["sorted"]
+    assert np.any(np.abs(np.diff(gi[ms])) + np.abs(np.diff(gj[ms])) > 1)
+    rank = np.empty(k * k, np.int64)
+    rank[h] = np.arange(k * k)
+    for c in range(kt["ncells"]):
+        r = np.sort(rank[ms[kt["start"][c]: kt["start"][c] + kt["count"][c]]])
+        assert r[-1] - r[0] == r.size - 1, c
+
+
+def test_hilbert_order_below_l1_is_slot_order(oracle):
+    """Below level l1 the order follows the slot codes: with l1 = 0 it is the
+    reference's sort order."""
+    st = ic(1, 3000)
+    bb, kt = _tree(oracle, st)
+    assert np.array_equal(oracle.hilbert_order(kt, 3000, l1=0), kt["sorted"])
+    assert not np.array_equal(oracle.hilbert_order(kt, 3000), kt["sorted"])
+
+
+def test_hilbert_groups_walk(oracle):
+    """Walk groups taken along the Hilbert order: the stack walk equals the
+    recursive statement with the same groups; against Morton groups fewer
+    interactions per body on the colliding disks, forces as accurate."""
+    st = ic(2, 20000)
+    bb, kt = _tree(oracle, st)
+    oracle.quad_exact(kt, st["x"], st["y"], st["mass"])
+    h = oracle.hilbert_order(kt, 20000)
+    args = (kt, st["x"], st["y"], st["mass"], bb, 0.5, 1e-4, 64)
+    ax, ay, it = oracle.walk(*args, opening=1, quad=True, order=h)
+    rx, ry, rit = oracle.walk_recursive(*args, opening=1, quad=True, order=h)
+    assert np.array_equal(it, rit)
+    assert np.sqrt(np.mean(((ax - rx) ** 2 + (ay - ry) ** 2) / (rx ** 2 + ry ** 2))) < 5e-6
+    mx, my, mit = oracle.walk(*args, opening=1, quad=True)
+    assert it.mean() < mit.mean()
+    tg = np.arange(0, 20000, 10, dtype=np.int32)
+    dx, dy = oracle.direct_fp64(st["x"], st["y"], st["mass"], 1e-4, targets=tg)
+    e = lambda gx, gy: np.sqrt(np.mean(((gx[tg] - dx) ** 2 + (gy[tg] - dy) ** 2) / (dx ** 2 + dy ** 2)))
+    assert e(ax, ay) < 1.5 * e(mx, my) and e(ax, ay) < 1e-3
+
+
 # ------------------------------------------------------------ quadrupole ---
 def test_quadrupole_matches_body_range_moments(oracle):
     """Q of each cell = sum over its bodies of m (3 d d^T - |d|^2 I), d from
