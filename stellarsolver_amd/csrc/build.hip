@@ -235,37 +235,24 @@ __device__ __forceinline__ int common_levels(const KeyView& K, int32_t s, int32_
   return kMaxLevel;
 }
 
-// Slot code of position s at level lev.  With Hilbert keys the code of a
-// level <= l1 is its curve position mapped through the frame its ancestors
-// give it; `cur` carries that frame from one call to the next (calls at
-// non-decreasing levels cost one step per level; a lower level starts over).
-struct SlotCursor {
-  int lev = 0;          // levels the frame has taken in
-  uint32_t frame = 0;   // frame after `lev` levels
-  int last_lev = 0;     // the last answer, for a repeated level
-  int last_code = 0;
-};
-
-__device__ __forceinline__ int digit_at(const KeyView& K, int32_t s, int lev, SlotCursor& cur) {
+// Slot code of position s at level lev.  With Hilbert keys a level <= l1
+// holds the curve position h; the frame it is read in is the XOR of the
+// frame changes {1, 0, 0, 3} of the levels above, so its bit 0 is the parity
+// of the digits above with both bits equal (h = 0 or 3) and its bit 1 the
+// parity of those equal to 3: two popcounts of the key's prefix.
+__device__ __forceinline__ int digit_at(const KeyView& K, int32_t s, int lev) {
   if (lev <= K.l1) {
     const uint64_t k = K.key[s];
-    if (!K.hilbert) return (int)((k >> (2 * (K.l1 - lev))) & 3u);
-    if (lev == cur.last_lev) return cur.last_code;
-    if (lev <= cur.lev) {
-      cur.lev = 0;
-      cur.frame = 0;
-    }
-    uint32_t h = 0;
-    while (cur.lev < lev) {
-      ++cur.lev;
-      h = (uint32_t)(k >> (2 * (K.l1 - cur.lev))) & 3u;
-      if (cur.lev < lev) cur.frame = hilbert_child_frame(cur.frame, h);
-    }
-    const int code = (int)hilbert_slot(cur.frame, h);
-    cur.frame = hilbert_child_frame(cur.frame, h);
-    cur.last_lev = lev;
-    cur.last_code = code;
-    return code;
+    const uint32_t h = (uint32_t)(k >> (2 * (K.l1 - lev))) & 3u;
+    if (!K.hilbert) return (int)h;
+    constexpr uint64_t kEven = 0x5555555555555555ull;
+    // digits of levels 1..lev-1 (the instance bits above them masked off)
+    const uint64_t p = lev > 1 ? k >> (2 * (K.l1 - lev + 1)) : 0;
+    const uint64_t m = lev - 1 >= 32 ? kEven : (((uint64_t)1 << (2 * (lev - 1))) - 1) & kEven;
+    const uint64_t lo = p & m, hi = (p >> 1) & m;
+    const uint32_t frame = (uint32_t)(__popcll(~(lo ^ hi) & m) & 1) |
+                           (uint32_t)((__popcll(lo & hi) & 1) << 1);
+    return (int)hilbert_slot(frame, h);
   }
   if (lev <= K.l1 + 32) return (int)((K.key2[s] >> (2 * (K.l1 + 32 - lev))) & 3u);
   return (int)((K.key3[s] >> (30 - 2 * (lev - K.l1 - 33))) & 3u);
@@ -583,11 +570,6 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
     pblk = pid < A.cap ? pid + A.nroot : -1;
   }
 
-  // slot codes of s are asked for at non-decreasing levels (B's first: it is
-  // also the first created node's slot)
-  SlotCursor cur;
-  const int dcur = first || pblk < 0 ? 0 : lp < kMaxLevel ? digit_at(A.K, s, lp + 1, cur) : s - a;
-
   // A: nodes created at s
   if (first || ls > lp) {
     for (int L = first ? 0 : lp + 1; L <= ls; ++L) {
@@ -595,8 +577,8 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
       if (b < 0) break;
       if (L > 0)
         A.cmeta[b] = cell_meta(s, (int32_t)(L == lp + 1 && !first ? pblk : created(L - 1)),
-                               digit_at(A.K, s, L, cur), L);
-      const int d = L < kMaxLevel ? digit_at(A.K, s, L + 1, cur) : 0;
+                               digit_at(A.K, s, L), L);
+      const int d = L < kMaxLevel ? digit_at(A.K, s, L + 1) : 0;
       if (L < ls) {
         const int64_t cb = created(L + 1);
         if (cb >= 0) put_cell(A, b, d, cb);
@@ -608,6 +590,7 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
 
   // B: s starts a new child of the node at level lp
   if (!first && pblk >= 0) {
+    const int dcur = lp < kMaxLevel ? digit_at(A.K, s, lp + 1) : s - a;
     if (dcur > 3) {
       atomicOr(&A.dev->err, kErrBucket);
     } else {
