@@ -55,13 +55,13 @@ DEFAULT_KEY = (2, 1 << 20, 0.5, 64, "quad", "textbook", "hilbert")
 # device copy; "forces" is the walk launch (with the update inside it).
 STAGE_TRAFFIC_BYTES = {DEFAULT_KEY: {
     "bbox": (4125 * 2 + 8) * 1024,
-    "build": (91785 * 2 + 198985) * 1024,
-    "com": (32022 * 2 + 39583) * 1024,
-    "forces": (181246 * 2 + 27876) * 1024,
+    "build": (90589 * 2 + 197550) * 1024,
+    "com": (32028 * 2 + 39696) * 1024,
+    "forces": (162230 * 2 + 27876) * 1024,
 }}
 # The walk's VALU busy fraction: SQ_ACTIVE_INST_VALU / (256 CUs x GRBM_GUI_ACTIVE
-# per XCD) per dispatch, 457.8e6 / (256 x 17.75e6 / 8).
-WALK_VALU_BUSY = {DEFAULT_KEY: 457.8e6 / (256 * 17.75e6 / 8)}
+# per XCD) per dispatch, 413.9e6 / (256 x 15.70e6 / 8).
+WALK_VALU_BUSY = {DEFAULT_KEY: 413.9e6 / (256 * 15.70e6 / 8)}
 
 
 def progress(msg: str) -> None:

@@ -151,14 +151,15 @@ struct KeyView {
   int hilbert;
 };
 
+template <bool HILBERT>
 __global__ __launch_bounds__(256) void keys_kernel(
     const float* __restrict__ x, const float* __restrict__ y, int32_t n,
     const int32_t* __restrict__ inst_of, const float4* __restrict__ ibbox,
-    int l1, int ibits, int hilbert, uint64_t* __restrict__ key, uint32_t* __restrict__ perm) {
+    int l1, int ibits, uint64_t* __restrict__ key, uint32_t* __restrict__ perm) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t inst = inst_of[i];
-  key[i] = path_key(x[i], y[i], ibbox[inst], inst, l1, ibits, hilbert != 0);
+  key[i] = path_key<HILBERT>(x[i], y[i], ibbox[inst], inst, l1, ibits);
   perm[i] = (uint32_t)i;
 }
 
@@ -235,27 +236,46 @@ __device__ __forceinline__ int common_levels(const KeyView& K, int32_t s, int32_
   return kMaxLevel;
 }
 
-// Slot code of position s at level lev.  With Hilbert keys a level <= l1
-// holds the curve position h; the frame it is read in is the XOR of the
-// frame changes {1, 0, 0, 3} of the levels above, so its bit 0 is the parity
-// of the digits above with both bits equal (h = 0 or 3) and its bit 1 the
-// parity of those equal to 3: two popcounts of the key's prefix.
-__device__ __forceinline__ int digit_at(const KeyView& K, int32_t s, int lev) {
-  if (lev <= K.l1) {
-    const uint64_t k = K.key[s];
-    const uint32_t h = (uint32_t)(k >> (2 * (K.l1 - lev))) & 3u;
-    if (!K.hilbert) return (int)h;
+// Slot codes of one tree position at consecutive levels.  With Hilbert keys
+// a level <= l1 holds the curve position h, read in the frame that the levels
+// above give it: the XOR of their frame changes {1, 0, 0, 3}, whose bit 0 is
+// the parity of the digits above with both bits equal (h = 0 or 3) and bit 1
+// the parity of those equal to 3 -- two popcounts of the key's prefix for the
+// first level asked for, then one frame change per level.
+struct SlotWalk {
+  int lev;         // the level slot_code answers for
+  uint32_t frame;  // Hilbert frame of that level
+  uint64_t k;
+};
+
+__device__ __forceinline__ SlotWalk slot_walk(const KeyView& K, int32_t s, int lev) {
+  SlotWalk w{lev, 0u, K.key[s]};
+  if (K.hilbert && lev <= K.l1 && lev > 1) {
     constexpr uint64_t kEven = 0x5555555555555555ull;
     // digits of levels 1..lev-1 (the instance bits above them masked off)
-    const uint64_t p = lev > 1 ? k >> (2 * (K.l1 - lev + 1)) : 0;
-    const uint64_t m = lev - 1 >= 32 ? kEven : (((uint64_t)1 << (2 * (lev - 1))) - 1) & kEven;
+    const uint64_t p = w.k >> (2 * (K.l1 - lev + 1));
+    const uint64_t m = (((uint64_t)1 << (2 * (lev - 1))) - 1) & kEven;
     const uint64_t lo = p & m, hi = (p >> 1) & m;
-    const uint32_t frame = (uint32_t)(__popcll(~(lo ^ hi) & m) & 1) |
-                           (uint32_t)((__popcll(lo & hi) & 1) << 1);
-    return (int)hilbert_slot(frame, h);
+    w.frame = (uint32_t)(__popcll(~(lo ^ hi) & m) & 1) |
+              (uint32_t)((__popcll(lo & hi) & 1) << 1);
+  }
+  return w;
+}
+
+__device__ __forceinline__ int slot_code(const KeyView& K, int32_t s, const SlotWalk& w) {
+  const int lev = w.lev;
+  if (lev <= K.l1) {
+    const uint32_t h = (uint32_t)(w.k >> (2 * (K.l1 - lev))) & 3u;
+    return (int)(K.hilbert ? hilbert_slot(w.frame, h) : h);
   }
   if (lev <= K.l1 + 32) return (int)((K.key2[s] >> (2 * (K.l1 + 32 - lev))) & 3u);
   return (int)((K.key3[s] >> (30 - 2 * (lev - K.l1 - 33))) & 3u);
+}
+
+__device__ __forceinline__ void slot_next(const KeyView& K, SlotWalk& w) {
+  if (K.hilbert && w.lev <= K.l1)
+    w.frame = hilbert_child_frame(w.frame, (uint32_t)(w.k >> (2 * (K.l1 - w.lev))) & 3u);
+  ++w.lev;
 }
 
 // Minimum tree over lam: level 0 is lam itself (one byte per tree position,
@@ -570,15 +590,22 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
     pblk = pid < A.cap ? pid + A.nroot : -1;
   }
 
+  // slot codes of s from level lp + 1 on (level 1 for the instance's first
+  // position); B's slot is the first
+  SlotWalk w = slot_walk(A.K, s, first ? 1 : lp + 1);
+  const int dcur = first || pblk < 0 ? 0 : lp < kMaxLevel ? slot_code(A.K, s, w) : s - a;
+
   // A: nodes created at s
   if (first || ls > lp) {
     for (int L = first ? 0 : lp + 1; L <= ls; ++L) {
       const int64_t b = created(L);
       if (b < 0) break;
-      if (L > 0)
+      if (L > 0) {
         A.cmeta[b] = cell_meta(s, (int32_t)(L == lp + 1 && !first ? pblk : created(L - 1)),
-                               digit_at(A.K, s, L), L);
-      const int d = L < kMaxLevel ? digit_at(A.K, s, L + 1) : 0;
+                               slot_code(A.K, s, w), L);
+        slot_next(A.K, w);
+      }
+      const int d = L < kMaxLevel ? slot_code(A.K, s, w) : 0;
       if (L < ls) {
         const int64_t cb = created(L + 1);
         if (cb >= 0) put_cell(A, b, d, cb);
@@ -590,7 +617,6 @@ __global__ __launch_bounds__(kScanBlock) void link_kernel(LinkArgs A) {
 
   // B: s starts a new child of the node at level lp
   if (!first && pblk >= 0) {
-    const int dcur = lp < kMaxLevel ? digit_at(A.K, s, lp + 1) : s - a;
     if (dcur > 3) {
       atomicOr(&A.dev->err, kErrBucket);
     } else {
@@ -663,9 +689,8 @@ void launch_build(stellar_ctx* c) {
   if (fused)
     launch_keys_count(c, l1, ibits, hilbert);
   else
-    keys_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(B.x, B.y, n, B.inst_of, B.ibbox,
-                                                          l1, ibits, hilbert ? 1 : 0, B.key,
-                                                          B.perm);
+    (hilbert ? keys_kernel<true> : keys_kernel<false>)<<<grid_for(n, 256), 256, 0, c->stream>>>(
+        B.x, B.y, n, B.inst_of, B.ibbox, l1, ibits, B.key, B.perm);
   if (ss) {
     launch_sample_sort(c, fused);  // sorted data back in (key, perm)
   } else {

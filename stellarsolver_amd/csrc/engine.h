@@ -114,15 +114,16 @@ __host__ __device__ inline uint32_t hilbert_child_frame(uint32_t frame, uint32_t
 // box (hilbert: their positions along each node's Hilbert curve instead, so
 // that sorting orders the bodies along a Hilbert curve of the same tree), with
 // the instance index in the top ibits bits (ibits = 0: one instance, l1 = 32).
+template <bool HILBERT>
 __device__ __forceinline__ uint64_t path_key(float px, float py, float4 bb, int32_t inst, int l1,
-                                             int ibits, bool hilbert) {
+                                             int ibits) {
   float l = bb.x, r = bb.y, b = bb.z, t = bb.w;
   uint64_t k = 0;
   uint32_t frame = 0;
 #pragma unroll 4
   for (int lev = 0; lev < l1; ++lev) {
     uint32_t c = split2(px, py, l, r, b, t);
-    if (hilbert) c = hilbert_step(frame, c);
+    if (HILBERT) c = hilbert_step(frame, c);
     k = (k << 2) | c;
   }
   if (ibits > 0) k |= (uint64_t)inst << (64 - ibits);

@@ -161,11 +161,11 @@ __global__ __launch_bounds__(kSsTileThreads) void ss_count_kernel(const uint64_t
 // 1+2. keys of a tile (as the build's keys kernel computes them) and their
 // bucket totals, when the splitters are already known (every sort after the
 // first).  MAXB >= nb.
-template <int MAXB>
+template <int MAXB, bool HILBERT>
 __global__ __launch_bounds__(kSsTileThreads) void ss_keys_count_kernel(
     const float* __restrict__ x, const float* __restrict__ y, int32_t n,
     const int32_t* __restrict__ inst_of, const float4* __restrict__ ibbox, int l1, int ibits,
-    int hilbert, int32_t nb, int32_t tile, const uint64_t* __restrict__ spl, uint64_t* __restrict__ key,
+    int32_t nb, int32_t tile, const uint64_t* __restrict__ spl, uint64_t* __restrict__ key,
     uint32_t* __restrict__ perm, uint32_t* __restrict__ tot) {
   __shared__ uint64_t sp[MAXB];
   __shared__ uint32_t cnt[MAXB];
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kSsTileThreads) void ss_keys_count_kernel(
   const int64_t t1 = t0 + tile < n ? t0 + tile : n;
   for (int64_t e = t0 + threadIdx.x; e < t1; e += blockDim.x) {
     const int32_t inst = inst_of[e];
-    const uint64_t k = path_key(x[e], y[e], ibbox[inst], inst, l1, ibits, hilbert != 0);
+    const uint64_t k = path_key<HILBERT>(x[e], y[e], ibbox[inst], inst, l1, ibits);
     key[e] = k;
     perm[e] = (uint32_t)e;
     atomicAdd(&cnt[bucket_of(sp, nb, k)], 1u);
@@ -497,12 +497,14 @@ void launch_keys_count(stellar_ctx* c, int l1, int ibits, bool hilbert) {
   Buffers& B = c->b;
   const int32_t n = c->n;
   const int32_t nb = sample_sort_buckets(n);
-  auto k = nb <= kSsBucketsSmall ? ss_keys_count_kernel<kSsBucketsSmall>
-           : nb <= kSsBucketsMid ? ss_keys_count_kernel<kSsBucketsMid>
-                                 : ss_keys_count_kernel<kSsMaxBuckets>;
+  auto k = hilbert ? (nb <= kSsBucketsSmall ? ss_keys_count_kernel<kSsBucketsSmall, true>
+                      : nb <= kSsBucketsMid ? ss_keys_count_kernel<kSsBucketsMid, true>
+                                            : ss_keys_count_kernel<kSsMaxBuckets, true>)
+                   : (nb <= kSsBucketsSmall ? ss_keys_count_kernel<kSsBucketsSmall, false>
+                      : nb <= kSsBucketsMid ? ss_keys_count_kernel<kSsBucketsMid, false>
+                                            : ss_keys_count_kernel<kSsMaxBuckets, false>);
   k<<<sample_sort_tiles(n), kSsTileThreads, 0, c->stream>>>(B.x, B.y, n, B.inst_of, B.ibbox, l1,
-                                                            ibits, hilbert ? 1 : 0, nb,
-                                                            sample_sort_tile(n),
+                                                            ibits, nb, sample_sort_tile(n),
                                                             B.ss_spl, B.key, B.perm, B.ss_tot);
 }
 
