@@ -116,9 +116,9 @@ typedef struct stellar_params {
  * with STELLAR_FLAG_QUADRUPOLE or STELLAR_FLAG_BMAX_OPENING). */
 #define STELLAR_FLAG_REFERENCE_ROOT 64
 /* Store bodies and number cells along a Hilbert curve of the same tree (not
- * in the reference): the children of a node at a level above 32 (fewer with
- * ensembles: above 32 - ceil(log4 instances)) are visited along the node's
- * Hilbert curve, deeper children in slot order.  The tree -- its cells, each
+ * in the reference): the children at levels 1..16 are visited along their
+ * parent's Hilbert curve, deeper children in slot order (an instance's order
+ * is the same alone and in an ensemble).  The tree -- its cells, each
  * child's slot, counts, moments -- is the same; the walk groups, which are
  * consecutive bodies of the storage order, become more compact (fewer
  * interactions per body).  Exported trees stay in the reference's numbering

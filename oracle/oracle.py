@@ -307,9 +307,9 @@ def walk_recursive(tree: dict, x, y, mass, bb, theta: float, eps2: float, group:
     return ax, ay, it
 
 
-def hilbert_order(tree: dict, n: int, l1: int = 32) -> np.ndarray:
-    """Body ids in Hilbert order of a canonical tree (Hilbert children above
-    level l1, slot order below; or_hilbert_order)."""
+def hilbert_order(tree: dict, n: int, l1: int = 16) -> np.ndarray:
+    """Body ids in Hilbert order of a canonical tree (Hilbert children at
+    levels 1..l1, slot order below; or_hilbert_order)."""
     out = np.zeros(max(n, 1), np.int32)
     ch = tree["child"] if tree["child"].size else np.zeros(4, np.int32)
     rc = lib().or_hilbert_order(n, _i(tree["root"]), _i(ch), int(l1), _i(out))

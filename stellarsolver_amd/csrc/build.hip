@@ -146,8 +146,8 @@ struct KeyView {
   const uint32_t* key3;
   int l1;
   int ibits;
-  // key holds Hilbert curve positions instead of slot codes (levels 1..l1;
-  // key2 and key3 hold slot codes either way)
+  // key holds Hilbert curve positions instead of slot codes for levels
+  // 1..kHilbertLevels (<= l1); deeper levels are slot codes either way
   int hilbert;
 };
 
@@ -237,7 +237,7 @@ __device__ __forceinline__ int common_levels(const KeyView& K, int32_t s, int32_
 }
 
 // Slot codes of one tree position at consecutive levels.  With Hilbert keys
-// a level <= l1 holds the curve position h, read in the frame that the levels
+// a level <= kHilbertLevels holds the curve position h, read in the frame that the levels
 // above give it: the XOR of their frame changes {1, 0, 0, 3}, whose bit 0 is
 // the parity of the digits above with both bits equal (h = 0 or 3) and bit 1
 // the parity of those equal to 3 -- two popcounts of the key's prefix for the
@@ -250,7 +250,7 @@ struct SlotWalk {
 
 __device__ __forceinline__ SlotWalk slot_walk(const KeyView& K, int32_t s, int lev) {
   SlotWalk w{lev, 0u, K.key[s]};
-  if (K.hilbert && lev <= K.l1 && lev > 1) {
+  if (K.hilbert && lev <= kHilbertLevels && lev > 1) {
     constexpr uint64_t kEven = 0x5555555555555555ull;
     // digits of levels 1..lev-1 (the instance bits above them masked off)
     const uint64_t p = w.k >> (2 * (K.l1 - lev + 1));
@@ -266,14 +266,14 @@ __device__ __forceinline__ int slot_code(const KeyView& K, int32_t s, const Slot
   const int lev = w.lev;
   if (lev <= K.l1) {
     const uint32_t h = (uint32_t)(w.k >> (2 * (K.l1 - lev))) & 3u;
-    return (int)(K.hilbert ? hilbert_slot(w.frame, h) : h);
+    return (int)(K.hilbert && lev <= kHilbertLevels ? hilbert_slot(w.frame, h) : h);
   }
   if (lev <= K.l1 + 32) return (int)((K.key2[s] >> (2 * (K.l1 + 32 - lev))) & 3u);
   return (int)((K.key3[s] >> (30 - 2 * (lev - K.l1 - 33))) & 3u);
 }
 
 __device__ __forceinline__ void slot_next(const KeyView& K, SlotWalk& w) {
-  if (K.hilbert && w.lev <= K.l1)
+  if (K.hilbert && w.lev <= kHilbertLevels)
     w.frame = hilbert_child_frame(w.frame, (uint32_t)(w.k >> (2 * (K.l1 - w.lev))) & 3u);
   ++w.lev;
 }

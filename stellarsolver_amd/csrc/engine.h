@@ -109,10 +109,16 @@ __host__ __device__ inline uint32_t hilbert_slot(uint32_t frame, uint32_t h) {
 __host__ __device__ inline uint32_t hilbert_child_frame(uint32_t frame, uint32_t h) {
   return frame ^ ((kHilbertChild >> (2 * h)) & 3u);
 }
+// Levels whose children follow the Hilbert curve (deeper ones: slot order).
+// Every context's key holds at least 16 levels (l1 >= 16), so a tree's order
+// does not depend on the number of instances beside it; below the walk
+// groups' scale the order changes no group's shape.
+constexpr int kHilbertLevels = 16;
 
 // Sort key of one body: the slot codes of levels 1..l1 below the instance's
-// box (hilbert: their positions along each node's Hilbert curve instead, so
-// that sorting orders the bodies along a Hilbert curve of the same tree), with
+// box (HILBERT: those of levels 1..kHilbertLevels replaced by their positions
+// along each node's Hilbert curve, so that sorting orders the bodies along a
+// Hilbert curve of the same tree), with
 // the instance index in the top ibits bits (ibits = 0: one instance, l1 = 32).
 template <bool HILBERT>
 __device__ __forceinline__ uint64_t path_key(float px, float py, float4 bb, int32_t inst, int l1,
@@ -123,7 +129,7 @@ __device__ __forceinline__ uint64_t path_key(float px, float py, float4 bb, int3
 #pragma unroll 4
   for (int lev = 0; lev < l1; ++lev) {
     uint32_t c = split2(px, py, l, r, b, t);
-    if (HILBERT) c = hilbert_step(frame, c);
+    if (HILBERT && lev < kHilbertLevels) c = hilbert_step(frame, c);
     k = (k << 2) | c;
   }
   if (ibits > 0) k |= (uint64_t)inst << (64 - ibits);

@@ -172,8 +172,9 @@ COUNTS = [1000, 64, 5000, 1, 2, 333, 4096, 777, 65, 20000]
 @pytest.mark.parametrize("group", [64, 16])
 def test_hilbert_ensemble(S, oracle, group):
     """Ensembles: each instance's tree bit-exact, its walk groups consecutive
-    bodies of its own Hilbert order (Hilbert children above level
-    32 - instance bits / 2, slot order below)."""
+    bodies of its own Hilbert order (the same order as alone: Hilbert children
+    at levels 1..16, which every key holds), and its forces, counts and steps
+    equal to a single context's on the same bodies bit for bit."""
     sts = []
     for i, c in enumerate(COUNTS):
         sts.append(S.generate_ic(S.ic_params([0, 1, 2][i % 3], seed=5 + i), c))
@@ -190,14 +191,14 @@ def test_hilbert_ensemble(S, oracle, group):
         e.sort()
         e.forces()
         f = e.export_forces()
-    ibits = 4  # ten instances: 2 bits per level, 4 >= log2(10)
-    l1 = 32 - ibits // 2
+        e.run(3)
+        after = e.download()
     off = np.concatenate([[0], np.cumsum(COUNTS)])
     for i, st in enumerate(sts):
         print(f"hilbert ensemble instance {i} n={COUNTS[i]} group={group}", flush=True)
         obb, ot = oracle_tree(oracle, st)
         assert tree_equal(trees[i], ot), (i, first_tree_difference(trees[i], ot))
-        order = oracle.hilbert_order(ot, COUNTS[i], l1)
+        order = oracle.hilbert_order(ot, COUNTS[i])
         oracle.quad_exact(ot, st["x"], st["y"], st["mass"])
         ax, ay, it = oracle.walk(ot, st["x"], st["y"], st["mass"], obb, 0.5, eps2, group,
                                  opening=1, quad=True, order=order)
@@ -205,3 +206,48 @@ def test_hilbert_ensemble(S, oracle, group):
         assert np.array_equal(f["inter"][sl], it), i
         if COUNTS[i] > 1:  # a lone body feels no force
             assert rms_rel(f["ax"][sl], f["ay"][sl], ax, ay) < FORCE_RMS_TOL, i
+        s1 = S.default_params(eps2=eps2, group=group, flags=flags)
+        with S.Engine(COUNTS[i], s1) as one:
+            one.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
+            one.bbox()
+            one.build()
+            one.com()
+            one.sort()
+            one.forces()
+            fo = one.export_forces()
+            one.run(3)
+            ao = one.download()
+        for k in fo:
+            assert np.array_equal(f[k][sl], fo[k]), (i, k)
+        for k in ao:
+            assert np.array_equal(after[k][sl], ao[k]), (i, k)
+
+
+@pytest.mark.parametrize("ensemble", [False, True])
+def test_hilbert_graph_replay(S, ensemble):
+    """Graph-replayed steps in Hilbert order equal direct steps bit for bit
+    (replays interleaved with direct steps and downloads)."""
+    flags = S.FLAG_HILBERT_ORDER | S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING
+    if ensemble:
+        sts = [S.generate_ic(S.ic_params(i % 3, seed=40 + i), c) for i, c in enumerate(COUNTS)]
+        st = {k: np.concatenate([s[k] for s in sts]) for k in ("mass", "x", "y", "vx", "vy")}
+        counts = COUNTS
+    else:
+        st = ic(2, 65536)
+        counts = None
+    eps2 = float(np.float32(1e-4))
+    out = []
+    for graph in (False, True):
+        p = S.default_params(eps2=eps2, flags=flags | (S.FLAG_GRAPH if graph else 0))
+        e = S.Engine(params=p, counts=counts) if counts else S.Engine(st["x"].size, p)
+        with e:
+            e.upload(st["mass"], st["x"], st["y"], st["vx"], st["vy"])
+            got = []
+            for k in (2, 4, 1, 2):
+                print(f"hilbert graph={graph} ensemble={ensemble} run {k}", flush=True)
+                e.run(k)
+                got.append(e.download())
+            out.append(got)
+    for a, b in zip(*out):
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
