@@ -54,14 +54,14 @@ DEFAULT_KEY = (2, 1 << 20, 0.5, 64, "quad", "textbook", "hilbert")
 # (KiB per dispatch), the factors being what the same counters read for a 1 GiB
 # device copy; "forces" is the walk launch (with the update inside it).
 STAGE_TRAFFIC_BYTES = {DEFAULT_KEY: {
-    "bbox": (4125 * 2 + 8) * 1024,
-    "build": (90589 * 2 + 197550) * 1024,
+    "bbox": (4130 * 2 + 8) * 1024,
+    "build": (89755 * 2 + 193908) * 1024,
     "com": (32028 * 2 + 39696) * 1024,
-    "forces": (162230 * 2 + 27876) * 1024,
+    "forces": (161850 * 2 + 27876) * 1024,
 }}
 # The walk's VALU busy fraction: SQ_ACTIVE_INST_VALU / (256 CUs x GRBM_GUI_ACTIVE
-# per XCD) per dispatch, 413.9e6 / (256 x 15.70e6 / 8).
-WALK_VALU_BUSY = {DEFAULT_KEY: 413.9e6 / (256 * 15.70e6 / 8)}
+# per XCD) per dispatch, 413.9e6 / (256 x 15.62e6 / 8).
+WALK_VALU_BUSY = {DEFAULT_KEY: 413.9e6 / (256 * 15.62e6 / 8)}
 
 
 def progress(msg: str) -> None:
