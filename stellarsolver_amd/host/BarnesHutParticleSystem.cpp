@@ -18,7 +18,8 @@ stellar_params BarnesHutParticleSystem::contextParameters(const SimulationParame
   s.gravity = p.gravity;
   s.group = opt.group;
   s.flags = (opt.quadrupole ? STELLAR_FLAG_QUADRUPOLE : 0) |
-            (opt.textbook ? STELLAR_FLAG_TEXTBOOK_OPENING : 0);
+            (opt.textbook ? STELLAR_FLAG_TEXTBOOK_OPENING : 0) |
+            (opt.hilbert ? STELLAR_FLAG_HILBERT_ORDER : 0);
   return s;
 }
 

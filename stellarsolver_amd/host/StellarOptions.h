@@ -17,6 +17,7 @@ struct StellarOptions {
   uint64_t seed = 0;       // seed of a seeded workload when seed_given
   bool quadrupole = false; // add cell quadrupole terms
   bool textbook = false;   // opening test w/d <= theta on the unsoftened distance
+  bool hilbert = false;    // bodies and walk groups along a Hilbert curve of the tree
 };
 
 #endif

@@ -7,7 +7,7 @@
 // -dampening= (and -barnes-hut, which the README shows and which selects
 // nothing).  Added: -numbodies= (reference: 512*256), -theta=, -eps=,
 // -group=, -workload= (seeded workload 0, 1, 2 or 4), -seed=, -quad,
-// -textbook; -frames=DIR, -frame-every=K, -view=S, -resolution=R.
+// -textbook, -hilbert; -frames=DIR, -frame-every=K, -view=S, -resolution=R.
 //
 // -opengl: the reference opens a window and draws every body after every
 // step (StellarSolverVisualizer.cpp:208-253).  Headless here: every step is
@@ -101,6 +101,7 @@ int main(int argc, char** argv) {
   if (seed) options.seed = (uint64_t)std::strtoull(seed, nullptr, 10);
   options.quadrupole = checkCmdLineFlag(argc, argv, "quad");
   options.textbook = checkCmdLineFlag(argc, argv, "textbook");
+  options.hilbert = checkCmdLineFlag(argc, argv, "hilbert");
   const char* frames = cmdLineValue(argc, argv, "frames");
   if (frames) parameters.opengl = true;
   const int frameEvery = std::max(1, getCmdLineParameterInt(argc, argv, "frame-every", 1));

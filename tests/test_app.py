@@ -59,6 +59,18 @@ def test_reference_models_and_debug_checks(gpu, model):
     assert all("PASS" in ln for ln in res), res
 
 
+@pytest.mark.gpu
+def test_debug_checks_in_hilbert_order(gpu):
+    """-hilbert (bodies stored along a Hilbert curve of the tree) with the
+    quadrupole model: the exported tree is the reference's numbering, so the
+    reference's own checks pass on it as in slot order."""
+    p = run("-colliding-disks", "-debug", "-iterations=2", "-hilbert", "-quad", "-textbook")
+    assert p.returncode == 0, p.stderr
+    res = [ln for ln in p.stdout.splitlines() if "RESULT" in ln]
+    assert len(res) == 16
+    assert all("PASS" in ln for ln in res), res
+
+
 def _read_pgm(path):
     data = Path(path).read_bytes()
     head = data.split(b"\n", 3)
