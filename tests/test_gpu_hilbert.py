@@ -65,6 +65,7 @@ MODELS = {
     "bmax+quad_g16": ("BMAX_OPENING|QUADRUPOLE", 0, 16, 0.6, {"bmax": True, "quad": True}),
     "mixed": ("QUADRUPOLE|TEXTBOOK_OPENING", 1, 64, 0.5,
               {"opening": 1, "quad": True, "mixed": True}),
+    "reference_root": ("REFERENCE_ROOT", 0, 64, 0.5, {"ref_root": True}),
 }
 
 
@@ -251,3 +252,29 @@ def test_hilbert_graph_replay(S, ensemble):
     for a, b in zip(*out):
         for k in a:
             assert np.array_equal(a[k], b[k]), k
+
+
+def test_hilbert_headline_size(S, oracle):
+    """The bench default at its real size (N = 2^20 colliding disks, theta 0.5,
+    quadrupoles, textbook opening, Hilbert order): bit-exact tree and
+    interaction counts, forces within the rsqrt tolerance of the oracle's
+    walk, and inside the 1e-3 bound against the FP64 direct sum."""
+    n = 1 << 20
+    print(f"hilbert headline parity n={n}", flush=True)
+    st = ic(2, n)
+    eps2 = float(np.float32(1e-4))
+    flags = S.FLAG_HILBERT_ORDER | S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING
+    with make_engine(S, st, eps2, flags=flags) as e:
+        gbb, gt, f = gpu_tree_and_forces(e)
+        tg = np.sort(np.random.default_rng(2).choice(n, 8192, replace=False)).astype(np.int32)
+        dx, dy = e.direct_fp64(tg)
+    obb, ot = oracle_tree(oracle, st)
+    assert tree_equal(gt, ot), first_tree_difference(gt, ot)
+    oracle.quad_exact(ot, st["x"], st["y"], st["mass"])
+    ax, ay, it = oracle.walk(ot, st["x"], st["y"], st["mass"], obb, 0.5, eps2, 64, opening=1,
+                             quad=True, order=oracle.hilbert_order(ot, n))
+    assert np.array_equal(f["inter"], it)
+    assert rms_rel(f["ax"], f["ay"], ax, ay) < 1e-5
+    err = rms_rel(f["ax"][tg], f["ay"][tg], dx, dy)
+    print(f"rms relative error vs FP64 direct sum: {err:.3e}", flush=True)
+    assert err < 1e-3
