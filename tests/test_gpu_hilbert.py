@@ -278,3 +278,26 @@ def test_hilbert_headline_size(S, oracle):
     err = rms_rel(f["ax"][tg], f["ay"][tg], dx, dy)
     print(f"rms relative error vs FP64 direct sum: {err:.3e}", flush=True)
     assert err < 1e-3
+
+
+def test_hilbert_below_level_16(S, oracle):
+    """A tree deeper than the curve's 16 levels (deep Plummer core, N = 2^20,
+    cells to level 23): the children below level 16 follow the slot order on
+    both sides; tree bit-exact, counts equal to the oracle's."""
+    n = 1 << 20
+    st = ic(4, n)
+    eps2 = float(np.float32(EPS[4] ** 2))
+    flags = S.FLAG_HILBERT_ORDER | S.FLAG_QUADRUPOLE | S.FLAG_TEXTBOOK_OPENING
+    with make_engine(S, st, eps2, flags=flags) as e:
+        gbb, gt, f = gpu_tree_and_forces(e)
+    obb, ot = oracle_tree(oracle, st)
+    assert int(ot["level"][: ot["ncells"]].max()) > 16
+    assert tree_equal(gt, ot), first_tree_difference(gt, ot)
+    oracle.quad_exact(ot, st["x"], st["y"], st["mass"])
+    order = oracle.hilbert_order(ot, n)
+    _, _, it = oracle.walk(ot, st["x"], st["y"], st["mass"], obb, 0.5, eps2, 64, opening=1,
+                           quad=True, order=order)
+    assert np.array_equal(f["inter"], it)
+    # the curve to a deeper level is a different order: its groups differ
+    deeper = oracle.hilbert_order(ot, n, 32)
+    assert not np.array_equal(order, deeper)
